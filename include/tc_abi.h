@@ -1,0 +1,617 @@
+/*
+ * tc_abi.h -- C ABI of the MI355X tessellated-cluster frame path.
+ *
+ * Every record below mirrors, field for field and byte for byte, the
+ * host/device contract of nvpro-samples/vk_tessellated_clusters
+ * (shaders/shaderio_core.h, shaderio_scene.h, shaderio_building.h,
+ * shaderio.h).  Device addresses are uint64_t exactly as the reference's
+ * C++ side declares BUFFER_REF (shaderio_core.h:29).  Sizes and offsets are
+ * pinned with static asserts at the end of this file.
+ *
+ * Entry points take plain pointers: a host-side tc_bindings (the
+ * reference's single descriptor set, shaderio.h:48-58) whose members are
+ * device pointers, a tc_config by value (the reference's compile-time
+ * macros, renderer_raytrace_clusters_tess.cpp:117-138), and a HIP stream.
+ * Calls are asynchronous on the stream.  They return TC_OK or an argument /
+ * launch error; data-dependent capacity overflow is never an error return: as
+ * in the reference it clamps counters and drops work, visible through
+ * tc_readback (build_setup.comp.glsl:105-264).
+ */
+#ifndef TC_ABI_H
+#define TC_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef enum tc_status {
+  TC_OK = 0,
+  TC_ERROR_INVALID_ARGUMENT = 1,
+  TC_ERROR_LAUNCH = 2,
+  TC_ERROR_OUT_OF_MEMORY = 3,
+  TC_ERROR_NO_DEVICE = 4,
+  TC_ERROR_UNSUPPORTED = 5,
+  TC_ERROR_CAPACITY = 6
+} tc_status;
+
+/* HIP stream handle (hipStream_t), kept opaque so this header needs no HIP. */
+typedef void* tc_stream;
+
+/* --------------------------------------------------------------- constants */
+
+#define TC_TESSTABLE_COORD_MAX (1u << 15) /* shaderio_scene.h:25 */
+#define TC_TESSTABLE_SIZE 11u             /* shaderio_scene.h:26 */
+#define TC_TESSTABLE_LOOKUP_SIZE 16u      /* tessellation_table.cpp:39-40 */
+#define TC_TESSTABLE_NUM_LOOKUP (16u * 16u * 16u)
+
+#define TC_INSTANCE_FRUSTUM_BIT 1u /* shaderio_building.h:54 */
+#define TC_INSTANCE_VISIBLE_BIT 2u /* shaderio_building.h:55 */
+
+/* build_setup modes, shaderio.h:67-72 */
+#define TC_BUILD_SETUP_CLASSIFY 0
+#define TC_BUILD_SETUP_INSTANTIATE_TESS 1
+#define TC_BUILD_SETUP_DRAW_TESS 2
+#define TC_BUILD_SETUP_SPLIT 3
+#define TC_BUILD_SETUP_SPLIT_PASS 4
+#define TC_BUILD_SETUP_BUILD_BLAS 5
+
+/* cluster id mode bits, shaderio.h:93-99 */
+#define TC_RT_CLUSTER_MODE_FULL_CLUSTER 0u
+#define TC_RT_CLUSTER_MODE_SINGLE_TESSELLATED 1u
+#define TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER 2u
+#define TC_RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED 3u
+
+/* ------------------------------------------------------- records (L1 mirror) */
+
+/* shaderio_core.h:96-101 */
+typedef struct tc_dispatch_indirect {
+  uint32_t gridX, gridY, gridZ;
+} tc_dispatch_indirect;
+
+/* shaderio_core.h:103-107 */
+typedef struct tc_draw_mesh_tasks_indirect {
+  uint32_t count, first;
+} tc_draw_mesh_tasks_indirect;
+
+/* shaderio_scene.h:65-71 */
+typedef struct tc_tess_table_entry {
+  uint16_t firstTriangle;
+  uint16_t firstVertex;
+  uint16_t numTriangles;
+  uint16_t numVertices;
+} tc_tess_table_entry;
+
+/* shaderio_scene.h:75-87: device addresses of the table arrays */
+typedef struct tc_tessellation_table {
+  uint64_t vertices;                   /* u32[]: u | v << 16                   */
+  uint64_t triangles;                  /* u32[]: 3 x u8 CCW indices            */
+  uint64_t entries;                    /* tc_tess_table_entry[4096] lookup     */
+  uint64_t templateAddresses;          /* u64[4096] -> tc_clas_template        */
+  uint64_t templateInstantiationSizes; /* u32[4096] worst-case CLAS bytes      */
+} tc_tessellation_table;
+
+/* shaderio_scene.h:89-95 */
+typedef struct tc_bbox {
+  float lo[3];
+  float hi[3];
+  float shortestEdge;
+  float longestEdge;
+} tc_bbox;
+
+/* shaderio_scene.h:101-111 */
+typedef struct tc_cluster {
+  uint16_t numVertices;
+  uint16_t numTriangles;
+  uint32_t firstTriangle;
+  uint32_t firstLocalVertex;
+  uint32_t firstLocalTriangle;
+} tc_cluster;
+
+/* shaderio_scene.h:116-156 (column-major world matrix, glm layout) */
+typedef struct tc_render_instance {
+  float worldMatrix[16];
+  uint32_t geometryID;
+  uint32_t numTriangles;
+  uint32_t numVertices;
+  uint32_t numClusters;
+  int32_t displacementIndex;
+  float displacementScale;
+  float displacementOffset;
+  float _pad;
+  float geoLo[4];
+  float geoHi[4]; /* w = length of the geometry bbox diagonal */
+  uint64_t positions;                        /* float3[]                      */
+  uint64_t normals;                          /* float3[]                      */
+  uint64_t texcoords;                        /* float2[]                      */
+  uint64_t clusters;                         /* tc_cluster[]                  */
+  uint64_t clusterLocalTriangles;            /* u8[3 * T]                     */
+  uint64_t clusterBboxes;                    /* tc_bbox[]                     */
+  uint64_t clusterTemplateAdresses;          /* u64[] -> tc_clas_template     */
+  uint64_t clusterTemplateInstantiatonSizes; /* u32[] worst-case CLAS bytes   */
+} tc_render_instance;
+
+/* shaderio_building.h:62-66 */
+typedef struct tc_cluster_info {
+  uint32_t instanceID;
+  uint32_t clusterID;
+} tc_cluster_info;
+
+/* shaderio_building.h:71-82 */
+typedef struct tc_sub_triangle_info {
+  uint32_t vtxEncoded[3];     /* 2 x 16-bit barycentrics per corner     */
+  uint32_t triangleID_config; /* [15:0] triangle, [31:16] config+flip   */
+} tc_sub_triangle_info;
+
+/* shaderio_building.h:85-89 */
+typedef struct tc_tess_triangle_info {
+  tc_cluster_info cluster;
+  tc_sub_triangle_info subTriangle;
+} tc_tess_triangle_info;
+
+/* shaderio_building.h:130-137 */
+typedef struct tc_template_instantiate_info {
+  uint32_t clusterIdOffset;
+  uint32_t geometryIndexOffset;
+  uint64_t clusterTemplateAddress;
+  uint64_t vertexBufferAddress;
+  uint64_t vertexBufferStride;
+} tc_template_instantiate_info;
+
+/* shaderio_building.h:141-149 */
+typedef struct tc_blas_build_info {
+  uint32_t clusterReferencesCount;
+  uint32_t clusterReferencesStride;
+  uint64_t clusterReferences;
+} tc_blas_build_info;
+
+/* shaderio_building.h:156-263, every field in order */
+typedef struct tc_scene_building {
+  float viewPos[3];
+  uint32_t _pad;
+
+  uint32_t numRenderInstances;
+  uint32_t visibleClusterCounter;
+
+  uint32_t fullClusterCounter;
+  uint32_t partTriangleCounter;
+
+  uint64_t dualPartTriangleCounter;
+
+  int32_t splitTriangleCounter;
+  uint32_t splitReadCounter;
+  uint32_t splitWriteCounter;
+  uint32_t splitPass;
+  uint32_t splitPassStart;
+  uint32_t splitPassEnd;
+
+  uint32_t genVertexCounter;
+  uint32_t genClusterCounter;
+  uint64_t genClusterDataCounter;
+
+  tc_dispatch_indirect dispatchClassify;
+  tc_dispatch_indirect dispatchTriangleSplit;
+
+  uint64_t instanceStates;  /* u32[]                   */
+  uint64_t visibleClusters; /* tc_cluster_info[]       */
+  uint64_t fullClusters;    /* tc_cluster_info[]       */
+  uint64_t splitTriangles;  /* tc_tess_triangle_info[] */
+  uint64_t partTriangles;   /* tc_tess_triangle_info[] */
+
+  tc_draw_mesh_tasks_indirect drawFullClusters;
+  tc_draw_mesh_tasks_indirect drawPartTriangles;
+
+  tc_dispatch_indirect dispatchClusterInstantiate;
+  tc_dispatch_indirect dispatchTriangleInstantiate;
+  tc_dispatch_indirect dispatchBlasTempInsert;
+  tc_dispatch_indirect dispatchBlasTransInsert;
+
+  uint32_t positionTruncateBitCount;
+
+  uint32_t blasClusterCounter;
+  uint32_t tempInstantiateCounter;
+  uint32_t transBuildCounter;
+
+  uint64_t basicClusterSizes;
+
+  uint64_t genClusterData;
+  uint64_t genVertices; /* float3[] */
+
+  uint64_t tempInstanceIDs;      /* u32[]                          */
+  uint64_t tempInstantiations;   /* tc_template_instantiate_info[] */
+  uint64_t tempClusterAddresses; /* u64[]                          */
+  uint64_t tempClusterSizes;     /* u32[]                          */
+
+  uint64_t transInstanceIDs;
+  uint64_t transBuilds;
+  uint64_t transClusterAddresses;
+  uint64_t transClusterSizes;
+
+  uint64_t transTriMappings;
+  uint64_t transTriIndices;
+
+  uint64_t blasBuildInfos;       /* tc_blas_build_info[] per instance */
+  uint64_t blasBuildSizes;       /* u32[] per instance                */
+  uint64_t blasClusterAddresses; /* u64[] per-instance regions        */
+  uint64_t blasBuildData;
+
+  uint32_t numBlasReservedSizes;
+} tc_scene_building;
+
+/* shaderio.h:263-309 (C++ side layout) */
+typedef struct tc_readback {
+  uint32_t numVisibleClusters;
+  uint32_t numFullClusters;
+  uint32_t numSplitTriangles;
+  uint32_t numPartTriangles;
+  uint32_t numTotalTriangles;
+  uint32_t numTempInstantiations;
+  uint32_t numGenVertices;
+  uint32_t numBlasClusters;
+  uint32_t numTransBuilds;
+  uint32_t numTransPartTriangles;
+  uint32_t numActualTransBuilds;
+  uint32_t numActualTempInstantiations;
+  uint64_t numGenDatas;
+  uint64_t numGenActualDatas;
+  uint32_t numBlasReservedSizes;
+  uint32_t numBlasActualSizes;
+  uint64_t debugU64;
+  uint32_t clusterTriangleId;
+  uint32_t _packedDepth0;
+  uint32_t instanceId;
+  uint32_t _packedDepth1;
+  int32_t debugI;
+  uint32_t debugUI;
+  uint32_t debugF;
+  uint32_t debugA[64];
+  uint32_t debugB[64];
+  uint32_t debugC[64];
+} tc_readback;
+
+/* Sun direction of the reference's SkySimpleParameters (an nvpro_core2 type
+ * that is not part of the reference tree); the path reads only this member
+ * (render_raytrace_clusters.rchit.glsl:287-289). */
+typedef struct tc_sky_params {
+  float sunDirection[3];
+  float _pad;
+} tc_sky_params;
+
+/* shaderio.h:180-261; all matrices column-major (glm).  The trailing
+ * SkySimpleParameters is replaced by tc_sky_params. */
+typedef struct tc_frame_constants {
+  float projMatrix[16];
+  float projMatrixI[16];
+  float viewProjMatrix[16];
+  float viewProjMatrixI[16];
+  float viewMatrix[16];
+  float viewMatrixI[16];
+  float viewPos[4];
+  float viewDir[4];
+  float viewPlane[4];
+  float skyProjMatrixI[16];
+  int32_t viewport[2];
+  float viewportf[2];
+  float viewPixelSize[2];
+  float viewClipSize[2];
+  float wLightPos[3];
+  float tessRate;
+  float displacementScale;
+  float displacementOffset;
+  float lightMixer;
+  uint32_t doShadow;
+  float wUpDir[3];
+  float sceneSize;
+  float bgColor[4];
+  float lodScale;
+  float animationState;
+  float ambientOcclusionRadius;
+  int32_t ambientOcclusionSamples;
+  int32_t animationRippleEnabled;
+  float animationRippleFrequency;
+  float animationRippleAmplitude;
+  float animationRippleSpeed;
+  uint32_t _pad[3];
+  uint32_t visualize;
+  uint32_t doAnimation;
+  uint32_t flipWinding;
+  float nearPlane;
+  float farPlane;
+  float hizSizeFactors[4];
+  float nearSizeFactors[4];
+  float hizSizeMax;
+  int32_t facetShading;
+  int32_t supersample;
+  uint32_t colorXor;
+  uint32_t dbgUint;
+  float dbgFloat;
+  float time;
+  uint32_t frame;
+  uint32_t mousePosition[2];
+  float wireThickness;
+  float wireSmoothing;
+  float wireColor[3];
+  uint32_t wireStipple;
+  float wireBackfaceColor[3];
+  float wireStippleRepeats;
+  float wireStippleLength;
+  uint32_t doWireframe;
+  uint32_t visFilterInstanceID;
+  uint32_t visFilterClusterID;
+  tc_sky_params skyParams;
+} tc_frame_constants;
+
+/* ------------------------------------------------ inputs without a Vulkan type */
+
+/* Stands for one entry of `sampler2D displacementTextures[]`
+ * (cluster_classify.comp.glsl:122).  Sampled with FP32 bilinear weights,
+ * REPEAT addressing, texel centres at (i + 0.5) / size. */
+#define TC_TEXTURE_R16_UNORM 0u
+#define TC_TEXTURE_R32_FLOAT 1u
+typedef struct tc_displacement_texture {
+  uint64_t texels; /* device address, row-major, width * height texels */
+  uint32_t width;
+  uint32_t height;
+  uint32_t format; /* TC_TEXTURE_* */
+  uint32_t _pad;
+} tc_displacement_texture;
+
+/* ----------------------------------------------- own cluster acceleration data */
+
+/* Replaces an NV cluster template (tessellation_table.cpp:102-404,
+ * raytracing_cluster_data.cpp:29-280): the triangle order in which a
+ * cluster's triangles are grouped for its small BVH.  `triangles` holds
+ * numTriangles packed words v0 | v1 << 8 | v2 << 16 | primitiveID << 24,
+ * already in group order; groups are runs of TC_CLAS_GROUP_TRIANGLES. */
+#define TC_CLAS_GROUP_TRIANGLES 4u
+typedef struct tc_clas_template {
+  uint32_t numTriangles;
+  uint32_t numVertices;
+  uint32_t numGroups;
+  uint32_t clusterID; /* added to clusterIdOffset when instantiated */
+  uint64_t triangles;
+  uint64_t _pad;
+} tc_clas_template;
+
+/* Header of one instantiated cluster (CLAS) at tempClusterAddresses[i]:
+ *   tc_clas_header | groups: numGroups x tc_aabb | tris: u32[numTriangles]
+ *   | vertices: float3[numVertices]   (sizes rounded to 16 bytes) */
+typedef struct tc_aabb {
+  float lo[3];
+  float hi[3];
+} tc_aabb;
+
+typedef struct tc_clas_header {
+  float lo[3];
+  uint32_t clusterID; /* mode << 30 | id, the value of gl_ClusterIDNV */
+  float hi[3];
+  uint32_t counts; /* numTriangles | numVertices << 8 | numGroups << 16 */
+} tc_clas_header;
+
+/* Two-child BVH node used by the per-instance BLAS and the TLAS.  A child
+ * index with the top bit set is a leaf: bits [30:0] index the leaf list. */
+typedef struct tc_bvh_node {
+  float lo0[3];
+  uint32_t child0;
+  float hi0[3];
+  uint32_t child1;
+  float lo1[3];
+  uint32_t _pad0;
+  float hi1[3];
+  uint32_t _pad1;
+} tc_bvh_node;
+#define TC_BVH_LEAF 0x80000000u
+
+/* One TLAS instance (replaces VkAccelerationStructureInstanceKHR,
+ * renderer.cpp:240-266). */
+typedef struct tc_tlas_instance {
+  float objectToWorld[12]; /* 3x4 row-major */
+  float worldToObject[12]; /* 3x4 row-major */
+  uint64_t blasNodes;      /* tc_bvh_node[] of this instance's BLAS       */
+  uint64_t blasLeaves;     /* u64[] CLAS addresses (clusterReferences)    */
+  uint32_t blasRoot;       /* root child word (leaf flag if single CLAS)  */
+  uint32_t numClusters;
+  uint32_t flags; /* bit 0: force opaque, bit 1: front CCW flip */
+  uint32_t _pad;
+} tc_tlas_instance;
+
+/* Ray and hit records of the traversal kernel. */
+typedef struct tc_ray {
+  float origin[3];
+  float tMin;
+  float direction[3];
+  float tMax;
+} tc_ray;
+
+typedef struct tc_hit {
+  float t;              /* < 0 : miss                                    */
+  float u, v;           /* hit barycentrics (gl_HitAttribute)            */
+  uint32_t instanceID;  /* gl_InstanceID                                 */
+  uint32_t clusterID;   /* gl_ClusterIDNV incl. mode bits                */
+  uint32_t primitiveID; /* gl_PrimitiveID within the CLAS                */
+} tc_hit;
+
+/* Closest-hit decode output (render_raytrace_clusters.rchit.glsl:128-236,
+ * 338-339). */
+typedef struct tc_hit_decoded {
+  uint32_t instanceID;
+  uint32_t clusterID;     /* base cluster of the instance's geometry  */
+  uint32_t triangleID;    /* base triangle within that cluster        */
+  uint32_t subTriangleID; /* micro triangle within the tessellation   */
+  float baryBase[3];      /* barycentrics on the base triangle        */
+  float depth;            /* projPos.z / projPos.w, 1.0 on miss       */
+  float wPos[3];
+  uint32_t shadowed;      /* 1 if the shadow ray hit something        */
+} tc_hit_decoded;
+
+/* --------------------------------------------------------- configuration */
+
+#define TC_FLAG_DISPLACEMENT (1u << 0)     /* HAS_DISPLACEMENT_TEXTURES */
+#define TC_FLAG_CULLING (1u << 1)          /* DO_CULLING               */
+#define TC_FLAG_ANIMATION (1u << 2)        /* DO_ANIMATION             */
+#define TC_FLAG_PN (1u << 3)               /* TESS_USE_PN              */
+#define TC_FLAG_1X_TRANSIENT (1u << 4)     /* TESS_USE_1X_TRANSIENTBUILDS */
+#define TC_FLAG_2X_TRANSIENT (1u << 5)     /* TESS_USE_2X_TRANSIENTBUILDS */
+#define TC_FLAG_PERSISTENT_SPLIT (1u << 6) /* TESS_USE_PERSISTENT_KERNEL */
+
+/* The reference's specialisation macros
+ * (renderer_raytrace_clusters_tess.cpp:117-138). */
+typedef struct tc_config {
+  uint32_t flags;
+  uint32_t maxVisibleClusters;      /* MAX_VISIBLE_CLUSTERS        */
+  uint32_t maxPartTriangles;        /* MAX_PART_TRIANGLES          */
+  uint32_t maxSplitTriangles;       /* MAX_SPLIT_TRIANGLES         */
+  uint32_t maxGeneratedClusters;    /* MAX_GENERATED_CLUSTERS      */
+  uint32_t maxGeneratedClusterMegs; /* MAX_GENERATED_CLUSTER_MEGS  */
+  uint32_t maxGeneratedVertices;    /* MAX_GENERATED_VERTICES      */
+  uint32_t clusterVertexCount;      /* CLUSTER_VERTEX_COUNT  (<=255) */
+  uint32_t clusterTriangleCount;    /* CLUSTER_TRIANGLE_COUNT (<=255) */
+  uint32_t maxSplitFactor;          /* TESS_MAX_SPLIT_FACTOR       */
+} tc_config;
+
+/* The reference's descriptor set (shaderio.h:48-58) as device pointers,
+ * plus the caller-owned workspace that replaces dispatch-indirect state. */
+typedef struct tc_bindings {
+  const tc_frame_constants* view;                      /* FRAME_UBO        */
+  const tc_tessellation_table* tessTable;              /* TESSTABLE_UBO    */
+  tc_readback* readback;                               /* READBACK_SSBO    */
+  const tc_render_instance* instances;                 /* RENDERINSTANCES  */
+  tc_scene_building* build;                            /* SCENEBUILDING    */
+  const tc_displacement_texture* displacementTextures; /* DISPLACED_TEXTURES */
+  uint32_t numRenderInstances;
+  uint32_t numDisplacementTextures;
+  void* workspace;         /* tc_workspace_size() bytes, device memory */
+  uint64_t workspaceBytes;
+} tc_bindings;
+
+/* ------------------------------------------------------------ entry points */
+
+/* Device count / basic probe.  Returns TC_ERROR_NO_DEVICE without a GPU. */
+tc_status tc_device_check(int* deviceCount);
+
+/* Bytes of workspace the stage kernels need for `config`. */
+uint64_t tc_workspace_size(const tc_config* config);
+
+/* One-time workspace initialisation (clears tile status words). */
+tc_status tc_workspace_init(const tc_bindings* bindings, const tc_config* config, tc_stream stream);
+
+/* Uploads the built-in table: vertices, triangles, the 16^3 lookup
+ * (tessellation_table.cpp:36-100) and per-pattern templates with their
+ * instantiation sizes.  Allocates device memory owned by the library until
+ * tc_table_free.  *tableDev receives a device pointer to a
+ * tc_tessellation_table. */
+tc_status tc_table_upload(tc_tessellation_table** tableDev);
+tc_status tc_table_free(tc_tessellation_table* tableDev);
+
+/* Host copy of the expanded 16^3 lookup (4096 entries) for inspection. */
+tc_status tc_table_lookup_host(tc_tess_table_entry* out4096);
+
+/* Host copy of the per-pattern templates: out4096[i].triangles holds the
+ * word offset of lookup entry i's triangle words in `words` (*numWords
+ * words; pass words = NULL to query the count), sizes4096 the CLAS sizes. */
+tc_status tc_table_templates_host(tc_clas_template* out4096, uint32_t* sizes4096, uint32_t* words, uint32_t* numWords);
+
+/* Synchronous copy between any host / device pointers (hipMemcpyDefault). */
+tc_status tc_memcpy(void* dst, const void* src, uint64_t bytes);
+tc_status tc_device_synchronize(void);
+
+/* Worst-case CLAS bytes of a cluster with the given counts, the quantity the
+ * reference takes from its template builds (raytracing_cluster_data.cpp:362-388). */
+uint32_t tc_clas_size(uint32_t numTriangles, uint32_t numVertices);
+
+/* Builds a per-cluster template in host memory: writes the packed triangle
+ * words in group order into trianglesOut (numTriangles u32).
+ * localTriangles: 3 x u8 per triangle; positions: float3 per vertex. */
+tc_status tc_cluster_template_order(uint32_t numTriangles, uint32_t numVertices, const uint8_t* localTriangles,
+                                    const float* positions, uint32_t* trianglesOut);
+
+/* instances_classify.comp.glsl:100-127: per-instance frustum state bits
+ * (no HiZ pyramid exists here, so the occlusion test passes). */
+tc_status tc_instances_classify(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* clusters_cull.comp.glsl:112-161 in ray-tracing mode: appends every
+ * cluster of every instance, instance by instance, to build.visibleClusters. */
+tc_status tc_clusters_cull(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* build_setup.comp.glsl:102-264, one device thread. */
+tc_status tc_build_setup(const tc_bindings* b, tc_config config, int mode, tc_stream stream);
+
+/* cluster_classify.comp.glsl:150-907 over build.visibleClusters. */
+tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* triangle_split.comp.glsl:598-632, one pass over
+ * [build.splitPassStart, build.splitPassEnd). */
+tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* triangle_tess_template_instantiate.comp.glsl:126-373 over build.partTriangles. */
+tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* blas_setup_insertion.comp.glsl:100-116 (exclusive scan over instances). */
+tc_status tc_blas_setup_insertion(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* blas_clusters_insert.comp.glsl:97-135; transient = push constant 1. */
+tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int transient, tc_stream stream);
+
+/* Runs the reference's per-frame chain classify .. insert
+ * (renderer_raytrace_clusters_tess.cpp:410-686) with numSplitPasses split
+ * passes (0: derive from maxSplitFactor as the reference does). */
+tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t numSplitPasses, tc_stream stream);
+
+/* Our CLAS build (replaces the NV template instantiation,
+ * renderer_raytrace_clusters_tess.cpp:595-623): one block per
+ * tempInstantiations entry at tempClusterAddresses[i]; writes
+ * tempClusterSizes[i]. */
+tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* ---- frame driver (renderer_raytrace_clusters_tess.cpp:166-310, 410-780) ---- */
+
+typedef struct tc_renderer tc_renderer;
+
+/* Allocates every per-frame buffer for `config` (one device arena), copies
+ * the instance and texture records (whose addresses are device memory the
+ * caller keeps alive) and uploads the tessellation table. */
+tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t numInstances,
+                             const tc_displacement_texture* texturesHost, uint32_t numTextures, const tc_config* config,
+                             tc_renderer** out);
+tc_status tc_renderer_destroy(tc_renderer* renderer);
+/* Device pointers of the renderer's bindings, for the stage entry points. */
+tc_status tc_renderer_get_bindings(tc_renderer* renderer, tc_bindings* out);
+/* Initial per-frame SceneBuilding (device addresses of all buffers). */
+tc_status tc_renderer_get_building(tc_renderer* renderer, tc_scene_building* out);
+/* Uploads frame constants, resets SceneBuilding / Readback, fills the split
+ * list with ~0 (renderer_raytrace_clusters_tess.cpp:412-419). */
+tc_status tc_renderer_begin_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
+/* begin_frame, instances_classify, clusters_cull and tc_frame_tessellate. */
+tc_status tc_renderer_tessellate(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
+
+/* Number of split passes the reference records for a split factor
+ * (renderer_raytrace_clusters_tess.cpp:516-543). */
+uint32_t tc_num_split_passes(uint32_t splitFactor);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+static_assert(sizeof(tc_cluster_info) == 8, "ClusterInfo_size");            /* shaderio_building.h:68  */
+static_assert(sizeof(tc_tess_triangle_info) == 24, "TessTriangleInfo_size"); /* shaderio_building.h:92  */
+static_assert(sizeof(tc_template_instantiate_info) == 32, "TemplateInstantiateInfo");
+static_assert(sizeof(tc_blas_build_info) == 16, "BlasBuildInfo");
+static_assert(sizeof(tc_tess_table_entry) == 8, "TessTableEntry");
+static_assert(sizeof(tc_cluster) == 16, "Cluster");
+static_assert(sizeof(tc_bbox) == 32, "BBox");
+static_assert(sizeof(tc_render_instance) == 192, "RenderInstance");
+static_assert(offsetof(tc_scene_building, genClusterDataCounter) == 72, "SceneBuilding layout");
+static_assert(offsetof(tc_scene_building, instanceStates) == 104, "SceneBuilding layout");
+static_assert(offsetof(tc_scene_building, positionTruncateBitCount) == 208, "SceneBuilding layout");
+static_assert(offsetof(tc_scene_building, numBlasReservedSizes) == 360, "SceneBuilding layout");
+static_assert(sizeof(tc_scene_building) == 368, "SceneBuilding");
+static_assert(offsetof(tc_readback, numGenDatas) == 48, "Readback layout");
+static_assert(sizeof(tc_clas_header) == 32, "clas header");
+static_assert(sizeof(tc_bvh_node) == 64, "bvh node");
+static_assert(sizeof(tc_clas_template) == 32, "clas template");
+#endif
+
+#endif /* TC_ABI_H */

@@ -1,0 +1,955 @@
+/*
+ * tc_oracle.cpp -- serial CPU restatement of the reference's frame stages.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tc_oracle.h).  Each function cites the GLSL
+ * or C++ lines of nvpro-samples/vk_tessellated_clusters it restates.
+ */
+#include "tc_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+namespace {
+
+#include "../vk_tessellated_clusters_amd/csrc/tess_table_nv_data.inc"
+
+/* ------------------------------------------------------------ small math */
+
+struct v3 {
+  float x, y, z;
+};
+struct v2 {
+  float x, y;
+};
+
+inline v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline v3 mul(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+inline float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline float length(v3 a) { return sqrtf(dot(a, a)); }
+inline float distance(v3 a, v3 b) { return length(sub(a, b)); }
+inline v3 normalize(v3 a) { return mul(a, 1.0f / length(a)); }
+
+/* glm column-major mat4 * vec4(p, 1), summed column by column */
+inline v3 xform_point(const float* m, v3 p) {
+  v3 r;
+  r.x = ((m[0] * p.x + m[4] * p.y) + m[8] * p.z) + m[12];
+  r.y = ((m[1] * p.x + m[5] * p.y) + m[9] * p.z) + m[13];
+  r.z = ((m[2] * p.x + m[6] * p.y) + m[10] * p.z) + m[14];
+  return r;
+}
+
+/* tessellation.glsl:191-201 (precise) */
+inline v3 interp(const v3 b[3], v3 w) {
+  return mk((b[0].x * w.x + b[1].x * w.y) + b[2].x * w.z, (b[0].y * w.x + b[1].y * w.y) + b[2].y * w.z,
+            (b[0].z * w.x + b[1].z * w.y) + b[2].z * w.z);
+}
+inline v2 interp2(const v2 b[3], v3 w) {
+  return v2{(b[0].x * w.x + b[1].x * w.y) + b[2].x * w.z, (b[0].y * w.x + b[1].y * w.y) + b[2].y * w.z};
+}
+
+template <class T>
+inline T* P(uint64_t a) {
+  return reinterpret_cast<T*>(static_cast<uintptr_t>(a));
+}
+
+inline v3 ld3(uint64_t base, uint32_t i) {
+  const float* f = P<const float>(base) + 3ull * i;
+  return mk(f[0], f[1], f[2]);
+}
+inline v2 ld2(uint64_t base, uint32_t i) {
+  const float* f = P<const float>(base) + 2ull * i;
+  return v2{f[0], f[1]};
+}
+
+inline uint32_t amax3(uint32_t a, uint32_t b, uint32_t c) { return std::max(std::max(a, b), c); }
+
+template <class T>
+inline T fadd(T* p, T v) {
+  return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+}
+
+/* --------------------------------------------- table + 16^3 lookup */
+
+struct Table {
+  tc_tess_table_entry lookup[TC_TESSTABLE_NUM_LOOKUP];
+  Table() {
+    memset(lookup, 0, sizeof(lookup));
+    /* tessellation_table.cpp:53-80: raw order x >= y >= z, (x,z,y) aliases */
+    uint32_t idx = 0;
+    for (uint32_t x = 1; x <= TC_TABLE_MAX_EDGE_SEGMENTS; x++)
+      for (uint32_t y = 1; y <= x; y++)
+        for (uint32_t z = 1; z <= y; z++, idx++) {
+          tc_tess_table_entry e;
+          e.firstTriangle = tc_table_nv_configs[idx * 4 + 0];
+          e.firstVertex = tc_table_nv_configs[idx * 4 + 1];
+          e.numTriangles = tc_table_nv_configs[idx * 4 + 2];
+          e.numVertices = tc_table_nv_configs[idx * 4 + 3];
+          lookup[lookup_index(x, y, z)] = e;
+          if (z != y && x > 1) lookup[lookup_index(x, z, y)] = e;
+        }
+  }
+  /* tessellation_table.hpp:58-62 */
+  static uint32_t lookup_index(uint32_t x, uint32_t y, uint32_t z) { return x + y * 16 + z * 256 - (1 + 16 + 256); }
+};
+
+const Table& table() {
+  static Table t;
+  return t;
+}
+
+/* ------------------------------------------------ tessellation.glsl */
+
+/* tessellation.glsl:48-59 */
+uint32_t encode_bary(v3 wuv) {
+  uint32_t ix = (uint32_t)(wuv.x * 32768.0f + 0.5f);
+  uint32_t iy = (uint32_t)(wuv.y * 32768.0f + 0.5f);
+  uint32_t iz = (uint32_t)(wuv.z * 32768.0f + 0.5f);
+  if (ix > std::max(iy, iz))
+    ix = 32768u - iy - iz;
+  else if (iy > iz)
+    iy = 32768u - ix - iz;
+  else
+    iz = 32768u - ix - iy;
+  return iy | (iz << 16);
+}
+
+/* tessellation.glsl:66-76 */
+v3 decode_bary(uint32_t vtx) {
+  v3 w;
+  w.y = float(vtx & 0xFFFF) / 32768.0f;
+  w.z = float(vtx >> 16) / 32768.0f;
+  w.x = (1.0f - w.y) - w.z;
+  return w;
+}
+
+/* tessellation.glsl:78-99 */
+v3 tess_factors(v3 a, v3 b, v3 c, v3 eye, float nearPlane, float vpy, float rate) {
+  float dA = distance(a, eye), dB = distance(b, eye), dC = distance(c, eye);
+  v3 spl;
+  spl.x = 1.0f / std::max(nearPlane, std::min(dA, dB));
+  spl.y = 1.0f / std::max(nearPlane, std::min(dB, dC));
+  spl.z = 1.0f / std::max(nearPlane, std::min(dC, dA));
+  v3 len = mk(distance(a, b), distance(b, c), distance(c, a));
+  v3 f;
+  f.x = roundf(((len.x * spl.x) * vpy) * rate);
+  f.y = roundf(((len.y * spl.y) * vpy) * rate);
+  f.z = roundf(((len.z * spl.z) * vpy) * rate);
+  f.x = std::min(std::max(f.x, 1.0f), 32768.0f);
+  f.y = std::min(std::max(f.y, 1.0f), 32768.0f);
+  f.z = std::min(std::max(f.z, 1.0f), 32768.0f);
+  return f;
+}
+
+/* tessellation.glsl:101-104 */
+inline uint32_t split_factor(uint32_t f, uint32_t maxSplit) { return std::min((f + 11 - 1) / 11, maxSplit); }
+
+/* tessellation.glsl:119-144 */
+uint32_t get_config(uint32_t f[3], uint32_t vtx[3]) {
+  uint32_t m = amax3(f[0], f[1], f[2]);
+  if (m == f[1]) {
+    uint32_t t0 = f[0], t1 = f[1], t2 = f[2];
+    f[0] = t1, f[1] = t2, f[2] = t0;
+    uint32_t v0 = vtx[0], v1 = vtx[1], v2 = vtx[2];
+    vtx[0] = v1, vtx[1] = v2, vtx[2] = v0;
+  } else if (m == f[2]) {
+    uint32_t t0 = f[0], t1 = f[1], t2 = f[2];
+    f[0] = t2, f[1] = t0, f[2] = t1;
+    uint32_t v0 = vtx[0], v1 = vtx[1], v2 = vtx[2];
+    vtx[0] = v2, vtx[1] = v0, vtx[2] = v1;
+  }
+  uint32_t idx = f[0] + f[1] * 16 + f[2] * 256 - (1 + 16 + 256);
+  if (f[2] > f[1]) idx |= 1u << 15;
+  return idx;
+}
+
+inline uint32_t cfg_index(uint32_t cfg) { return cfg & ~(1u << 15); }
+inline bool cfg_flipped(uint32_t cfg) { return (cfg & (1u << 15)) != 0; }
+
+/* tessellation.glsl:151-163 */
+inline const tc_tess_table_entry& cfg_entry(uint32_t cfg) { return table().lookup[cfg_index(cfg)]; }
+
+/* tessellation.glsl:165-176 */
+void cfg_tri_vertices(uint32_t cfg, uint32_t tri, uint32_t out[3]) {
+  uint32_t packed = tc_table_nv_triangles[cfg_entry(cfg).firstTriangle + tri];
+  out[0] = packed & 0xFF;
+  out[1] = (packed >> 8) & 0xFF;
+  out[2] = (packed >> 16) & 0xFF;
+  if (cfg_flipped(cfg)) std::swap(out[1], out[2]);
+}
+
+/* tessellation.glsl:178-189 */
+v3 cfg_vertex_bary(uint32_t cfg, uint32_t vert) {
+  v3 w = decode_bary(tc_table_nv_vertices[cfg_entry(cfg).firstVertex + vert]);
+  if (cfg_flipped(cfg)) std::swap(w.x, w.y);
+  return w;
+}
+
+/* ------------------------------------------------------ displacement */
+
+inline float texel(const tc_displacement_texture& t, uint32_t x, uint32_t y) {
+  uint64_t i = (uint64_t)y * t.width + x;
+  if (t.format == TC_TEXTURE_R16_UNORM) return float(P<const uint16_t>(t.texels)[i]) / 65535.0f;
+  return P<const float>(t.texels)[i];
+}
+
+inline int32_t wrap(int32_t i, int32_t n) {
+  int32_t r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+/* Manual FP32 bilinear, REPEAT addressing (stands for texture() with the
+ * linear sampler, cluster_classify.comp.glsl:477, resources.cpp:128). */
+float sample_height(const tc_displacement_texture& t, float u, float v) {
+  float x = u * float(t.width) - 0.5f;
+  float y = v * float(t.height) - 0.5f;
+  x = std::min(std::max(x, -16777216.0f), 16777216.0f);
+  y = std::min(std::max(y, -16777216.0f), 16777216.0f);
+  float x0 = floorf(x), y0 = floorf(y);
+  float fx = x - x0, fy = y - y0;
+  int32_t ix = (int32_t)x0, iy = (int32_t)y0;
+  int32_t W = (int32_t)t.width, H = (int32_t)t.height;
+  uint32_t xa = (uint32_t)wrap(ix, W), xb = (uint32_t)wrap(ix + 1, W);
+  uint32_t ya = (uint32_t)wrap(iy, H), yb = (uint32_t)wrap(iy + 1, H);
+  float t00 = texel(t, xa, ya), t10 = texel(t, xb, ya);
+  float t01 = texel(t, xa, yb), t11 = texel(t, xb, yb);
+  float top = t00 * (1.0f - fx) + t10 * fx;
+  float bot = t01 * (1.0f - fx) + t11 * fx;
+  return top * (1.0f - fy) + bot * fy;
+}
+
+/* displacement.glsl:30-104: PN triangle; pow(u,3) written as u*u*u */
+struct PN {
+  v3 b030, b003, b300, b021, b012, b102, b201, b210, b120, b111;
+};
+
+inline v3 project_to_plane(v3 p, v3 plane, v3 n) {
+  v3 d = sub(p, plane);
+  return sub(p, mul(n, dot(d, n)));
+}
+
+void setup_pn(PN& b, const v3 vt[3], const v3 n[3]) {
+  b.b030 = vt[0];
+  b.b003 = vt[1];
+  b.b300 = vt[2];
+  v3 e300 = sub(b.b003, b.b030);
+  v3 e030 = sub(b.b300, b.b003);
+  v3 e003 = sub(b.b030, b.b300);
+  b.b021 = add(b.b030, mk(e300.x / 3.0f, e300.y / 3.0f, e300.z / 3.0f));
+  b.b012 = add(b.b030, mk((e300.x * 2.0f) / 3.0f, (e300.y * 2.0f) / 3.0f, (e300.z * 2.0f) / 3.0f));
+  b.b102 = add(b.b003, mk(e030.x / 3.0f, e030.y / 3.0f, e030.z / 3.0f));
+  b.b201 = add(b.b003, mk((e030.x * 2.0f) / 3.0f, (e030.y * 2.0f) / 3.0f, (e030.z * 2.0f) / 3.0f));
+  b.b210 = add(b.b300, mk(e003.x / 3.0f, e003.y / 3.0f, e003.z / 3.0f));
+  b.b120 = add(b.b300, mk((e003.x * 2.0f) / 3.0f, (e003.y * 2.0f) / 3.0f, (e003.z * 2.0f) / 3.0f));
+  b.b021 = project_to_plane(b.b021, b.b030, n[0]);
+  b.b012 = project_to_plane(b.b012, b.b003, n[1]);
+  b.b102 = project_to_plane(b.b102, b.b003, n[1]);
+  b.b201 = project_to_plane(b.b201, b.b300, n[2]);
+  b.b210 = project_to_plane(b.b210, b.b300, n[2]);
+  b.b120 = project_to_plane(b.b120, b.b030, n[0]);
+  v3 s3 = add(add(b.b003, b.b030), b.b300);
+  v3 center = mk(s3.x / 3.0f, s3.y / 3.0f, s3.z / 3.0f);
+  v3 s6 = add(add(add(add(add(b.b021, b.b012), b.b102), b.b201), b.b210), b.b120);
+  b.b111 = mk(s6.x / 6.0f, s6.y / 6.0f, s6.z / 6.0f);
+  v3 d = sub(b.b111, center);
+  b.b111 = add(b.b111, mk(d.x / 2.0f, d.y / 2.0f, d.z / 2.0f));
+}
+
+v3 get_pn(const PN& b, v3 bary) {
+  float u = bary.x, v = bary.y, w = bary.z;
+  float u3 = u * u * u, v3_ = v * v * v, w3 = w * w * w;
+  float u2 = u * u, v2_ = v * v, w2 = w * w;
+  v3 r = mul(b.b300, w3);
+  r = add(r, mul(b.b030, u3));
+  r = add(r, mul(b.b003, v3_));
+  r = add(r, mul(b.b210, (3.0f * w2) * u));
+  r = add(r, mul(b.b120, (3.0f * w) * u2));
+  r = add(r, mul(b.b201, (3.0f * w2) * v));
+  r = add(r, mul(b.b021, (3.0f * u2) * v));
+  r = add(r, mul(b.b102, (3.0f * w) * v2_));
+  r = add(r, mul(b.b012, (3.0f * u) * v2_));
+  r = add(r, mul(b.b111, ((6.0f * w) * u) * v));
+  return r;
+}
+
+/* height -> offset along the normal, cluster_classify.comp.glsl:472-480 */
+inline v3 displace(const oracle_ctx* c, const tc_render_instance& inst, v3 pos, v3 nrm, v2 uv) {
+  const tc_displacement_texture& tex = c->displacementTextures[inst.displacementIndex];
+  float h = sample_height(tex, uv.x, uv.y);
+  h = (((h * inst.displacementScale) * c->view->displacementScale) + inst.displacementOffset) + c->view->displacementOffset;
+  return add(pos, mul(normalize(nrm), h));
+}
+
+inline bool has_disp(const oracle_ctx* c, tc_config cfg, const tc_render_instance& inst) {
+  return (cfg.flags & TC_FLAG_DISPLACEMENT) && inst.displacementIndex >= 0 &&
+         (uint32_t)inst.displacementIndex < c->numDisplacementTextures;
+}
+
+inline uint64_t max_data_bytes(tc_config cfg) { return uint64_t(cfg.maxGeneratedClusterMegs) * 1024ull * 1024ull; }
+
+inline uint64_t gen_vertices_value(const oracle_ctx* c) {
+  return c->genVerticesValue ? c->genVerticesValue : c->build->genVertices;
+}
+
+/* --------------------------------------------------- cluster_classify */
+
+/* Allocation counters the stage advances; the serial run uses the live
+ * SceneBuilding values, the threaded run starts each unit at its prefix. */
+struct ClassifyCounters {
+  uint32_t split, part, genCluster, genVertex;
+  uint64_t genData;
+};
+
+/* cluster_classify.comp.glsl:150-666 for one visible cluster (template-only
+ * build: 1X/2X transient builds off, no animation).  WRITE=false only
+ * advances the counters. */
+template <bool WRITE>
+void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, ClassifyCounters& k) {
+  tc_scene_building* b = c->build;
+  tc_cluster_info ci = P<tc_cluster_info>(b->visibleClusters)[visIndex];
+  const tc_render_instance& inst = c->instances[ci.instanceID];
+  const tc_cluster& cl = P<const tc_cluster>(inst.clusters)[ci.clusterID];
+  uint32_t state = P<const uint32_t>(b->instanceStates)[ci.instanceID];
+  uint32_t nv = cl.numVertices, nt = cl.numTriangles;
+  const uint8_t* lt = P<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+
+  /* :177-195 */
+  v3 sv[256];
+  for (uint32_t v = 0; v < nv; v++) sv[v] = ld3(inst.positions, cl.firstLocalVertex + v);
+
+  /* :200-246 */
+  uint32_t fac[256][3];
+  uint32_t simple = 0;
+  bool invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
+  if (invisible) {
+    simple = nt;
+  } else {
+    v3 eye = mk(b->viewPos[0], b->viewPos[1], b->viewPos[2]);
+    for (uint32_t t = 0; t < nt; t++) {
+      v3 w0 = xform_point(inst.worldMatrix, sv[lt[t * 3 + 0]]);
+      v3 w1 = xform_point(inst.worldMatrix, sv[lt[t * 3 + 1]]);
+      v3 w2 = xform_point(inst.worldMatrix, sv[lt[t * 3 + 2]]);
+      v3 f = tess_factors(w0, w1, w2, eye, c->view->nearPlane, c->view->viewportf[1], c->view->tessRate);
+      fac[t][0] = (uint32_t)f.x, fac[t][1] = (uint32_t)f.y, fac[t][2] = (uint32_t)f.z;
+      if (amax3(fac[t][0], fac[t][1], fac[t][2]) == 1) simple++;
+    }
+  }
+
+  if (simple == nt) {
+    /* :300-445 full cluster -> template instantiation */
+    uint32_t genOffset = k.genCluster++;
+    uint64_t dataSize = P<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID];
+    uint64_t dataOffset = k.genData;
+    k.genData += dataSize;
+    uint32_t vertexOffset = k.genVertex;
+    k.genVertex += nv;
+    if (!WRITE) return;
+    fadd(&c->readback->numFullClusters, 1u);
+    bool ok = !(uint64_t(vertexOffset) + nv > cfg.maxGeneratedVertices || uint64_t(genOffset) + 1 > cfg.maxGeneratedClusters ||
+                dataOffset + dataSize > max_data_bytes(cfg));
+    if (ok) {
+      uint32_t tempOffset = fadd(&b->tempInstantiateCounter, 1u);
+      tc_template_instantiate_info ti;
+      ti.clusterIdOffset = 0;
+      ti.geometryIndexOffset = 0;
+      ti.clusterTemplateAddress = P<const uint64_t>(inst.clusterTemplateAdresses)[ci.clusterID];
+      ti.vertexBufferAddress = gen_vertices_value(c) + uint64_t(vertexOffset) * 12ull;
+      ti.vertexBufferStride = 12;
+      P<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
+      P<uint32_t>(b->tempInstanceIDs)[tempOffset] = ci.instanceID;
+      P<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
+      fadd(&P<tc_blas_build_info>(b->blasBuildInfos)[ci.instanceID].clusterReferencesCount, 1u);
+      fadd(&c->readback->numTotalTriangles, simple);
+      /* :459-488 displaced cluster vertices */
+      bool disp = has_disp(c, cfg, inst);
+      for (uint32_t v = 0; v < nv; v++) {
+        v3 p = sv[v];
+        if (disp) {
+          uint32_t vi = cl.firstLocalVertex + v;
+          p = displace(c, inst, p, ld3(inst.normals, vi), ld2(inst.texcoords, vi));
+        }
+        float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + v);
+        o[0] = p.x, o[1] = p.y, o[2] = p.z;
+      }
+    }
+    return;
+  }
+
+  /* :543-666 per-triangle split / part emission */
+  for (uint32_t t = 0; t < nt; t++) {
+    uint32_t f[3] = {fac[t][0], fac[t][1], fac[t][2]};
+    uint32_t m = amax3(f[0], f[1], f[2]);
+    bool needSplit = m > TC_TESSTABLE_SIZE;
+    bool needPart = m <= TC_TESSTABLE_SIZE;
+    uint32_t offSplit = needSplit ? k.split++ : 0;
+    uint32_t offPart = needPart ? k.part++ : 0;
+    if (!WRITE) continue;
+    tc_tess_triangle_info ti;
+    ti.cluster = ci;
+    ti.subTriangle.vtxEncoded[0] = 0;
+    ti.subTriangle.vtxEncoded[1] = 32768u;
+    ti.subTriangle.vtxEncoded[2] = 32768u << 16;
+    ti.subTriangle.triangleID_config = t;
+    if (needSplit && offSplit < cfg.maxSplitTriangles) {
+      for (int i = 0; i < 3; i++) f[i] = split_factor(f[i], cfg.maxSplitFactor);
+      uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
+      ti.subTriangle.triangleID_config |= cf << 16;
+      P<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
+    } else if (needPart && offPart < cfg.maxPartTriangles) {
+      uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
+      ti.subTriangle.triangleID_config |= cf << 16;
+      P<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
+    }
+  }
+}
+
+/* ------------------------------------------------------ triangle_split */
+
+struct SplitCounters {
+  uint32_t split, part;
+};
+
+/* triangle_split.comp.glsl:146-175 */
+void fill_base_vertices(const oracle_ctx* c, const tc_tess_triangle_info& ti, v3 pos[3], v3 bary[3]) {
+  const tc_render_instance& inst = c->instances[ti.cluster.instanceID];
+  const tc_cluster& cl = P<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
+  const uint8_t* lt = P<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+  uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
+  for (int v = 0; v < 3; v++) {
+    pos[v] = xform_point(inst.worldMatrix, ld3(inst.positions, lt[tri * 3 + v] + cl.firstLocalVertex));
+    bary[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
+  }
+}
+
+/* triangle_split.comp.glsl:201-331 for all children of one item, in
+ * child order (the order the bit-search distributor visits them). */
+template <bool WRITE>
+void split_item(const oracle_ctx* c, tc_config cfg, uint32_t itemIndex, SplitCounters& k) {
+  tc_scene_building* b = c->build;
+  tc_tess_triangle_info parent = P<tc_tess_triangle_info>(b->splitTriangles)[itemIndex];
+  uint32_t cfgP = parent.subTriangle.triangleID_config >> 16;
+  uint32_t n = cfg_entry(cfgP).numTriangles; /* setupTask :193-199 */
+  v3 pos[3], bary0[3];
+  fill_base_vertices(c, parent, pos, bary0);
+  v3 eye = mk(b->viewPos[0], b->viewPos[1], b->viewPos[2]);
+  for (uint32_t child = 0; child < n; child++) {
+    tc_tess_triangle_info ti = parent;
+    uint32_t vi[3];
+    cfg_tri_vertices(cfgP, child, vi);
+    for (int v = 0; v < 3; v++) {
+      v3 w = cfg_vertex_bary(cfgP, vi[v]);
+      v3 rebased = interp(bary0, w); /* :240-242 (precise) */
+      ti.subTriangle.vtxEncoded[v] = encode_bary(rebased);
+    }
+    v3 bary[3], wp[3];
+    for (int v = 0; v < 3; v++) bary[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
+    for (int v = 0; v < 3; v++) wp[v] = interp(pos, bary[v]); /* getFactors :177-189 */
+    v3 ff = tess_factors(wp[0], wp[1], wp[2], eye, c->view->nearPlane, c->view->viewportf[1], c->view->tessRate);
+    uint32_t f[3] = {(uint32_t)ff.x, (uint32_t)ff.y, (uint32_t)ff.z};
+    uint32_t m = amax3(f[0], f[1], f[2]);
+    bool needSplit = m > TC_TESSTABLE_SIZE;
+    bool needPart = m <= TC_TESSTABLE_SIZE;
+    uint32_t offSplit = needSplit ? k.split++ : 0;
+    uint32_t offPart = needPart ? k.part++ : 0;
+    if (!WRITE) continue;
+    if (needSplit)
+      for (int i = 0; i < 3; i++) f[i] = split_factor(f[i], cfg.maxSplitFactor);
+    uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
+    ti.subTriangle.triangleID_config = (ti.subTriangle.triangleID_config & 0xFFFFu) | (cf << 16);
+    if (needSplit && offSplit < cfg.maxSplitTriangles)
+      P<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
+    else if (needPart && offPart < cfg.maxPartTriangles)
+      P<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
+  }
+}
+
+/* ------------------------------------------- template instantiate */
+
+/* triangle_tess_template_instantiate.comp.glsl:126-373 for one part
+ * triangle with its allocation already made. */
+void instantiate_part(const oracle_ctx* c, tc_config cfg, uint32_t p, uint32_t genOffset, uint64_t dataOffset,
+                      uint32_t vertexOffset) {
+  tc_scene_building* b = c->build;
+  const tc_tessellation_table* tt = c->tessTable;
+  tc_tess_triangle_info ti = P<tc_tess_triangle_info>(b->partTriangles)[p];
+  uint32_t cf = ti.subTriangle.triangleID_config >> 16;
+  uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
+  const tc_tess_table_entry& e = cfg_entry(cf);
+  uint32_t nv = e.numVertices;
+  uint64_t dataSize = P<const uint32_t>(tt->templateInstantiationSizes)[cfg_index(cf)];
+  bool ok = !(uint64_t(vertexOffset) + nv > cfg.maxGeneratedVertices || uint64_t(genOffset) + 1 > cfg.maxGeneratedClusters ||
+              dataOffset + dataSize > max_data_bytes(cfg));
+  if (!ok) return;
+  /* :172-198 */
+  tc_template_instantiate_info info;
+  info.clusterIdOffset = p | (TC_RT_CLUSTER_MODE_SINGLE_TESSELLATED << 30);
+  info.geometryIndexOffset = 0;
+  info.clusterTemplateAddress = P<const uint64_t>(tt->templateAddresses)[cfg_index(cf)];
+  info.vertexBufferAddress = gen_vertices_value(c) + uint64_t(vertexOffset) * 12ull;
+  info.vertexBufferStride = 12;
+  uint32_t tempOffset = fadd(&b->tempInstantiateCounter, 1u);
+  P<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = info;
+  P<uint32_t>(b->tempInstanceIDs)[tempOffset] = ti.cluster.instanceID;
+  P<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
+  fadd(&P<tc_blas_build_info>(b->blasBuildInfos)[ti.cluster.instanceID].clusterReferencesCount, 1u);
+  fadd(&c->readback->numTotalTriangles, (uint32_t)e.numTriangles);
+
+  /* :295-372 vertex generation */
+  const tc_render_instance& inst = c->instances[ti.cluster.instanceID];
+  const tc_cluster& cl = P<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
+  const uint8_t* lt = P<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+  v3 bb[3], bp[3], bn[3];
+  v2 bt[3];
+  for (int v = 0; v < 3; v++) {
+    bb[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
+    uint32_t idx = lt[tri * 3 + v] + cl.firstLocalVertex;
+    bp[v] = ld3(inst.positions, idx);
+    bn[v] = normalize(ld3(inst.normals, idx));
+    bt[v] = ld2(inst.texcoords, idx);
+  }
+  bool pn = (cfg.flags & TC_FLAG_PN) != 0;
+  PN basePN;
+  if (pn) setup_pn(basePN, bp, bn);
+  bool disp = has_disp(c, cfg, inst);
+  for (uint32_t vert = 0; vert < nv; vert++) {
+    v3 w = interp(bb, cfg_vertex_bary(cf, vert));
+    v3 pos = pn ? get_pn(basePN, w) : interp(bp, w);
+    if (disp) pos = displace(c, inst, pos, interp(bn, w), interp2(bt, w));
+    float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + vert);
+    o[0] = pos.x, o[1] = pos.y, o[2] = pos.z;
+  }
+}
+
+/* -------------------------------------------------- parallel helper */
+
+template <class F>
+void parallel_for(uint32_t n, uint32_t nthreads, F&& fn) {
+  if (nthreads <= 1 || n < 2 * nthreads) {
+    for (uint32_t i = 0; i < n; i++) fn(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  uint32_t chunk = (n + nthreads - 1) / nthreads;
+  for (uint32_t t = 0; t < nthreads; t++) {
+    uint32_t lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([lo, hi, &fn] {
+      for (uint32_t i = lo; i < hi; i++) fn(i);
+    });
+  }
+  for (auto& x : th) x.join();
+}
+
+void classify_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
+  tc_scene_building* b = c->build;
+  uint32_t n = b->visibleClusterCounter; /* clamped by build_setup CLASSIFY */
+  ClassifyCounters base{(uint32_t)b->splitTriangleCounter, b->partTriangleCounter, b->genClusterCounter, b->genVertexCounter,
+                        b->genClusterDataCounter};
+  if (nthreads <= 1) {
+    ClassifyCounters k = base;
+    for (uint32_t i = 0; i < n; i++) classify_cluster<true>(c, cfg, i, k);
+    base = k;
+  } else {
+    std::vector<ClassifyCounters> cnt(n);
+    parallel_for(n, nthreads, [&](uint32_t i) {
+      ClassifyCounters k{0, 0, 0, 0, 0};
+      classify_cluster<false>(c, cfg, i, k);
+      cnt[i] = k;
+    });
+    ClassifyCounters run = base;
+    for (uint32_t i = 0; i < n; i++) {
+      ClassifyCounters d = cnt[i];
+      cnt[i] = run;
+      run.split += d.split, run.part += d.part, run.genCluster += d.genCluster, run.genVertex += d.genVertex,
+          run.genData += d.genData;
+    }
+    /* the full-cluster template records are appended with a counter whose
+     * order must follow cluster order: emit them serially, the rest in
+     * parallel.  Full clusters only touch tempInstantiateCounter. */
+    std::vector<uint8_t> isFull(n);
+    for (uint32_t i = 0; i < n; i++) isFull[i] = (i + 1 < n ? cnt[i + 1].genCluster : run.genCluster) != cnt[i].genCluster;
+    parallel_for(n, nthreads, [&](uint32_t i) {
+      if (isFull[i]) return;
+      ClassifyCounters k = cnt[i];
+      classify_cluster<true>(c, cfg, i, k);
+    });
+    for (uint32_t i = 0; i < n; i++)
+      if (isFull[i]) {
+        ClassifyCounters k = cnt[i];
+        classify_cluster<true>(c, cfg, i, k);
+      }
+    base = run;
+  }
+  b->splitTriangleCounter = (int32_t)base.split;
+  b->partTriangleCounter = base.part;
+  b->genClusterCounter = base.genCluster;
+  b->genVertexCounter = base.genVertex;
+  b->genClusterDataCounter = base.genData;
+}
+
+void split_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
+  tc_scene_building* b = c->build;
+  uint32_t start = b->splitPassStart, end = b->splitPassEnd;
+  uint32_t n = end > start ? end - start : 0;
+  SplitCounters base{b->splitWriteCounter, b->partTriangleCounter};
+  if (nthreads <= 1) {
+    SplitCounters k = base;
+    for (uint32_t i = 0; i < n; i++) split_item<true>(c, cfg, start + i, k);
+    base = k;
+  } else {
+    std::vector<SplitCounters> cnt(n);
+    parallel_for(n, nthreads, [&](uint32_t i) {
+      SplitCounters k{0, 0};
+      split_item<false>(c, cfg, start + i, k);
+      cnt[i] = k;
+    });
+    SplitCounters run = base;
+    for (uint32_t i = 0; i < n; i++) {
+      SplitCounters d = cnt[i];
+      cnt[i] = run;
+      run.split += d.split, run.part += d.part;
+    }
+    parallel_for(n, nthreads, [&](uint32_t i) {
+      SplitCounters k = cnt[i];
+      split_item<true>(c, cfg, start + i, k);
+    });
+    base = run;
+  }
+  b->splitWriteCounter = base.split;
+  b->partTriangleCounter = base.part;
+}
+
+void instantiate_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
+  tc_scene_building* b = c->build;
+  const tc_tessellation_table* tt = c->tessTable;
+  uint32_t n = b->partTriangleCounter; /* clamped by build_setup INSTANTIATE_TESS */
+  std::vector<uint32_t> gen(n), vtx(n);
+  std::vector<uint64_t> data(n);
+  uint32_t g = b->genClusterCounter, vv = b->genVertexCounter;
+  uint64_t d = b->genClusterDataCounter;
+  /* :155-160 allocation in part order */
+  for (uint32_t p = 0; p < n; p++) {
+    uint32_t cf = P<tc_tess_triangle_info>(b->partTriangles)[p].subTriangle.triangleID_config >> 16;
+    gen[p] = g++;
+    data[p] = d;
+    d += P<const uint32_t>(tt->templateInstantiationSizes)[cfg_index(cf)];
+    vtx[p] = vv;
+    vv += cfg_entry(cf).numVertices;
+  }
+  /* accepted parts form a prefix, so the template slot of part p is the
+   * running count of accepted parts before it; emit records serially and
+   * the vertices in parallel */
+  if (nthreads <= 1) {
+    for (uint32_t p = 0; p < n; p++) instantiate_part(c, cfg, p, gen[p], data[p], vtx[p]);
+  } else {
+    uint32_t t0 = b->tempInstantiateCounter;
+    uint32_t accepted = 0;
+    for (uint32_t p = 0; p < n; p++) {
+      uint32_t cf = P<tc_tess_triangle_info>(b->partTriangles)[p].subTriangle.triangleID_config >> 16;
+      uint64_t ds = P<const uint32_t>(tt->templateInstantiationSizes)[cfg_index(cf)];
+      bool ok = !(uint64_t(vtx[p]) + cfg_entry(cf).numVertices > cfg.maxGeneratedVertices ||
+                  uint64_t(gen[p]) + 1 > cfg.maxGeneratedClusters || data[p] + ds > max_data_bytes(cfg));
+      if (!ok) break;
+      accepted++;
+    }
+    parallel_for(accepted, nthreads, [&](uint32_t p) {
+      tc_scene_building local = *b;
+      local.tempInstantiateCounter = t0 + p;
+      oracle_ctx lc = *c;
+      lc.build = &local;
+      if (!lc.genVerticesValue) lc.genVerticesValue = b->genVertices;
+      instantiate_part(&lc, cfg, p, gen[p], data[p], vtx[p]);
+    });
+    b->tempInstantiateCounter = t0 + accepted;
+  }
+  b->genClusterCounter = g;
+  b->genVertexCounter = vv;
+  b->genClusterDataCounter = d;
+}
+
+void frame_tessellate(const oracle_ctx* c, tc_config cfg, uint32_t passes, uint32_t nthreads) {
+  if (passes == 0) passes = oracle_num_split_passes(cfg.maxSplitFactor);
+  oracle_build_setup(c, cfg, TC_BUILD_SETUP_CLASSIFY);
+  classify_all(c, cfg, nthreads);
+  oracle_build_setup(c, cfg, TC_BUILD_SETUP_SPLIT);
+  for (uint32_t p = 0; p < passes; p++) {
+    split_all(c, cfg, nthreads);
+    if (p + 1 < passes) oracle_build_setup(c, cfg, TC_BUILD_SETUP_SPLIT_PASS);
+  }
+  oracle_build_setup(c, cfg, TC_BUILD_SETUP_INSTANTIATE_TESS);
+  instantiate_all(c, cfg, nthreads);
+  oracle_build_setup(c, cfg, TC_BUILD_SETUP_BUILD_BLAS);
+}
+
+/* ------------------------------------------- culling front-end fixture */
+
+inline void xform4(const float* m, const float p[4], float o[4]) {
+  for (int r = 0; r < 4; r++) o[r] = ((m[r] * p[0] + m[4 + r] * p[1]) + m[8 + r] * p[2]) + m[12 + r] * p[3];
+}
+
+void mat4_mul(const float* a, const float* b, float* o) {
+  for (int col = 0; col < 4; col++)
+    for (int r = 0; r < 4; r++)
+      o[col * 4 + r] = ((a[r] * b[col * 4 + 0] + a[4 + r] * b[col * 4 + 1]) + a[8 + r] * b[col * 4 + 2]) + a[12 + r] * b[col * 4 + 3];
+}
+
+}  // namespace
+
+/* ======================================================== extern "C" */
+
+extern "C" {
+
+uint32_t oracle_table_num_vertices(void) { return TC_TABLE_NUM_VERTICES; }
+uint32_t oracle_table_num_triangles(void) { return TC_TABLE_NUM_TRIANGLES; }
+uint32_t oracle_table_num_configs(void) { return TC_TABLE_NUM_CONFIGS; }
+const uint32_t* oracle_table_vertices(void) { return tc_table_nv_vertices; }
+const uint32_t* oracle_table_triangles(void) { return tc_table_nv_triangles; }
+const uint16_t* oracle_table_configs(void) { return tc_table_nv_configs; }
+void oracle_table_lookup(tc_tess_table_entry out[4096]) { memcpy(out, table().lookup, sizeof(table().lookup)); }
+
+uint32_t oracle_encode_barycentrics(const float wuv[3]) { return encode_bary(mk(wuv[0], wuv[1], wuv[2])); }
+void oracle_decode_barycentrics(uint32_t vtx, float wuv[3]) {
+  v3 w = decode_bary(vtx);
+  wuv[0] = w.x, wuv[1] = w.y, wuv[2] = w.z;
+}
+void oracle_tess_factors(const float a[3], const float b[3], const float c[3], const float eye[3], float nearPlane, float vpy,
+                         float rate, float out[3]) {
+  v3 f = tess_factors(mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]), mk(c[0], c[1], c[2]), mk(eye[0], eye[1], eye[2]), nearPlane, vpy,
+                      rate);
+  out[0] = f.x, out[1] = f.y, out[2] = f.z;
+}
+void oracle_split_factor(const uint32_t f[3], uint32_t maxSplit, uint32_t out[3]) {
+  for (int i = 0; i < 3; i++) out[i] = split_factor(f[i], maxSplit);
+}
+uint32_t oracle_get_config(const uint32_t factors[3], uint32_t vtx[3]) {
+  uint32_t f[3] = {factors[0], factors[1], factors[2]};
+  return get_config(f, vtx);
+}
+/* tessellation.glsl:111-117 */
+void oracle_config_factors(uint32_t cfg, uint32_t out[3]) {
+  uint32_t idx = cfg_index(cfg) + (1 + 16 + 256);
+  out[0] = idx & 15, out[1] = (idx / 16) & 15, out[2] = (idx / 256) & 15;
+}
+void oracle_config_triangle_vertices(uint32_t cfg, uint32_t tri, uint32_t out[3]) { cfg_tri_vertices(cfg, tri, out); }
+void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]) {
+  v3 w = cfg_vertex_bary(cfg, vert);
+  wuv[0] = w.x, wuv[1] = w.y, wuv[2] = w.z;
+}
+float oracle_sample_height(const tc_displacement_texture* tex, float u, float v) { return sample_height(*tex, u, v); }
+
+/* instances_classify.comp.glsl:100-127 with culling.glsl:30-90 (frustum and
+ * size test; no HiZ pyramid exists, so the occlusion test passes). */
+void oracle_instances_classify(const oracle_ctx* c, tc_config) {
+  tc_scene_building* b = c->build;
+  const tc_frame_constants* vw = c->view;
+  for (uint32_t i = 0; i < c->numRenderInstances; i++) {
+    const tc_render_instance& inst = c->instances[i];
+    float wvp[16];
+    mat4_mul(vw->viewProjMatrix, inst.worldMatrix, wvp);
+    uint32_t bits = 0x7F;
+    float cmin[4], cmax[4];
+    bool clipValid = true;
+    for (int n = 0; n < 8; n++) {
+      float p[4] = {(n & 1) ? inst.geoHi[0] : inst.geoLo[0], (n & 2) ? inst.geoHi[1] : inst.geoLo[1],
+                    (n & 4) ? inst.geoHi[2] : inst.geoLo[2], 1.0f};
+      float h[4];
+      xform4(wvp, p, h);
+      bool valid = !(-1.2e-07f < h[3] && h[3] < 1.2e-07f);
+      float aw = fabsf(h[3]);
+      float clip[4] = {h[0] / aw, h[1] / aw, h[2] / aw, h[3]};
+      uint32_t cb = 0;
+      cb |= h[0] < -h[3] ? 1u : 0u;
+      cb |= h[0] > h[3] ? 2u : 0u;
+      cb |= h[1] < -h[3] ? 4u : 0u;
+      cb |= h[1] > h[3] ? 8u : 0u;
+      cb |= h[2] < 0 ? 16u : 0u;
+      cb |= h[2] > h[3] ? 32u : 0u;
+      cb |= h[3] <= 0 ? 64u : 0u;
+      bits &= cb;
+      for (int k = 0; k < 4; k++) {
+        cmin[k] = n == 0 ? clip[k] : std::min(cmin[k], clip[k]);
+        cmax[k] = n == 0 ? clip[k] : std::max(cmax[k], clip[k]);
+      }
+      clipValid = clipValid && valid;
+    }
+    for (int k = 0; k < 2; k++) {
+      cmin[k] = std::min(std::max(cmin[k], -1.0f), 1.0f);
+      cmax[k] = std::min(std::max(cmax[k], -1.0f), 1.0f);
+    }
+    bool inFrustum = bits == 0;
+    bool sizeOk = (cmax[0] - cmin[0]) > 2.0f / vw->viewportf[0] || (cmax[1] - cmin[1]) > 2.0f / vw->viewportf[1];
+    bool visible = inFrustum && (!clipValid || sizeOk);
+    P<uint32_t>(b->instanceStates)[i] = (inFrustum ? TC_INSTANCE_FRUSTUM_BIT : 0u) | (visible ? TC_INSTANCE_VISIBLE_BIT : 0u);
+    P<tc_blas_build_info>(b->blasBuildInfos)[i].clusterReferencesCount = 0;
+  }
+}
+
+/* clusters_cull.comp.glsl:112-161 in ray-tracing mode: every cluster of
+ * every instance is appended, instance by instance. */
+void oracle_clusters_cull(const oracle_ctx* c, tc_config cfg) {
+  tc_scene_building* b = c->build;
+  for (uint32_t i = 0; i < c->numRenderInstances; i++)
+    for (uint32_t cl = 0; cl < c->instances[i].numClusters; cl++) {
+      uint32_t off = b->visibleClusterCounter++;
+      if (off < cfg.maxVisibleClusters) P<tc_cluster_info>(b->visibleClusters)[off] = tc_cluster_info{i, cl};
+    }
+}
+
+/* build_setup.comp.glsl:102-264 (no transient builds) */
+void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode) {
+  tc_scene_building* b = c->build;
+  tc_readback* rb = c->readback;
+  if (mode == TC_BUILD_SETUP_CLASSIFY) {
+    uint32_t counter = b->visibleClusterCounter;
+    rb->numVisibleClusters = counter;
+    counter = std::min(counter, cfg.maxVisibleClusters);
+    b->visibleClusterCounter = counter;
+    b->dispatchClassify = tc_dispatch_indirect{counter, 1, 1};
+  } else if (mode == TC_BUILD_SETUP_DRAW_TESS) {
+    uint32_t full = b->fullClusterCounter, part = b->partTriangleCounter;
+    rb->numFullClusters = full;
+    rb->numPartTriangles = part;
+    rb->numSplitTriangles = b->splitWriteCounter;
+    full = std::min(full, cfg.maxVisibleClusters);
+    part = std::min(part, cfg.maxPartTriangles);
+    b->drawFullClusters = tc_draw_mesh_tasks_indirect{full, 0};
+    b->drawPartTriangles = tc_draw_mesh_tasks_indirect{(part + 31) / 32, 0};
+    rb->numBlasClusters = full;
+  } else if (mode == TC_BUILD_SETUP_SPLIT) {
+    /* min(int, uint) in GLSL converts the int operand to uint */
+    uint32_t count = std::min((uint32_t)b->splitTriangleCounter, cfg.maxSplitTriangles);
+    b->splitWriteCounter = count;
+    b->splitTriangleCounter = (int32_t)count;
+    b->splitPassStart = 0;
+    b->splitPassEnd = count;
+    b->dispatchTriangleSplit = tc_dispatch_indirect{(count + 63) / 64, 1, 1};
+  } else if (mode == TC_BUILD_SETUP_SPLIT_PASS) {
+    b->splitPass = b->splitPass + 1;
+    uint32_t start = std::min(b->splitPassEnd, cfg.maxSplitTriangles);
+    uint32_t end = std::min(b->splitWriteCounter, cfg.maxSplitTriangles);
+    b->splitPassStart = start;
+    b->splitPassEnd = end;
+    uint32_t count = end - start;
+    b->dispatchTriangleSplit = tc_dispatch_indirect{(count + 63) / 64, 1, 1};
+  } else if (mode == TC_BUILD_SETUP_BUILD_BLAS) {
+    uint32_t temp = b->tempInstantiateCounter;
+    rb->numBlasClusters = temp;
+    rb->numTempInstantiations = temp;
+    rb->numGenDatas = b->genClusterDataCounter;
+    rb->numGenVertices = b->genVertexCounter;
+    rb->numBlasReservedSizes = b->numBlasReservedSizes;
+    temp = std::min(cfg.maxGeneratedClusters, temp);
+    b->tempInstantiateCounter = temp;
+    rb->numActualTempInstantiations = temp;
+    b->dispatchBlasTempInsert = tc_dispatch_indirect{(temp + 63) / 64, 1, 1};
+  } else if (mode == TC_BUILD_SETUP_INSTANTIATE_TESS) {
+    uint32_t part = b->partTriangleCounter;
+    rb->numPartTriangles = part;
+    rb->numSplitTriangles = b->splitWriteCounter;
+    part = std::min(part, cfg.maxPartTriangles);
+    b->partTriangleCounter = part;
+    b->dispatchTriangleInstantiate = tc_dispatch_indirect{(part + 31) / 32, 1, 1};
+  }
+}
+
+void oracle_cluster_classify(const oracle_ctx* c, tc_config cfg) { classify_all(c, cfg, 1); }
+void oracle_triangle_split(const oracle_ctx* c, tc_config cfg) { split_all(c, cfg, 1); }
+void oracle_triangle_tess_template_instantiate(const oracle_ctx* c, tc_config cfg) { instantiate_all(c, cfg, 1); }
+
+/* blas_setup_insertion.comp.glsl:100-116 in instance order */
+void oracle_blas_setup_insertion(const oracle_ctx* c, tc_config) {
+  tc_scene_building* b = c->build;
+  for (uint32_t i = 0; i < c->numRenderInstances; i++) {
+    tc_blas_build_info& bi = P<tc_blas_build_info>(b->blasBuildInfos)[i];
+    uint32_t off = b->blasClusterCounter;
+    b->blasClusterCounter += bi.clusterReferencesCount;
+    bi.clusterReferencesCount = 0;
+    bi.clusterReferencesStride = 8;
+    bi.clusterReferences = b->blasClusterAddresses + uint64_t(off) * 8ull;
+    c->readback->numBlasActualSizes += P<const uint32_t>(b->blasBuildSizes)[i];
+  }
+}
+
+/* blas_clusters_insert.comp.glsl:97-135 in CLAS order.  clusterReferences
+ * holds the address of the per-instance list in the build's address space;
+ * blasClusterAddresses is the host pointer of the same array. */
+void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config, int transient) {
+  if (transient) return;
+  tc_scene_building* b = c->build;
+  uint32_t n = b->tempInstantiateCounter;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t inst = P<const uint32_t>(b->tempInstanceIDs)[i];
+    uint64_t addr = P<const uint64_t>(b->tempClusterAddresses)[i];
+    uint32_t size = P<const uint32_t>(b->tempClusterSizes)[i];
+    tc_blas_build_info& bi = P<tc_blas_build_info>(b->blasBuildInfos)[inst];
+    uint32_t idx = bi.clusterReferencesCount++;
+    P<uint64_t>(bi.clusterReferences)[idx] = addr;
+    c->readback->numGenActualDatas += size;
+  }
+}
+
+/* Our CLAS build (stands for the NV template instantiation,
+ * renderer_raytrace_clusters_tess.cpp:595-623): header, group boxes of
+ * TC_CLAS_GROUP_TRIANGLES triangles, triangle words and vertex copy. */
+void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
+  tc_scene_building* b = c->build;
+  uint32_t n = std::min(b->tempInstantiateCounter, cfg.maxGeneratedClusters);
+  auto r16 = [](uint32_t v) { return (v + 15u) & ~15u; };
+  for (uint32_t i = 0; i < n; i++) {
+    const tc_template_instantiate_info& info = P<const tc_template_instantiate_info>(b->tempInstantiations)[i];
+    const tc_clas_template& T = *P<const tc_clas_template>(info.clusterTemplateAddress);
+    uint32_t nt = T.numTriangles, nv = T.numVertices, ng = T.numGroups;
+    char* dst = P<char>(P<const uint64_t>(b->tempClusterAddresses)[i]);
+    tc_aabb* groups = reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header));
+    uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
+    float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
+    const char* src = P<const char>(info.vertexBufferAddress);
+    for (uint32_t v = 0; v < nv; v++)
+      for (int k = 0; k < 3; k++) verts[v * 3 + k] = reinterpret_cast<const float*>(src + uint64_t(v) * info.vertexBufferStride)[k];
+    for (uint32_t t = 0; t < nt; t++) tris[t] = P<const uint32_t>(T.triangles)[t];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t g = 0; g < ng; g++) {
+      tc_aabb box;
+      for (int k = 0; k < 3; k++) box.lo[k] = INFINITY, box.hi[k] = -INFINITY;
+      for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < std::min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES); t++)
+        for (int k = 0; k < 3; k++) {
+          uint32_t vi = (tris[t] >> (8 * k)) & 0xFF;
+          for (int a = 0; a < 3; a++) {
+            box.lo[a] = std::min(box.lo[a], verts[vi * 3 + a]);
+            box.hi[a] = std::max(box.hi[a], verts[vi * 3 + a]);
+          }
+        }
+      groups[g] = box;
+      for (int a = 0; a < 3; a++) lo[a] = std::min(lo[a], box.lo[a]), hi[a] = std::max(hi[a], box.hi[a]);
+    }
+    tc_clas_header h;
+    for (int a = 0; a < 3; a++) h.lo[a] = lo[a], h.hi[a] = hi[a];
+    h.clusterID = T.clusterID + info.clusterIdOffset;
+    h.counts = nt | (nv << 8) | (ng << 16);
+    memcpy(dst, &h, sizeof(h));
+    P<uint32_t>(b->tempClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+  }
+}
+
+uint32_t oracle_num_split_passes(uint32_t splitFactor) {
+  /* renderer_raytrace_clusters_tess.cpp:516-520 */
+  uint32_t coord = 32768, passes = 0;
+  if (splitFactor < 2) splitFactor = 2;
+  while (coord > splitFactor) {
+    coord /= splitFactor;
+    passes++;
+  }
+  return passes;
+}
+
+void oracle_frame_tessellate(const oracle_ctx* c, tc_config cfg, uint32_t passes) { frame_tessellate(c, cfg, passes, 1); }
+void oracle_frame_tessellate_mt(const oracle_ctx* c, tc_config cfg, uint32_t passes, uint32_t nthreads) {
+  frame_tessellate(c, cfg, passes, nthreads);
+}
+
+}  // extern "C"

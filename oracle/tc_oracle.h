@@ -1,0 +1,99 @@
+/*
+ * tc_oracle.h -- CPU oracle for the tessellated-cluster frame path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a serial C++ restatement of the
+ * reference's GLSL stages, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  The product path never
+ * links, loads or calls it.
+ *
+ * All tc_* records come from include/tc_abi.h.  Every uint64 "device
+ * address" field of the records handed to the oracle holds a host pointer
+ * (the oracle dereferences them), except the pure values the reference only
+ * copies into records (template addresses, genClusterData): those are
+ * carried through as numbers.
+ *
+ * Canonical order.  The reference appends to every list with atomicAdd, so
+ * its order depends on scheduling.  The oracle defines the order a serial
+ * execution yields when workgroups run one after another in index order:
+ * (visible cluster, triangle) for classify, (split item, child) for split,
+ * part index for instantiate, instance index for BLAS setup, CLAS index for
+ * insertion.  The HIP kernels produce the same order.
+ *
+ * Floating point.  Built with -ffp-contract=off.  Evaluation order is the
+ * left-to-right order written in the GLSL, with mat4*vec4 summed column by
+ * column, length = sqrtf(dot), normalize = v * (1/length), GLSL round =
+ * roundf (half away from zero).
+ */
+#ifndef TC_ORACLE_H
+#define TC_ORACLE_H
+
+#include "tc_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- tessellation table (tessellation_table.cpp:36-100) ---- */
+uint32_t oracle_table_num_vertices(void);
+uint32_t oracle_table_num_triangles(void);
+uint32_t oracle_table_num_configs(void);
+const uint32_t* oracle_table_vertices(void);
+const uint32_t* oracle_table_triangles(void);
+const uint16_t* oracle_table_configs(void);
+void oracle_table_lookup(tc_tess_table_entry out[4096]);
+
+/* ---- tessellation.glsl utilities ---- */
+uint32_t oracle_encode_barycentrics(const float wuv[3]);
+void oracle_decode_barycentrics(uint32_t vtx, float wuv[3]);
+void oracle_tess_factors(const float a[3], const float b[3], const float c[3], const float viewPos[3], float nearPlane,
+                         float viewportY, float tessRate, float factors[3]);
+void oracle_split_factor(const uint32_t f[3], uint32_t maxSplitFactor, uint32_t out[3]);
+uint32_t oracle_get_config(const uint32_t factors[3], uint32_t vtxEncoded[3]);
+void oracle_config_factors(uint32_t cfg, uint32_t out[3]);
+void oracle_config_triangle_vertices(uint32_t cfg, uint32_t tri, uint32_t out[3]);
+void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]);
+float oracle_sample_height(const tc_displacement_texture* tex, float u, float v);
+
+/* ---- frame stages (host-address records) ---- */
+typedef struct oracle_ctx {
+  const tc_frame_constants* view;
+  const tc_tessellation_table* tessTable; /* host addresses */
+  tc_readback* readback;
+  const tc_render_instance* instances; /* host addresses */
+  tc_scene_building* build;            /* host addresses */
+  const tc_displacement_texture* displacementTextures;
+  uint32_t numRenderInstances;
+  uint32_t numDisplacementTextures;
+  /* value (not dereferenced) the genVertices field stands for in the
+   * vertexBufferAddress records; 0 means use build->genVertices */
+  uint64_t genVerticesValue;
+} oracle_ctx;
+
+void oracle_instances_classify(const oracle_ctx* c, tc_config cfg);
+void oracle_clusters_cull(const oracle_ctx* c, tc_config cfg);
+void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode);
+void oracle_cluster_classify(const oracle_ctx* c, tc_config cfg);
+void oracle_triangle_split(const oracle_ctx* c, tc_config cfg);
+void oracle_triangle_tess_template_instantiate(const oracle_ctx* c, tc_config cfg);
+void oracle_blas_setup_insertion(const oracle_ctx* c, tc_config cfg);
+void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config cfg, int transient);
+
+void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg);
+
+uint32_t oracle_num_split_passes(uint32_t splitFactor);
+
+/* The chain of renderer_raytrace_clusters_tess.cpp:467-582 from
+ * build_setup CLASSIFY through build_setup BUILD_BLAS (visible clusters and
+ * instance states are inputs).  CLAS build and BLAS insertion follow as
+ * separate calls. */
+void oracle_frame_tessellate(const oracle_ctx* c, tc_config cfg, uint32_t numSplitPasses);
+
+/* Same chain on nthreads threads for the CPU baseline (parallel over
+ * clusters / items with a deterministic merge); identical output. */
+void oracle_frame_tessellate_mt(const oracle_ctx* c, tc_config cfg, uint32_t numSplitPasses, uint32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TC_ORACLE_H */

@@ -1,0 +1,355 @@
+"""Runs one frame of the tessellation path through the CPU oracle and through
+the HIP library on identical inputs, and returns comparable results.
+
+Device addresses differ between the two runs, so every address-valued field
+is turned into an offset from its buffer base (or a template identity)
+before comparison.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from vk_tessellated_clusters_amd import abi
+from vk_tessellated_clusters_amd.device import cluster_templates, template_records, instance_record
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
+
+P = C.c_void_p
+
+
+class OracleCtx(C.Structure):
+    _fields_ = [("view", P), ("tessTable", P), ("readback", P), ("instances", P), ("build", P),
+                ("displacementTextures", P), ("numRenderInstances", C.c_uint32), ("numDisplacementTextures", C.c_uint32),
+                ("genVerticesValue", C.c_uint64)]
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(str(ORACLE_LIB))
+        ctx = C.POINTER(OracleCtx)
+        for n in ("oracle_instances_classify", "oracle_clusters_cull", "oracle_cluster_classify", "oracle_triangle_split",
+                  "oracle_triangle_tess_template_instantiate", "oracle_blas_setup_insertion", "oracle_clas_instantiate"):
+            getattr(L, n).argtypes = [ctx, abi.Config]
+            getattr(L, n).restype = None
+        L.oracle_build_setup.argtypes = [ctx, abi.Config, C.c_int]
+        L.oracle_blas_clusters_insert.argtypes = [ctx, abi.Config, C.c_int]
+        L.oracle_frame_tessellate.argtypes = [ctx, abi.Config, C.c_uint32]
+        L.oracle_frame_tessellate_mt.argtypes = [ctx, abi.Config, C.c_uint32, C.c_uint32]
+        L.oracle_num_split_passes.argtypes = [C.c_uint32]
+        L.oracle_num_split_passes.restype = C.c_uint32
+        L.oracle_encode_barycentrics.argtypes = [P]
+        L.oracle_encode_barycentrics.restype = C.c_uint32
+        L.oracle_decode_barycentrics.argtypes = [C.c_uint32, P]
+        L.oracle_tess_factors.argtypes = [P, P, P, P, C.c_float, C.c_float, C.c_float, P]
+        L.oracle_split_factor.argtypes = [P, C.c_uint32, P]
+        L.oracle_get_config.argtypes = [P, P]
+        L.oracle_get_config.restype = C.c_uint32
+        L.oracle_config_factors.argtypes = [C.c_uint32, P]
+        L.oracle_config_triangle_vertices.argtypes = [C.c_uint32, C.c_uint32, P]
+        L.oracle_config_vertex_barycentrics.argtypes = [C.c_uint32, C.c_uint32, P]
+        L.oracle_sample_height.argtypes = [P, C.c_float, C.c_float]
+        L.oracle_sample_height.restype = C.c_float
+        L.oracle_table_lookup.argtypes = [P]
+        for n in ("oracle_table_num_vertices", "oracle_table_num_triangles", "oracle_table_num_configs"):
+            getattr(L, n).restype = C.c_uint32
+        for n, t in (("oracle_table_vertices", C.c_uint32), ("oracle_table_triangles", C.c_uint32),
+                     ("oracle_table_configs", C.c_uint16)):
+            getattr(L, n).restype = C.POINTER(t)
+        _oracle = L
+    return _oracle
+
+
+def small_config(**kw) -> abi.Config:
+    c = abi.default_config()
+    c.maxVisibleClusters = 1 << 14
+    c.maxPartTriangles = 1 << 18
+    c.maxSplitTriangles = 1 << 14
+    c.maxGeneratedClusters = (1 << 14) + (1 << 18)
+    c.maxGeneratedClusterMegs = 512
+    c.maxGeneratedVertices = 1 << 22
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+ADDR_FIELDS = {"instanceStates", "visibleClusters", "fullClusters", "splitTriangles", "partTriangles",
+               "basicClusterSizes", "genClusterData", "genVertices", "tempInstanceIDs", "tempInstantiations",
+               "tempClusterAddresses", "tempClusterSizes", "transInstanceIDs", "transBuilds", "transClusterAddresses",
+               "transClusterSizes", "transTriMappings", "transTriIndices", "blasBuildInfos", "blasBuildSizes",
+               "blasClusterAddresses", "blasBuildData"}
+
+
+def counters(sb: abi.SceneBuilding) -> dict:
+    out = {}
+    for name, _ in sb._fields_:
+        if name in ADDR_FIELDS or name.startswith("_"):
+            continue
+        v = getattr(sb, name)
+        if isinstance(v, C.Structure):
+            v = tuple(getattr(v, f) for f, _ in v._fields_)
+        elif isinstance(v, C.Array):
+            v = tuple(v)
+        out[name] = v
+    return out
+
+
+def readback_dict(rb: abi.Readback) -> dict:
+    return {n: (tuple(getattr(rb, n)) if isinstance(getattr(rb, n), C.Array) else getattr(rb, n))
+            for n, _ in rb._fields_}
+
+
+class Results:
+    """Normalised frame outputs."""
+
+    def __init__(self):
+        self.counters = {}
+        self.readback = {}
+        self.arrays = {}
+
+
+def _finish_results(sb, rb, fetch, cfg, template_id):
+    """fetch(addr, nbytes) -> np.uint8 array.  template_id maps a template
+    address to a comparable identity."""
+    R = Results()
+    R.counters = counters(sb)
+    R.readback = readback_dict(rb)
+    b = sb
+    base = {k: getattr(b, k) for k in ADDR_FIELDS}
+    n_inst = b.numRenderInstances
+    R.arrays["instanceStates"] = fetch(base["instanceStates"], 4 * n_inst).view(np.uint32)
+    nvis = min(b.visibleClusterCounter, cfg.maxVisibleClusters)
+    R.arrays["visibleClusters"] = fetch(base["visibleClusters"], 8 * nvis).view(np.uint32)
+    R.arrays["splitTriangles"] = fetch(base["splitTriangles"], 24 * cfg.maxSplitTriangles).view(abi.TESS_TRIANGLE_INFO)
+    npart = min(b.partTriangleCounter, cfg.maxPartTriangles)
+    R.arrays["partTriangles"] = fetch(base["partTriangles"], 24 * npart).view(abi.TESS_TRIANGLE_INFO)
+    ntemp = min(b.tempInstantiateCounter, cfg.maxGeneratedClusters)
+    ti = fetch(base["tempInstantiations"], 32 * ntemp).view(abi.TEMPLATE_INSTANTIATE_INFO).copy()
+    tmpl = [template_id(int(a)) for a in ti["clusterTemplateAddress"]]
+    R.arrays["templateIdentity"] = tmpl
+    vb = ti["vertexBufferAddress"].astype(np.int64) - np.int64(base["genVertices"])
+    ti["vertexBufferAddress"] = vb.astype(np.uint64)
+    ti["clusterTemplateAddress"] = 0
+    R.arrays["tempInstantiations"] = ti
+    R.arrays["tempInstanceIDs"] = fetch(base["tempInstanceIDs"], 4 * ntemp).view(np.uint32)
+    ca = fetch(base["tempClusterAddresses"], 8 * ntemp).view(np.uint64)
+    R.arrays["tempClusterOffsets"] = ca - np.uint64(base["genClusterData"])
+    sizes = fetch(base["tempClusterSizes"], 4 * ntemp).view(np.uint32)
+    R.arrays["tempClusterSizes"] = sizes
+    R.arrays["genVertices"] = fetch(base["genVertices"], 12 * b.genVertexCounter).view(np.float32).reshape(-1, 3) \
+        if b.genVertexCounter <= cfg.maxGeneratedVertices else None
+    # CLAS blocks, concatenated in CLAS order
+    blocks = [fetch(int(base["genClusterData"] + int(o)), int(s)) for o, s in zip(R.arrays["tempClusterOffsets"], sizes)]
+    R.arrays["clasBlocks"] = np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
+    bi = fetch(base["blasBuildInfos"], 16 * n_inst).view(abi.BLAS_BUILD_INFO).copy()
+    bi["clusterReferences"] = (bi["clusterReferences"].astype(np.int64) - np.int64(base["blasClusterAddresses"])
+                               ).astype(np.uint64)
+    R.arrays["blasBuildInfos"] = bi
+    nref = b.blasClusterCounter
+    refs = fetch(base["blasClusterAddresses"], 8 * nref).view(np.uint64)
+    R.arrays["blasClusterOffsets"] = refs - np.uint64(base["genClusterData"])
+    return R
+
+
+# ----------------------------------------------------------------- oracle
+
+
+class OracleFrame:
+    """Host buffers of one frame and the oracle chain over them."""
+
+    def __init__(self, host_scene, cfg: abi.Config):
+        L = abi.lib()
+        self.cfg = cfg
+        self.keep = []
+        self.tmpl_ids = {}
+
+        def keep(a):
+            a = np.ascontiguousarray(a)
+            self.keep.append(a)
+            return a
+
+        # table templates with host addresses
+        nwords = C.c_uint32(0)
+        abi.check(L.tc_table_templates_host(None, None, None, C.byref(nwords)))
+        words = keep(np.zeros(max(1, nwords.value), np.uint32))
+        recs = keep(np.zeros(4096 * 32, np.uint8))
+        sizes = keep(np.zeros(4096, np.uint32))
+        abi.check(L.tc_table_templates_host(recs.ctypes.data, sizes.ctypes.data, words.ctypes.data, C.byref(nwords)))
+        r = recs.view(np.dtype([("numTriangles", "<u4"), ("numVertices", "<u4"), ("numGroups", "<u4"),
+                                ("clusterID", "<u4"), ("triangles", "<u8"), ("_pad", "<u8")]))
+        r["triangles"] = np.uint64(words.ctypes.data) + r["triangles"] * np.uint64(4)
+        taddr = keep(np.where(r["numTriangles"] > 0,
+                              np.uint64(recs.ctypes.data) + np.arange(4096, dtype=np.uint64) * np.uint64(32),
+                              np.uint64(0)).astype(np.uint64))
+        for li in range(4096):
+            if taddr[li]:
+                self.tmpl_ids[int(taddr[li])] = ("pattern", li)
+        self.table = abi.TessellationTable(0, 0, 0, taddr.ctypes.data, sizes.ctypes.data)
+
+        # geometry with host addresses
+        self.geo_addrs = []
+        for gi, geo in enumerate(host_scene.geometries):
+            w, s = cluster_templates(geo, L)
+            w, s = keep(w), keep(s)
+            rec = keep(template_records(geo, w.ctypes.data))
+            ra = keep(np.uint64(rec.ctypes.data) + np.arange(geo.num_clusters, dtype=np.uint64) * np.uint64(32))
+            for c in range(geo.num_clusters):
+                self.tmpl_ids[int(ra[c])] = ("cluster", gi, c)
+            self.geo_addrs.append({"positions": keep(geo.positions).ctypes.data, "normals": keep(geo.normals).ctypes.data,
+                                   "texcoords": keep(geo.texcoords).ctypes.data, "clusters": keep(geo.clusters).ctypes.data,
+                                   "clusterLocalTriangles": keep(geo.local_triangles).ctypes.data,
+                                   "clusterBboxes": keep(geo.bboxes).ctypes.data, "clusterTemplateAdresses": ra.ctypes.data,
+                                   "clusterTemplateInstantiatonSizes": s.ctypes.data})
+        self.textures = (abi.DisplacementTexture * max(1, len(host_scene.textures)))()
+        for i, (arr, fmt) in enumerate(host_scene.textures):
+            a = keep(arr)
+            self.textures[i].texels = a.ctypes.data
+            self.textures[i].height, self.textures[i].width = a.shape
+            self.textures[i].format = fmt
+        self.instances = (abi.RenderInstance * len(host_scene.instances))()
+        for i, inst in enumerate(host_scene.instances):
+            self.instances[i] = instance_record(inst, host_scene.geometries[inst.geometry],
+                                                self.geo_addrs[inst.geometry])
+        # frame buffers
+        n = len(host_scene.instances)
+        self.buf = {}
+
+        def alloc(name, nbytes):
+            a = keep(np.zeros(max(16, int(nbytes)), np.uint8))
+            self.buf[name] = a
+            return a.ctypes.data
+
+        sb = abi.SceneBuilding()
+        sb.numRenderInstances = n
+        sb.instanceStates = alloc("instanceStates", 4 * n)
+        sb.visibleClusters = alloc("visibleClusters", 8 * cfg.maxVisibleClusters)
+        sb.splitTriangles = alloc("splitTriangles", 24 * cfg.maxSplitTriangles)
+        sb.partTriangles = alloc("partTriangles", 24 * cfg.maxPartTriangles)
+        sb.genVertices = alloc("genVertices", 12 * cfg.maxGeneratedVertices)
+        sb.tempInstanceIDs = alloc("tempInstanceIDs", 4 * cfg.maxGeneratedClusters)
+        sb.tempInstantiations = alloc("tempInstantiations", 32 * cfg.maxGeneratedClusters)
+        sb.tempClusterAddresses = alloc("tempClusterAddresses", 8 * cfg.maxGeneratedClusters)
+        sb.tempClusterSizes = alloc("tempClusterSizes", 4 * cfg.maxGeneratedClusters)
+        sb.blasBuildInfos = alloc("blasBuildInfos", 16 * n)
+        sb.blasBuildSizes = alloc("blasBuildSizes", 4 * n)
+        sb.blasClusterAddresses = alloc("blasClusterAddresses", 8 * cfg.maxGeneratedClusters)
+        self.clas_bytes = min(cfg.maxGeneratedClusterMegs << 20, 1 << 30)
+        sb.genClusterData = alloc("genClusterData", self.clas_bytes)
+        self.sb = sb
+        self.rb = abi.Readback()
+        self.view = abi.FrameConstants()
+        self.ctx = OracleCtx(C.addressof(self.view), C.addressof(self.table), C.addressof(self.rb),
+                             C.addressof(self.instances), C.addressof(self.sb), C.addressof(self.textures), n,
+                             len(host_scene.textures), 0)
+
+    def run(self, fc: abi.FrameConstants, nthreads: int = 1, stages: str = "all"):
+        O = oracle()
+        C.memmove(C.addressof(self.view), C.addressof(fc), C.sizeof(fc))
+        # frame reset (renderer_raytrace_clusters_tess.cpp:412-419)
+        for name, _ in self.sb._fields_:
+            if name not in ADDR_FIELDS and name != "numRenderInstances" and not name.startswith("_"):
+                v = getattr(self.sb, name)
+                if isinstance(v, C.Structure):
+                    C.memset(C.addressof(v), 0, C.sizeof(v))
+                elif isinstance(v, C.Array):
+                    C.memset(C.addressof(v), 0, C.sizeof(v))
+                else:
+                    setattr(self.sb, name, 0)
+        self.sb.viewPos = (abi.f32 * 3)(*fc.viewPos[:3])
+        C.memset(C.addressof(self.rb), 0, C.sizeof(self.rb))
+        self.buf["splitTriangles"][:] = 0xFF
+        ctx = C.byref(self.ctx)
+        O.oracle_instances_classify(ctx, self.cfg)
+        O.oracle_clusters_cull(ctx, self.cfg)
+        if nthreads > 1:
+            O.oracle_frame_tessellate_mt(ctx, self.cfg, 0, nthreads)
+        else:
+            O.oracle_frame_tessellate(ctx, self.cfg, 0)
+        if stages == "tess":
+            return
+        O.oracle_clas_instantiate(ctx, self.cfg)
+        O.oracle_blas_setup_insertion(ctx, self.cfg)
+        O.oracle_blas_clusters_insert(ctx, self.cfg, 0)
+
+    def results(self) -> Results:
+        def fetch(addr, nbytes):
+            return np.frombuffer(C.string_at(addr, nbytes), np.uint8).copy() if nbytes else np.zeros(0, np.uint8)
+
+        return _finish_results(self.sb, self.rb, fetch, self.cfg, lambda a: self.tmpl_ids.get(a, ("unknown", a)))
+
+
+# -------------------------------------------------------------------- GPU
+
+
+class GpuFrame:
+    def __init__(self, host_scene, cfg: abi.Config):
+        from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
+        self.cfg = cfg
+        self.ds = DeviceScene(host_scene)
+        self.r = Renderer(self.ds, cfg)
+        L = abi.lib()
+        tb = self.r.read_struct(abi.TessellationTable, self.r.bindings.tessTable)
+        taddr = self.r.copy(tb.templateAddresses, 8 * 4096).view(np.uint64)
+        self.tmpl_ids = {int(a): ("pattern", li) for li, a in enumerate(taddr) if a}
+        for gi, a in enumerate(self.ds.geo_addrs):
+            geo = host_scene.geometries[gi]
+            for c in range(geo.num_clusters):
+                self.tmpl_ids[int(a["templateRecords"] + 32 * c)] = ("cluster", gi, c)
+
+    def run(self, fc: abi.FrameConstants, stages: str = "all"):
+        self.r.tessellate(fc)
+        if stages != "tess":
+            self.r.clas_and_blas()
+        abi.check(abi.lib().tc_device_synchronize(), "synchronize")
+
+    def results(self) -> Results:
+        sb = self.r.scene_building()
+        rb = self.r.readback()
+        return _finish_results(sb, rb, self.r.copy, self.cfg, lambda a: self.tmpl_ids.get(a, ("unknown", a)))
+
+    def close(self):
+        self.r.close()
+
+
+def ulp_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """|ULP distance| between float32 arrays of equal shape."""
+    ia = a.astype(np.float32).view(np.int32).astype(np.int64)
+    ib = b.astype(np.float32).view(np.int32).astype(np.int64)
+    ia = np.where(ia < 0, np.int64(-2 ** 31) - ia, ia)
+    ib = np.where(ib < 0, np.int64(-2 ** 31) - ib, ib)
+    return np.abs(ia - ib)
+
+
+def compare(gpu: Results, ref: Results, max_ulp: int = 2):
+    """Raises AssertionError naming the first differing output."""
+    assert gpu.counters == ref.counters, _diff_dict("SceneBuilding", gpu.counters, ref.counters)
+    assert gpu.readback == ref.readback, _diff_dict("Readback", gpu.readback, ref.readback)
+    for k in ("instanceStates", "visibleClusters", "splitTriangles", "partTriangles", "tempInstantiations",
+              "tempInstanceIDs", "tempClusterOffsets", "tempClusterSizes", "blasBuildInfos", "blasClusterOffsets"):
+        a, b = gpu.arrays[k], ref.arrays[k]
+        assert a.shape == b.shape, "%s: shape %s vs %s" % (k, a.shape, b.shape)
+        if not np.array_equal(a, b):
+            bad = np.nonzero(a != b)[0]
+            raise AssertionError("%s differs at %d entries, first %d: gpu %s oracle %s" % (
+                k, bad.size, bad[0], a[bad[0]], b[bad[0]]))
+    assert gpu.arrays["templateIdentity"] == ref.arrays["templateIdentity"], "template identities differ"
+    gv, rv = gpu.arrays["genVertices"], ref.arrays["genVertices"]
+    if gv is not None and rv is not None:
+        assert gv.shape == rv.shape, "genVertices shape"
+        d = ulp_diff(gv, rv)
+        assert d.max(initial=0) <= max_ulp, "genVertices differ by %d ULP" % d.max()
+    cb, rbk = gpu.arrays["clasBlocks"], ref.arrays["clasBlocks"]
+    assert cb.shape == rbk.shape, "CLAS blocks size"
+    assert np.array_equal(cb, rbk), "CLAS blocks differ at %d bytes" % int((cb != rbk).sum())
+
+
+def _diff_dict(name, a, b):
+    d = {k: (a.get(k), b.get(k)) for k in set(a) | set(b) if a.get(k) != b.get(k)}
+    return "%s differs: %s" % (name, dict(list(d.items())[:12]))

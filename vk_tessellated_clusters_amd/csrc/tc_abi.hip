@@ -1,0 +1,494 @@
+// tc_abi.hip -- extern "C" entry points of the tessellation stages.
+//
+// Host code only validates arguments, assembles the kernel argument block
+// and enqueues kernels on the caller's stream.  No call synchronises the
+// stream; nothing here allocates except tc_table_upload.
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "tc_sort.h"
+
+namespace tc {
+
+#include "tess_table_nv_data.inc"
+
+constexpr uint32_t kMaxInstances = 1u << 16;
+
+static uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+
+WsLayout ws_layout(const tc_config& cfg, uint32_t) {
+  WsLayout L{};
+  uint32_t tilesSplit = (cfg.maxSplitTriangles + kWave - 1) / kWave;
+  uint32_t tilesPart = (cfg.maxPartTriangles + kWave - 1) / kWave;
+  L.maxTiles = std::max(std::max(cfg.maxVisibleClusters, tilesSplit), std::max(tilesPart, 1u));
+  L.sortCapacity = std::max(cfg.maxGeneratedClusters, 1u);
+  L.sortMaxTiles = (L.sortCapacity + kSortTile - 1) / kSortTile;
+  L.instanceCapacity = kMaxInstances;
+  uint64_t o = kWsHeaderBytes;
+  L.status = o;
+  o = align256(o + (uint64_t)L.maxTiles * kMaxCounters * 8);
+  for (int i = 0; i < 2; i++) {
+    L.sortKeys[i] = o;
+    o = align256(o + (uint64_t)L.sortCapacity * 8);
+  }
+  for (int i = 0; i < 2; i++) {
+    L.sortVals[i] = o;
+    o = align256(o + (uint64_t)L.sortCapacity * 4);
+  }
+  L.sortHist = o;
+  o = align256(o + (uint64_t)kSortRadix * L.sortMaxTiles * 4);
+  L.sortAux = o;
+  o = align256(o + (uint64_t)(kSortRadix + kMaxInstances) * 4);
+  L.cullOffsets = o;
+  o = align256(o + (uint64_t)kMaxInstances * 4);
+  L.total = o;
+  return L;
+}
+
+static int g_numCU = 0;
+static std::once_flag g_cuOnce;
+
+static int num_cu() {
+  std::call_once(g_cuOnce, [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      g_numCU = n;
+    if (g_numCU <= 0) g_numCU = 256;
+  });
+  return g_numCU;
+}
+
+static bool config_ok(const tc_config& c) {
+  if (c.clusterTriangleCount == 0 || c.clusterTriangleCount > kMaxClusterTris) return false;
+  if (c.clusterVertexCount == 0 || c.clusterVertexCount > kMaxClusterVerts) return false;
+  if (c.maxSplitFactor < 2 || c.maxSplitFactor > TC_TESSTABLE_SIZE) return false;
+  if (c.maxVisibleClusters == 0 || c.maxPartTriangles == 0 || c.maxSplitTriangles == 0) return false;
+  if (c.maxGeneratedClusters == 0 || c.maxGeneratedVertices == 0) return false;
+  return true;
+}
+
+static constexpr uint32_t kSupportedFlags = TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN;
+
+static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
+  if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
+  if (!config_ok(cfg)) return TC_ERROR_INVALID_ARGUMENT;
+  if (cfg.flags & ~kSupportedFlags) return TC_ERROR_UNSUPPORTED;
+  if (b->numRenderInstances == 0 || b->numRenderInstances > kMaxInstances) return TC_ERROR_INVALID_ARGUMENT;
+  if ((cfg.flags & TC_FLAG_DISPLACEMENT) && b->numDisplacementTextures && !b->displacementTextures) return TC_ERROR_INVALID_ARGUMENT;
+  WsLayout L = ws_layout(cfg, b->numRenderInstances);
+  if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
+  a.view = b->view;
+  a.tessTable = b->tessTable;
+  a.readback = b->readback;
+  a.instances = b->instances;
+  a.build = b->build;
+  a.textures = b->displacementTextures;
+  a.numInstances = b->numRenderInstances;
+  a.numTextures = (cfg.flags & TC_FLAG_DISPLACEMENT) ? b->numDisplacementTextures : 0;
+  a.ws = reinterpret_cast<WsHeader*>(b->workspace);
+  a.status = reinterpret_cast<uint64_t*>(static_cast<char*>(b->workspace) + L.status);
+  a.maxTiles = L.maxTiles;
+  a._pad = 0;
+  a.cfg = cfg;
+  if (layout) *layout = L;
+  return TC_OK;
+}
+
+static SortBuffers sort_buffers(const tc_bindings* b, const WsLayout& L) {
+  char* w = static_cast<char*>(b->workspace);
+  SortBuffers sb;
+  for (int i = 0; i < 2; i++) {
+    sb.keys[i] = reinterpret_cast<uint64_t*>(w + L.sortKeys[i]);
+    sb.vals[i] = reinterpret_cast<uint32_t*>(w + L.sortVals[i]);
+  }
+  sb.hist = reinterpret_cast<uint32_t*>(w + L.sortHist);
+  sb.digitTotals = reinterpret_cast<uint32_t*>(w + L.sortAux);
+  sb.capacity = L.sortCapacity;
+  sb.maxTiles = L.sortMaxTiles;
+  return sb;
+}
+
+tc_status make_args_public(const tc_bindings* b, const tc_config& cfg, KArgs& a) { return make_args(b, cfg, a); }
+int num_cu_public() { return num_cu(); }
+
+static tc_status launched() { return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH; }
+
+// ------------------------------------------------------ table templates
+
+static uint32_t round16(uint32_t v) { return (v + 15u) & ~15u; }
+
+// 30-bit Morton code of a point in [0,1]^3 (10 bits per axis)
+static uint32_t morton3(float x, float y, float z) {
+  auto q = [](float f) {
+    float c = std::min(std::max(f, 0.0f), 1.0f);
+    uint32_t v = (uint32_t)(c * 1023.0f + 0.5f);
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+  };
+  return q(x) | (q(y) << 1) | (q(z) << 2);
+}
+
+// Orders triangles along a Morton curve of their centroids (normalised to
+// the cluster's bounds) so runs of TC_CLAS_GROUP_TRIANGLES are compact.
+static void order_triangles(uint32_t nt, const uint32_t* words, const float* pos3, uint32_t* out) {
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  std::vector<float> cen(nt * 3);
+  for (uint32_t t = 0; t < nt; t++)
+    for (int k = 0; k < 3; k++) {
+      float c = 0;
+      for (int v = 0; v < 3; v++) c += pos3[((words[t] >> (8 * v)) & 0xFF) * 3 + k];
+      c /= 3.0f;
+      cen[t * 3 + k] = c;
+      lo[k] = std::min(lo[k], c);
+      hi[k] = std::max(hi[k], c);
+    }
+  std::vector<std::pair<uint32_t, uint32_t>> keys(nt);
+  for (uint32_t t = 0; t < nt; t++) {
+    float n[3];
+    for (int k = 0; k < 3; k++) n[k] = hi[k] > lo[k] ? (cen[t * 3 + k] - lo[k]) / (hi[k] - lo[k]) : 0.5f;
+    keys[t] = {morton3(n[0], n[1], n[2]), t};
+  }
+  std::stable_sort(keys.begin(), keys.end(), [](auto& a, auto& b) { return a.first < b.first; });
+  for (uint32_t i = 0; i < nt; i++) out[i] = words[keys[i].second];
+}
+
+struct TableHost {
+  tc_tess_table_entry lookup[TC_TESSTABLE_NUM_LOOKUP];
+  std::vector<uint32_t> templWords;                      // all template triangle words
+  std::vector<tc_clas_template> templ;                   // per lookup index (device addresses patched)
+  std::vector<uint32_t> templWordOffset;                 // per lookup index
+  uint32_t sizes[TC_TESSTABLE_NUM_LOOKUP];
+
+  TableHost() {
+    memset(lookup, 0, sizeof(lookup));
+    memset(sizes, 0, sizeof(sizes));
+    // tessellation_table.cpp:53-80
+    uint32_t idx = 0;
+    for (uint32_t x = 1; x <= TC_TABLE_MAX_EDGE_SEGMENTS; x++)
+      for (uint32_t y = 1; y <= x; y++)
+        for (uint32_t z = 1; z <= y; z++, idx++) {
+          tc_tess_table_entry e{tc_table_nv_configs[idx * 4 + 0], tc_table_nv_configs[idx * 4 + 1], tc_table_nv_configs[idx * 4 + 2],
+                                tc_table_nv_configs[idx * 4 + 3]};
+          lookup[x + y * 16 + z * 256 - 273] = e;
+          if (z != y && x > 1) lookup[x + z * 16 + y * 256 - 273] = e;
+        }
+    // per-pattern templates: one per used lookup entry; the (x,z,y) alias
+    // (z > y) is the flipped pattern (tessellation_table.cpp:116-153)
+    templ.assign(TC_TESSTABLE_NUM_LOOKUP, tc_clas_template{});
+    templWordOffset.assign(TC_TESSTABLE_NUM_LOOKUP, 0);
+    for (uint32_t li = 0; li < TC_TESSTABLE_NUM_LOOKUP; li++) {
+      const tc_tess_table_entry& e = lookup[li];
+      if (e.numTriangles == 0) continue;
+      uint32_t f = li + 273;
+      uint32_t fy = (f >> 4) & 15, fz = (f >> 8) & 15;
+      bool flipped = fz > fy;
+      std::vector<float> uv(e.numVertices * 3);
+      for (uint32_t v = 0; v < e.numVertices; v++) {
+        uint32_t raw = tc_table_nv_vertices[e.firstVertex + v];
+        float u = float(raw & 0xFFFF) / 32768.0f, w = float(raw >> 16) / 32768.0f;
+        float x = (1.0f - u) - w;
+        if (flipped) std::swap(x, u);  // barycentric (w,u,v) -> (u,w,v)
+        uv[v * 3 + 0] = u;
+        uv[v * 3 + 1] = w;
+        uv[v * 3 + 2] = 0.0f;
+      }
+      std::vector<uint32_t> words(e.numTriangles), ordered(e.numTriangles);
+      for (uint32_t t = 0; t < e.numTriangles; t++) {
+        uint32_t packed = tc_table_nv_triangles[e.firstTriangle + t];
+        uint32_t i0 = packed & 0xFF, i1 = (packed >> 8) & 0xFF, i2 = (packed >> 16) & 0xFF;
+        if (flipped) std::swap(i1, i2);
+        words[t] = i0 | (i1 << 8) | (i2 << 16) | (t << 24);
+      }
+      order_triangles(e.numTriangles, words.data(), uv.data(), ordered.data());
+      templWordOffset[li] = (uint32_t)templWords.size();
+      templWords.insert(templWords.end(), ordered.begin(), ordered.end());
+      tc_clas_template& T = templ[li];
+      T.numTriangles = e.numTriangles;
+      T.numVertices = e.numVertices;
+      T.numGroups = (e.numTriangles + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+      T.clusterID = 0;
+      sizes[li] = tc_clas_size(e.numTriangles, e.numVertices);
+    }
+  }
+};
+
+static const TableHost& table_host() {
+  static TableHost t;
+  return t;
+}
+
+// device allocation bundle behind one uploaded table
+struct TableDev {
+  tc_tessellation_table hdr;
+  void* mem;
+};
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" {
+
+tc_status tc_device_check(int* deviceCount) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    if (deviceCount) *deviceCount = 0;
+    return TC_ERROR_NO_DEVICE;
+  }
+  if (deviceCount) *deviceCount = n;
+  return TC_OK;
+}
+
+uint64_t tc_workspace_size(const tc_config* config) {
+  if (!config || !config_ok(*config)) return 0;
+  return ws_layout(*config, 1).total;
+}
+
+tc_status tc_workspace_init(const tc_bindings* b, const tc_config* config, tc_stream stream) {
+  if (!b || !config || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
+  if (!config_ok(*config)) return TC_ERROR_INVALID_ARGUMENT;
+  WsLayout L = ws_layout(*config, 1);
+  if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
+  if (hipMemsetAsync(b->workspace, 0, L.total, (hipStream_t)stream) != hipSuccess) return TC_ERROR_LAUNCH;
+  return TC_OK;
+}
+
+uint32_t tc_clas_size(uint32_t numTriangles, uint32_t numVertices) {
+  uint32_t groups = (numTriangles + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+  return (uint32_t)sizeof(tc_clas_header) + round16(groups * (uint32_t)sizeof(tc_aabb)) + round16(numTriangles * 4u) +
+         round16(numVertices * 12u);
+}
+
+tc_status tc_cluster_template_order(uint32_t numTriangles, uint32_t numVertices, const uint8_t* localTriangles,
+                                    const float* positions, uint32_t* trianglesOut) {
+  if (!localTriangles || !positions || !trianglesOut || numTriangles == 0 || numTriangles > 256 || numVertices > 256)
+    return TC_ERROR_INVALID_ARGUMENT;
+  std::vector<uint32_t> words(numTriangles);
+  for (uint32_t t = 0; t < numTriangles; t++) {
+    uint32_t i0 = localTriangles[t * 3], i1 = localTriangles[t * 3 + 1], i2 = localTriangles[t * 3 + 2];
+    if (i0 >= numVertices || i1 >= numVertices || i2 >= numVertices) return TC_ERROR_INVALID_ARGUMENT;
+    words[t] = i0 | (i1 << 8) | (i2 << 16) | ((t & 0xFF) << 24);
+  }
+  order_triangles(numTriangles, words.data(), positions, trianglesOut);
+  return TC_OK;
+}
+
+tc_status tc_table_lookup_host(tc_tess_table_entry* out4096) {
+  if (!out4096) return TC_ERROR_INVALID_ARGUMENT;
+  memcpy(out4096, table_host().lookup, sizeof(table_host().lookup));
+  return TC_OK;
+}
+
+tc_status tc_table_templates_host(tc_clas_template* out4096, uint32_t* sizes4096, uint32_t* words, uint32_t* numWords) {
+  const TableHost& H = table_host();
+  if (numWords) {
+    if (words && *numWords < H.templWords.size()) return TC_ERROR_INVALID_ARGUMENT;
+    *numWords = (uint32_t)H.templWords.size();
+  }
+  if (words) memcpy(words, H.templWords.data(), H.templWords.size() * 4);
+  if (out4096)
+    for (uint32_t li = 0; li < TC_TESSTABLE_NUM_LOOKUP; li++) {
+      out4096[li] = H.templ[li];
+      out4096[li].triangles = H.templWordOffset[li];
+    }
+  if (sizes4096) memcpy(sizes4096, H.sizes, sizeof(H.sizes));
+  return TC_OK;
+}
+
+tc_status tc_memcpy(void* dst, const void* src, uint64_t bytes) {
+  if ((!dst || !src) && bytes) return TC_ERROR_INVALID_ARGUMENT;
+  if (!bytes) return TC_OK;
+  return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+tc_status tc_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH; }
+
+tc_status tc_table_upload(tc_tessellation_table** tableDev) {
+  if (!tableDev) return TC_ERROR_INVALID_ARGUMENT;
+  const TableHost& H = table_host();
+  // one allocation: header | vertices | triangles | lookup | templates |
+  // template words | template addresses | sizes
+  uint64_t o = 256;
+  uint64_t oV = o;
+  o = align256(o + sizeof(tc_table_nv_vertices));
+  uint64_t oT = o;
+  o = align256(o + sizeof(tc_table_nv_triangles));
+  uint64_t oL = o;
+  o = align256(o + sizeof(H.lookup));
+  uint64_t oTm = o;
+  o = align256(o + H.templ.size() * sizeof(tc_clas_template));
+  uint64_t oW = o;
+  o = align256(o + H.templWords.size() * 4);
+  uint64_t oA = o;
+  o = align256(o + TC_TESSTABLE_NUM_LOOKUP * 8);
+  uint64_t oS = o;
+  o = align256(o + sizeof(H.sizes));
+  void* mem = nullptr;
+  if (hipMalloc(&mem, o) != hipSuccess) return TC_ERROR_OUT_OF_MEMORY;
+  uint64_t base = reinterpret_cast<uint64_t>(mem);
+  std::vector<char> host(o, 0);
+  tc_tessellation_table hdr;
+  hdr.vertices = base + oV;
+  hdr.triangles = base + oT;
+  hdr.entries = base + oL;
+  hdr.templateAddresses = base + oA;
+  hdr.templateInstantiationSizes = base + oS;
+  memcpy(host.data(), &hdr, sizeof(hdr));
+  memcpy(host.data() + oV, tc_table_nv_vertices, sizeof(tc_table_nv_vertices));
+  memcpy(host.data() + oT, tc_table_nv_triangles, sizeof(tc_table_nv_triangles));
+  memcpy(host.data() + oL, H.lookup, sizeof(H.lookup));
+  std::vector<tc_clas_template> templ = H.templ;
+  std::vector<uint64_t> addrs(TC_TESSTABLE_NUM_LOOKUP, 0);
+  for (uint32_t li = 0; li < TC_TESSTABLE_NUM_LOOKUP; li++) {
+    if (templ[li].numTriangles == 0) continue;
+    templ[li].triangles = base + oW + (uint64_t)H.templWordOffset[li] * 4;
+    addrs[li] = base + oTm + (uint64_t)li * sizeof(tc_clas_template);
+  }
+  memcpy(host.data() + oTm, templ.data(), templ.size() * sizeof(tc_clas_template));
+  memcpy(host.data() + oW, H.templWords.data(), H.templWords.size() * 4);
+  memcpy(host.data() + oA, addrs.data(), addrs.size() * 8);
+  memcpy(host.data() + oS, H.sizes, sizeof(H.sizes));
+  if (hipMemcpy(mem, host.data(), o, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(mem);
+    return TC_ERROR_LAUNCH;
+  }
+  *tableDev = reinterpret_cast<tc_tessellation_table*>(mem);
+  return TC_OK;
+}
+
+tc_status tc_table_free(tc_tessellation_table* tableDev) {
+  if (!tableDev) return TC_ERROR_INVALID_ARGUMENT;
+  return hipFree(tableDev) == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+uint32_t tc_num_split_passes(uint32_t splitFactor) {
+  // renderer_raytrace_clusters_tess.cpp:516-520
+  uint32_t coord = TC_TESSTABLE_COORD_MAX, passes = 0;
+  splitFactor = std::max(2u, std::min(splitFactor, TC_TESSTABLE_SIZE));
+  while (coord > splitFactor) {
+    coord /= splitFactor;
+    passes++;
+  }
+  return passes;
+}
+
+tc_status tc_build_setup(const tc_bindings* b, tc_config config, int mode, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  if (mode < TC_BUILD_SETUP_CLASSIFY || mode > TC_BUILD_SETUP_BUILD_BLAS) return TC_ERROR_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(k_build_setup, dim3(1), dim3(1), 0, (hipStream_t)stream, a, mode);
+  return launched();
+}
+
+tc_status tc_instances_classify(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  hipLaunchKernelGGL(k_instances_classify, dim3((a.numInstances + 63) / 64), dim3(64), 0, (hipStream_t)stream, a);
+  return launched();
+}
+
+tc_status tc_clusters_cull(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  WsLayout L;
+  tc_status s = make_args(b, config, a, &L);
+  if (s != TC_OK) return s;
+  uint32_t* offsets = reinterpret_cast<uint32_t*>(static_cast<char*>(b->workspace) + L.cullOffsets);
+  hipLaunchKernelGGL(k_clusters_cull_scan, dim3(1), dim3(1), 0, (hipStream_t)stream, a, offsets);
+  hipLaunchKernelGGL(k_clusters_cull_fill, dim3(a.numInstances), dim3(256), 0, (hipStream_t)stream, a, (const uint32_t*)offsets);
+  return launched();
+}
+
+tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)0);
+  uint32_t grid = std::min<uint32_t>(config.maxVisibleClusters, (uint32_t)num_cu() * 16);
+  hipLaunchKernelGGL(k_cluster_classify, dim3(grid), dim3(64), 0, st, a);
+  return launched();
+}
+
+tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)1);
+  uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 8);
+  hipLaunchKernelGGL(k_triangle_split, dim3(grid), dim3(64), 0, st, a);
+  return launched();
+}
+
+tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)2);
+  uint32_t tiles = (config.maxPartTriangles + kWave - 1) / kWave;
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 16);
+  if (config.flags & TC_FLAG_PN)
+    hipLaunchKernelGGL(k_instantiate<true>, dim3(grid), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_instantiate<false>, dim3(grid), dim3(64), 0, st, a);
+  return launched();
+}
+
+tc_status tc_blas_setup_insertion(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  hipLaunchKernelGGL(k_blas_setup_insertion, dim3(1), dim3(1024), 0, (hipStream_t)stream, a);
+  return launched();
+}
+
+tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int transient, tc_stream stream) {
+  KArgs a;
+  WsLayout L;
+  tc_status s = make_args(b, config, a, &L);
+  if (s != TC_OK) return s;
+  if (transient) return TC_OK;  // no transient builds in this configuration
+  hipStream_t st = (hipStream_t)stream;
+  SortBuffers sb = sort_buffers(b, L);
+  uint32_t* segStart = sb.digitTotals + kSortRadix;
+  uint32_t grid = std::min<uint32_t>((sb.capacity + 255) / 256, (uint32_t)num_cu() * 4);
+  hipLaunchKernelGGL(k_insert_keys, dim3(grid), dim3(256), 0, st, a, sb.keys[0], sb.vals[0], sb.capacity);
+  uint32_t bits = 1;
+  while ((1u << bits) < a.numInstances) bits++;
+  uint32_t res = 0;
+  radix_sort_pairs(sb, &b->build->tempInstantiateCounter, 0, bits, st, &res);
+  hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart, sb.capacity);
+  hipLaunchKernelGGL(k_insert_scatter, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res],
+                     (const uint32_t*)sb.vals[res], (const uint32_t*)segStart, sb.capacity);
+  return launched();
+}
+
+tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t numSplitPasses, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  if (numSplitPasses == 0) numSplitPasses = tc_num_split_passes(config.maxSplitFactor);
+  // renderer_raytrace_clusters_tess.cpp:467-582
+  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_CLASSIFY, stream)) != TC_OK) return s;
+  if ((s = tc_cluster_classify(b, config, stream)) != TC_OK) return s;
+  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT, stream)) != TC_OK) return s;
+  for (uint32_t p = 0; p < numSplitPasses; p++) {
+    if ((s = tc_triangle_split(b, config, stream)) != TC_OK) return s;
+    if (p + 1 < numSplitPasses && (s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT_PASS, stream)) != TC_OK) return s;
+  }
+  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_INSTANTIATE_TESS, stream)) != TC_OK) return s;
+  if ((s = tc_triangle_tess_template_instantiate(b, config, stream)) != TC_OK) return s;
+  return tc_build_setup(b, config, TC_BUILD_SETUP_BUILD_BLAS, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// tc_clas.hip -- cluster acceleration structure (CLAS) instantiation.
+//
+// Replaces vkCmdBuildClusterAccelerationStructureIndirectNV in template
+// instantiation mode (renderer_raytrace_clusters_tess.cpp:595-623): every
+// TemplateInstantiateInfo names a template (triangle order of a pattern or
+// a base cluster) and a vertex range; the result at tempClusterAddresses[i]
+// is a self-contained block
+//   tc_clas_header | group boxes | triangle words | vertices
+// and tempClusterSizes[i] receives its size.
+#include <algorithm>
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t r16(uint32_t v) { return (v + 15u) & ~15u; }
+
+// One wave per CLAS, grid-stride over tempInstantiateCounter.
+__global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
+  __shared__ float s_v[kMaxClusterVerts * 3];
+  __shared__ uint32_t s_t[kMaxClusterTris];
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->tempInstantiateCounter, a.cfg.maxGeneratedClusters);
+  const uint32_t lane = lane_id();
+  const tc_template_instantiate_info* infos = ptr<const tc_template_instantiate_info>(b->tempInstantiations);
+  const uint64_t* dsts = ptr<const uint64_t>(b->tempClusterAddresses);
+  uint32_t* sizes = ptr<uint32_t>(b->tempClusterSizes);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const tc_template_instantiate_info info = infos[i];
+    const tc_clas_template T = *ptr<const tc_clas_template>(info.clusterTemplateAddress);
+    const uint32_t nt = T.numTriangles, nv = T.numVertices, ng = T.numGroups;
+    char* dst = ptr<char>(dsts[i]);
+    tc_aabb* groups = reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header));
+    uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
+    float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
+    const char* src = ptr<const char>(info.vertexBufferAddress);
+    for (uint32_t k = lane; k < nv * 3; k += kWave) {
+      const uint32_t v = k / 3, c = k - v * 3;
+      const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
+      s_v[k] = f;
+      verts[k] = f;
+    }
+    for (uint32_t t = lane; t < nt; t += kWave) {
+      const uint32_t w = ptr<const uint32_t>(T.triangles)[t];
+      s_t[t] = w;
+      tris[t] = w;
+    }
+    __syncthreads();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t g = lane; g < ng; g += kWave) {
+      float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
+      for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
+        const uint32_t w = s_t[t];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const uint32_t vi = (w >> (8 * k)) & 0xFF;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            glo[c] = fminf(glo[c], s_v[vi * 3 + c]);
+            ghi[c] = fmaxf(ghi[c], s_v[vi * 3 + c]);
+          }
+        }
+      }
+      tc_aabb box;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        box.lo[c] = glo[c];
+        box.hi[c] = ghi[c];
+        lo[c] = fminf(lo[c], glo[c]);
+        hi[c] = fmaxf(hi[c], ghi[c]);
+      }
+      groups[g] = box;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      lo[c] = wave_min(lo[c]);
+      hi[c] = wave_max(hi[c]);
+    }
+    if (lane == 0) {
+      tc_clas_header h;
+      h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
+      h.hi[0] = hi[0], h.hi[1] = hi[1], h.hi[2] = hi[2];
+      h.clusterID = T.clusterID + info.clusterIdOffset;
+      h.counts = nt | (nv << 8) | (ng << 16);
+      *reinterpret_cast<tc_clas_header*>(dst) = h;
+      sizes[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args_public(b, config, a);
+  if (s != TC_OK) return s;
+  uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * 16);
+  hipLaunchKernelGGL(k_clas_instantiate, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}

@@ -1,0 +1,464 @@
+// tc_device.h -- device-side helpers shared by the gfx950 kernels.
+//
+// Floating-point evaluation order is fixed explicitly (files are built with
+// -ffp-contract=off and correctly rounded fp32 division / sqrt) so integer
+// decisions taken from float math (tessellation factors, barycentric
+// encoding) and generated positions are reproducible bit for bit:
+//   dot       (a.x*b.x + a.y*b.y) + a.z*b.z
+//   length    sqrtf(dot(v, v))
+//   normalize v * (1.0f / length(v))
+//   mat4*vec4 column-major, summed column by column
+//   round     roundf (half away from zero)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tc_abi.h"
+
+#pragma clang fp contract(off)
+
+namespace tc {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ math
+
+struct v3 {
+  float x, y, z;
+};
+struct v2 {
+  float x, y;
+};
+
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 mul(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float length(v3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float distance(v3 a, v3 b) { return length(sub(a, b)); }
+__device__ __forceinline__ v3 normalize(v3 a) { return mul(a, 1.0f / length(a)); }
+
+__device__ __forceinline__ v3 xform_point(const float* m, v3 p) {
+  v3 r;
+  r.x = ((m[0] * p.x + m[4] * p.y) + m[8] * p.z) + m[12];
+  r.y = ((m[1] * p.x + m[5] * p.y) + m[9] * p.z) + m[13];
+  r.z = ((m[2] * p.x + m[6] * p.y) + m[10] * p.z) + m[14];
+  return r;
+}
+
+// barycentric interpolation, tessellation.glsl:191-201 (precise)
+__device__ __forceinline__ v3 interp(v3 b0, v3 b1, v3 b2, v3 w) {
+  return mk((b0.x * w.x + b1.x * w.y) + b2.x * w.z, (b0.y * w.x + b1.y * w.y) + b2.y * w.z,
+            (b0.z * w.x + b1.z * w.y) + b2.z * w.z);
+}
+__device__ __forceinline__ v2 interp2(v2 b0, v2 b1, v2 b2, v3 w) {
+  return v2{(b0.x * w.x + b1.x * w.y) + b2.x * w.z, (b0.y * w.x + b1.y * w.y) + b2.y * w.z};
+}
+
+__device__ __forceinline__ uint32_t umax3(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
+
+template <class T>
+__device__ __forceinline__ T* ptr(uint64_t a) {
+  return reinterpret_cast<T*>(a);
+}
+
+__device__ __forceinline__ v3 ld3(uint64_t base, uint32_t i) {
+  const float* f = ptr<const float>(base) + 3ull * i;
+  return mk(f[0], f[1], f[2]);
+}
+__device__ __forceinline__ v2 ld2(uint64_t base, uint32_t i) {
+  const float2 f = ptr<const float2>(base)[i];
+  return v2{f.x, f.y};
+}
+__device__ __forceinline__ void st3(uint64_t base, uint64_t i, v3 p) {
+  float* f = ptr<float>(base) + 3ull * i;
+  f[0] = p.x;
+  f[1] = p.y;
+  f[2] = p.z;
+}
+
+// --------------------------------------------------- tessellation.glsl
+
+// tessellation.glsl:48-59
+__device__ __forceinline__ uint32_t encode_bary(v3 w) {
+  uint32_t ix = (uint32_t)(w.x * 32768.0f + 0.5f);
+  uint32_t iy = (uint32_t)(w.y * 32768.0f + 0.5f);
+  uint32_t iz = (uint32_t)(w.z * 32768.0f + 0.5f);
+  if (ix > max(iy, iz))
+    ix = 32768u - iy - iz;
+  else if (iy > iz)
+    iy = 32768u - ix - iz;
+  else
+    iz = 32768u - ix - iy;
+  return iy | (iz << 16);
+}
+
+// tessellation.glsl:66-76
+__device__ __forceinline__ v3 decode_bary(uint32_t vtx) {
+  v3 w;
+  w.y = float(vtx & 0xFFFF) / 32768.0f;
+  w.z = float(vtx >> 16) / 32768.0f;
+  w.x = (1.0f - w.y) - w.z;
+  return w;
+}
+
+struct ViewParams {
+  v3 eye;
+  float nearPlane;
+  float vpy;
+  float rate;
+};
+
+// tessellation.glsl:78-99, returned as integers
+__device__ __forceinline__ void tess_factors(v3 a, v3 b, v3 c, const ViewParams& vp, uint32_t f[3]) {
+  float dA = distance(a, vp.eye), dB = distance(b, vp.eye), dC = distance(c, vp.eye);
+  float sx = 1.0f / fmaxf(vp.nearPlane, fminf(dA, dB));
+  float sy = 1.0f / fmaxf(vp.nearPlane, fminf(dB, dC));
+  float sz = 1.0f / fmaxf(vp.nearPlane, fminf(dC, dA));
+  float lx = distance(a, b), ly = distance(b, c), lz = distance(c, a);
+  float fx = roundf(((lx * sx) * vp.vpy) * vp.rate);
+  float fy = roundf(((ly * sy) * vp.vpy) * vp.rate);
+  float fz = roundf(((lz * sz) * vp.vpy) * vp.rate);
+  f[0] = (uint32_t)fminf(fmaxf(fx, 1.0f), 32768.0f);
+  f[1] = (uint32_t)fminf(fmaxf(fy, 1.0f), 32768.0f);
+  f[2] = (uint32_t)fminf(fmaxf(fz, 1.0f), 32768.0f);
+}
+
+// tessellation.glsl:101-104
+__device__ __forceinline__ uint32_t split_factor(uint32_t f, uint32_t maxSplit) { return min((f + 10u) / 11u, maxSplit); }
+
+// tessellation.glsl:119-144
+__device__ __forceinline__ uint32_t get_config(uint32_t f[3], uint32_t vtx[3]) {
+  uint32_t m = umax3(f[0], f[1], f[2]);
+  if (m == f[1]) {
+    uint32_t t0 = f[0], v0 = vtx[0];
+    f[0] = f[1], f[1] = f[2], f[2] = t0;
+    vtx[0] = vtx[1], vtx[1] = vtx[2], vtx[2] = v0;
+  } else if (m == f[2]) {
+    uint32_t t2 = f[2], v2 = vtx[2];
+    f[2] = f[1], f[1] = f[0], f[0] = t2;
+    vtx[2] = vtx[1], vtx[1] = vtx[0], vtx[0] = v2;
+  }
+  uint32_t idx = f[0] + f[1] * 16u + f[2] * 256u - 273u;
+  if (f[2] > f[1]) idx |= 1u << 15;
+  return idx;
+}
+
+__device__ __forceinline__ uint32_t cfg_index(uint32_t cfg) { return cfg & 0x7FFFu; }
+__device__ __forceinline__ bool cfg_flipped(uint32_t cfg) { return (cfg & 0x8000u) != 0; }
+
+// Table view held in registers (wave-uniform addresses).
+struct TableView {
+  const uint32_t* vertices;
+  const uint32_t* triangles;
+  const tc_tess_table_entry* entries;
+  const uint64_t* templateAddresses;
+  const uint32_t* templateSizes;
+};
+
+__device__ __forceinline__ TableView load_table(const tc_tessellation_table* t) {
+  TableView v;
+  v.vertices = ptr<const uint32_t>(t->vertices);
+  v.triangles = ptr<const uint32_t>(t->triangles);
+  v.entries = ptr<const tc_tess_table_entry>(t->entries);
+  v.templateAddresses = ptr<const uint64_t>(t->templateAddresses);
+  v.templateSizes = ptr<const uint32_t>(t->templateInstantiationSizes);
+  return v;
+}
+
+__device__ __forceinline__ tc_tess_table_entry cfg_entry(const TableView& T, uint32_t cfg) {
+  // one 8-byte load
+  uint2 raw = reinterpret_cast<const uint2*>(T.entries)[cfg_index(cfg)];
+  tc_tess_table_entry e;
+  e.firstTriangle = (uint16_t)(raw.x & 0xFFFF);
+  e.firstVertex = (uint16_t)(raw.x >> 16);
+  e.numTriangles = (uint16_t)(raw.y & 0xFFFF);
+  e.numVertices = (uint16_t)(raw.y >> 16);
+  return e;
+}
+
+// tessellation.glsl:165-176
+__device__ __forceinline__ void cfg_tri_vertices(const TableView& T, uint32_t cfg, uint32_t firstTriangle, uint32_t tri,
+                                                 uint32_t out[3]) {
+  uint32_t packed = T.triangles[firstTriangle + tri];
+  out[0] = packed & 0xFF;
+  uint32_t i1 = (packed >> 8) & 0xFF, i2 = (packed >> 16) & 0xFF;
+  bool fl = cfg_flipped(cfg);
+  out[1] = fl ? i2 : i1;
+  out[2] = fl ? i1 : i2;
+}
+
+// tessellation.glsl:178-189
+__device__ __forceinline__ v3 cfg_vertex_bary(const TableView& T, uint32_t cfg, uint32_t firstVertex, uint32_t vert) {
+  v3 w = decode_bary(T.vertices[firstVertex + vert]);
+  if (cfg_flipped(cfg)) {
+    float t = w.x;
+    w.x = w.y;
+    w.y = t;
+  }
+  return w;
+}
+
+// --------------------------------------------------------- displacement
+
+__device__ __forceinline__ float texel(const tc_displacement_texture& t, uint32_t x, uint32_t y) {
+  uint64_t i = (uint64_t)y * t.width + x;
+  if (t.format == TC_TEXTURE_R16_UNORM) return float(ptr<const uint16_t>(t.texels)[i]) / 65535.0f;
+  return ptr<const float>(t.texels)[i];
+}
+
+__device__ __forceinline__ int32_t wrapi(int32_t i, int32_t n) {
+  int32_t r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// FP32 bilinear with REPEAT addressing; stands for texture() with a linear
+// sampler (cluster_classify.comp.glsl:477).
+__device__ __forceinline__ float sample_height(const tc_displacement_texture& t, float u, float v) {
+  float x = u * float(t.width) - 0.5f;
+  float y = v * float(t.height) - 0.5f;
+  x = fminf(fmaxf(x, -16777216.0f), 16777216.0f);
+  y = fminf(fmaxf(y, -16777216.0f), 16777216.0f);
+  float x0 = floorf(x), y0 = floorf(y);
+  float fx = x - x0, fy = y - y0;
+  int32_t ix = (int32_t)x0, iy = (int32_t)y0;
+  int32_t W = (int32_t)t.width, H = (int32_t)t.height;
+  uint32_t xa = (uint32_t)wrapi(ix, W), xb = (uint32_t)wrapi(ix + 1, W);
+  uint32_t ya = (uint32_t)wrapi(iy, H), yb = (uint32_t)wrapi(iy + 1, H);
+  float t00 = texel(t, xa, ya), t10 = texel(t, xb, ya);
+  float t01 = texel(t, xa, yb), t11 = texel(t, xb, yb);
+  float top = t00 * (1.0f - fx) + t10 * fx;
+  float bot = t01 * (1.0f - fx) + t11 * fx;
+  return top * (1.0f - fy) + bot * fy;
+}
+
+// displacement.glsl:30-104 (PN triangle), pow written as products
+struct PNBasis {
+  v3 b030, b003, b300, b021, b012, b102, b201, b210, b120, b111;
+};
+
+__device__ __forceinline__ v3 project_to_plane(v3 p, v3 plane, v3 n) {
+  v3 d = sub(p, plane);
+  return sub(p, mul(n, dot(d, n)));
+}
+__device__ __forceinline__ v3 div3(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ v3 mul2div3(v3 a) { return mk((a.x * 2.0f) / 3.0f, (a.y * 2.0f) / 3.0f, (a.z * 2.0f) / 3.0f); }
+
+__device__ __forceinline__ void setup_pn(PNBasis& b, v3 p0, v3 p1, v3 p2, v3 n0, v3 n1, v3 n2) {
+  b.b030 = p0;
+  b.b003 = p1;
+  b.b300 = p2;
+  v3 e300 = sub(b.b003, b.b030);
+  v3 e030 = sub(b.b300, b.b003);
+  v3 e003 = sub(b.b030, b.b300);
+  b.b021 = add(b.b030, div3(e300, 3.0f));
+  b.b012 = add(b.b030, mul2div3(e300));
+  b.b102 = add(b.b003, div3(e030, 3.0f));
+  b.b201 = add(b.b003, mul2div3(e030));
+  b.b210 = add(b.b300, div3(e003, 3.0f));
+  b.b120 = add(b.b300, mul2div3(e003));
+  b.b021 = project_to_plane(b.b021, b.b030, n0);
+  b.b012 = project_to_plane(b.b012, b.b003, n1);
+  b.b102 = project_to_plane(b.b102, b.b003, n1);
+  b.b201 = project_to_plane(b.b201, b.b300, n2);
+  b.b210 = project_to_plane(b.b210, b.b300, n2);
+  b.b120 = project_to_plane(b.b120, b.b030, n0);
+  v3 center = div3(add(add(b.b003, b.b030), b.b300), 3.0f);
+  v3 s6 = add(add(add(add(add(b.b021, b.b012), b.b102), b.b201), b.b210), b.b120);
+  b.b111 = div3(s6, 6.0f);
+  b.b111 = add(b.b111, div3(sub(b.b111, center), 2.0f));
+}
+
+__device__ __forceinline__ v3 get_pn(const PNBasis& b, v3 bary) {
+  float u = bary.x, v = bary.y, w = bary.z;
+  float u3 = u * u * u, v3_ = v * v * v, w3 = w * w * w;
+  float u2 = u * u, v2_ = v * v, w2 = w * w;
+  v3 r = mul(b.b300, w3);
+  r = add(r, mul(b.b030, u3));
+  r = add(r, mul(b.b003, v3_));
+  r = add(r, mul(b.b210, (3.0f * w2) * u));
+  r = add(r, mul(b.b120, (3.0f * w) * u2));
+  r = add(r, mul(b.b201, (3.0f * w2) * v));
+  r = add(r, mul(b.b021, (3.0f * u2) * v));
+  r = add(r, mul(b.b102, (3.0f * w) * v2_));
+  r = add(r, mul(b.b012, (3.0f * u) * v2_));
+  r = add(r, mul(b.b111, ((6.0f * w) * u) * v));
+  return r;
+}
+
+// ------------------------------------------------------------ wave ops
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ uint32_t mask_prefix(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t shfl(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src, kWave); }
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+  uint32_t lo = shfl((uint32_t)v, src), hi = shfl((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// inclusive wave scan (Hillis-Steele over 64 lanes)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t o = (uint32_t)__shfl_up((int)v, d, kWave);
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+  uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, kWave);
+    uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, kWave);
+    uint64_t o = ((uint64_t)hi << 32) | lo;
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d, kWave);
+    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, kWave);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+  return v;
+}
+
+// ------------------------------------------------ workspace + look-back
+
+// Status word of one tile and counter:
+//   [63:44] epoch  [43:42] flag (1 aggregate, 2 inclusive)  [41:0] value
+constexpr uint32_t kEpochBits = 20;
+constexpr uint64_t kValueMask = (1ull << 42) - 1;
+constexpr uint32_t kMaxCounters = 5;
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint64_t value) {
+  return ((uint64_t)epoch << 44) | ((uint64_t)flag << 42) | (value & kValueMask);
+}
+
+// Workspace header; the status words follow it.
+struct WsHeader {
+  uint32_t epoch;     // current look-back epoch (1 .. 2^20-1)
+  uint32_t ticket;    // next tile ticket
+  uint32_t numTiles;  // tiles of the current launch
+  uint32_t itemBegin; // first item of the current launch
+  uint32_t itemEnd;   // end of the items of the current launch
+  uint32_t _pad[3];
+  uint64_t base[8];   // counter values at launch start
+};
+constexpr uint64_t kWsHeaderBytes = 256;
+
+__device__ __forceinline__ uint64_t atomic_load_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Takes the next tile ticket (whole wave; returns a wave-uniform value).
+__device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
+  uint32_t t = 0;
+  if (lane_id() == 0) t = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return uniform(t);
+}
+
+// Decoupled look-back over N counters for `tile`, executed by one full
+// wave.  agg[] is this tile's wave-uniform aggregate; returns the exclusive
+// prefix of every counter over tiles 0..tile-1 of this launch.  Tiles are
+// handed out in ticket order, so every predecessor is held by a running
+// wave that publishes its aggregate before it waits on anyone; the wait
+// therefore ends.  Status words carry the launch epoch, so words from
+// earlier launches read as "not yet published".
+template <int N>
+__device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
+                                         const uint64_t (&agg)[N], uint64_t (&excl)[N]) {
+  const uint32_t lane = lane_id();
+  if (tile == 0) {
+    if (lane < (uint32_t)N) atomic_store_agent(&status[lane * (uint64_t)maxTiles], pack_status(epoch, 2, agg[lane]));
+#pragma unroll
+    for (int c = 0; c < N; c++) excl[c] = 0;
+    return;
+  }
+  if (lane < (uint32_t)N) atomic_store_agent(&status[lane * (uint64_t)maxTiles + tile], pack_status(epoch, 1, agg[lane]));
+
+  uint64_t sum[N];
+  bool done[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) sum[c] = 0, done[c] = false;
+
+  int64_t pred = (int64_t)tile - 1 - (int64_t)lane;
+  while (true) {
+    uint64_t w[N];
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      if (pred >= 0) {
+        w[c] = atomic_load_agent(&status[c * (uint64_t)maxTiles + (uint64_t)pred]);
+        ok = ok && ((uint32_t)(w[c] >> 44) == epoch) && ((w[c] >> 42) & 3u) != 0;
+      } else {
+        w[c] = pack_status(epoch, 2, 0);  // before tile 0: inclusive zero
+      }
+    }
+    if (ballot(!ok) != 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      if (done[c]) continue;
+      uint64_t incl = ballot(((w[c] >> 42) & 3u) == 2);
+      uint64_t val = w[c] & kValueMask;
+      if (incl) {
+        uint32_t first = (uint32_t)__builtin_ctzll(incl);
+        sum[c] += wave_sum64(lane <= first ? val : 0);
+        done[c] = true;
+      } else {
+        sum[c] += wave_sum64(val);
+        all = false;
+      }
+    }
+    if (all) break;
+    pred -= kWave;
+  }
+#pragma unroll
+  for (int c = 0; c < N; c++) excl[c] = uniform64(sum[c]);
+  if (lane < (uint32_t)N) {
+    uint64_t e = 0, a = 0;
+#pragma unroll
+    for (int c = 0; c < N; c++)
+      if ((uint32_t)c == lane) e = excl[c], a = agg[c];
+    atomic_store_agent(&status[lane * (uint64_t)maxTiles + tile], pack_status(epoch, 2, e + a));
+  }
+}
+
+}  // namespace tc

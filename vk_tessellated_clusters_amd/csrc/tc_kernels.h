@@ -1,0 +1,63 @@
+// tc_kernels.h -- kernel argument block and kernel declarations.
+#pragma once
+
+#include "tc_device.h"
+
+namespace tc {
+
+constexpr uint32_t kMaxClusterVerts = 256;
+constexpr uint32_t kMaxClusterTris = 256;
+
+// Everything a stage kernel reads: the bindings (device pointers), the
+// configuration and the look-back workspace.
+struct KArgs {
+  const tc_frame_constants* view;
+  const tc_tessellation_table* tessTable;
+  tc_readback* readback;
+  const tc_render_instance* instances;
+  tc_scene_building* build;
+  const tc_displacement_texture* textures;
+  uint32_t numInstances;
+  uint32_t numTextures;
+  WsHeader* ws;
+  uint64_t* status;  // kMaxCounters x maxTiles words
+  uint32_t maxTiles;
+  uint32_t _pad;
+  tc_config cfg;
+};
+
+// Workspace layout (bytes from the workspace start).
+struct WsLayout {
+  uint64_t status;       // look-back status words
+  uint64_t sortKeys[2];  // radix sort ping-pong keys (u64)
+  uint64_t sortVals[2];  // radix sort ping-pong values (u32)
+  uint64_t sortHist;     // per-digit per-tile counts
+  uint64_t sortAux;      // digit totals, segment starts, misc
+  uint64_t cullOffsets;  // per-instance offsets of clusters_cull
+  uint64_t total;
+  uint32_t maxTiles;
+  uint32_t sortCapacity;
+  uint32_t sortMaxTiles;
+  uint32_t instanceCapacity;
+};
+
+constexpr uint32_t kSortTile = 1024;  // keys per radix-sort workgroup
+constexpr uint32_t kSortRadix = 256;
+
+WsLayout ws_layout(const tc_config& cfg, uint32_t numInstances);
+
+tc_status make_args_public(const tc_bindings* b, const tc_config& cfg, KArgs& a);
+int num_cu_public();
+
+__global__ void k_prologue(KArgs a, uint32_t stage);
+__global__ void k_build_setup(KArgs a, int mode);
+__global__ void k_cluster_classify(KArgs a);
+__global__ void k_triangle_split(KArgs a);
+template <bool PN>
+__global__ void k_instantiate(KArgs a);
+__global__ void k_blas_setup_insertion(KArgs a);
+__global__ void k_instances_classify(KArgs a);
+__global__ void k_clusters_cull_scan(KArgs a, uint32_t* offsets);
+__global__ void k_clusters_cull_fill(KArgs a, const uint32_t* offsets);
+
+}  // namespace tc

@@ -1,0 +1,222 @@
+// tc_renderer.hip -- C++ frame driver: owns the per-frame buffers and
+// records the stage chain (renderer_raytrace_clusters_tess.cpp:166-310 for
+// the buffer layout, :410-780 for the frame).
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+struct Renderer {
+  tc_config cfg{};
+  uint32_t numInstances = 0, numTextures = 0;
+  // device memory
+  void* arena = nullptr;
+  uint64_t arenaBytes = 0;
+  tc_frame_constants* view = nullptr;
+  tc_readback* readback = nullptr;
+  tc_render_instance* instances = nullptr;
+  tc_displacement_texture* textures = nullptr;
+  tc_scene_building* build = nullptr;
+  tc_scene_building* buildTemplate = nullptr;  // per-frame initial state
+  tc_tessellation_table* table = nullptr;
+  void* workspace = nullptr;
+  uint64_t workspaceBytes = 0;
+  tc_scene_building hostBuild{};
+  tc_frame_constants* pinnedView = nullptr;
+  uint64_t splitBytes = 0;
+};
+
+static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
+
+// Resets the per-frame state: SceneBuilding from its template (with the
+// eye position of the frame), Readback to zero and the split list to ~0
+// (renderer_raytrace_clusters_tess.cpp:412-419).
+__global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building* tmpl, const tc_frame_constants* view,
+                              tc_readback* rb, uint32_t* split, uint64_t splitWords) {
+  const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  if (gid < sizeof(tc_scene_building) / 4) {
+    uint32_t v = reinterpret_cast<const uint32_t*>(tmpl)[gid];
+    if (gid < 3) v = __float_as_uint(view->viewPos[gid]);
+    reinterpret_cast<uint32_t*>(build)[gid] = v;
+  }
+  for (uint64_t i = gid; i < sizeof(tc_readback) / 4; i += stride) reinterpret_cast<uint32_t*>(rb)[i] = 0;
+  for (uint64_t i = gid; i < splitWords; i += stride) split[i] = ~0u;
+}
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" {
+
+tc_status tc_table_upload(tc_tessellation_table** tableDev);
+tc_status tc_table_free(tc_tessellation_table* tableDev);
+tc_status tc_workspace_init(const tc_bindings* b, const tc_config* config, tc_stream stream);
+tc_status tc_instances_classify(const tc_bindings* b, tc_config config, tc_stream stream);
+tc_status tc_clusters_cull(const tc_bindings* b, tc_config config, tc_stream stream);
+
+static Renderer* R(tc_renderer* r) { return reinterpret_cast<Renderer*>(r); }
+
+tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t numInstances,
+                             const tc_displacement_texture* texturesHost, uint32_t numTextures, const tc_config* config,
+                             tc_renderer** out) {
+  if (!instancesHost || !numInstances || !config || !out) return TC_ERROR_INVALID_ARGUMENT;
+  if (numTextures && !texturesHost) return TC_ERROR_INVALID_ARGUMENT;
+  uint64_t wsBytes = tc_workspace_size(config);
+  if (!wsBytes) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = new Renderer();
+  r->cfg = *config;
+  r->numInstances = numInstances;
+  r->numTextures = numTextures;
+  const tc_config& c = *config;
+
+  // carve one arena (renderer_raytrace_clusters_tess.cpp:200-310)
+  struct Slot {
+    uint64_t* dst;
+    uint64_t bytes;
+  };
+  tc_scene_building& B = r->hostBuild;
+  memset(&B, 0, sizeof(B));
+  B.numRenderInstances = numInstances;
+  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs;
+  std::vector<Slot> slots = {
+      {&oView, sizeof(tc_frame_constants)},
+      {&oRb, sizeof(tc_readback)},
+      {&oInst, sizeof(tc_render_instance) * (uint64_t)numInstances},
+      {&oTex, sizeof(tc_displacement_texture) * (uint64_t)std::max(numTextures, 1u)},
+      {&oBuild, sizeof(tc_scene_building)},
+      {&oTmpl, sizeof(tc_scene_building)},
+      {&B.instanceStates, 4ull * numInstances},
+      {&B.visibleClusters, sizeof(tc_cluster_info) * (uint64_t)c.maxVisibleClusters},
+      {&B.splitTriangles, sizeof(tc_tess_triangle_info) * (uint64_t)c.maxSplitTriangles},
+      {&B.partTriangles, sizeof(tc_tess_triangle_info) * (uint64_t)c.maxPartTriangles},
+      {&B.genVertices, 12ull * c.maxGeneratedVertices},
+      {&B.tempInstanceIDs, 4ull * c.maxGeneratedClusters},
+      {&B.tempInstantiations, sizeof(tc_template_instantiate_info) * (uint64_t)c.maxGeneratedClusters},
+      {&B.tempClusterAddresses, 8ull * c.maxGeneratedClusters},
+      {&B.tempClusterSizes, 4ull * c.maxGeneratedClusters},
+      {&B.blasBuildInfos, sizeof(tc_blas_build_info) * (uint64_t)numInstances},
+      {&B.blasBuildSizes, 4ull * numInstances},
+      {&B.blasClusterAddresses, 8ull * c.maxGeneratedClusters},
+      {&B.genClusterData, (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull},
+      {&oWs, wsBytes},
+  };
+  uint64_t o = 0;
+  for (auto& s : slots) {
+    *s.dst = o;
+    o = a256(o + s.bytes);
+  }
+  r->arenaBytes = o;
+  if (hipMalloc(&r->arena, o) != hipSuccess) {
+    delete r;
+    return TC_ERROR_OUT_OF_MEMORY;
+  }
+  const uint64_t base = reinterpret_cast<uint64_t>(r->arena);
+  char* mem = static_cast<char*>(r->arena);
+  r->view = reinterpret_cast<tc_frame_constants*>(mem + oView);
+  r->readback = reinterpret_cast<tc_readback*>(mem + oRb);
+  r->instances = reinterpret_cast<tc_render_instance*>(mem + oInst);
+  r->textures = reinterpret_cast<tc_displacement_texture*>(mem + oTex);
+  r->build = reinterpret_cast<tc_scene_building*>(mem + oBuild);
+  r->buildTemplate = reinterpret_cast<tc_scene_building*>(mem + oTmpl);
+  r->workspace = mem + oWs;
+  r->workspaceBytes = wsBytes;
+  for (uint64_t* f : {&B.instanceStates, &B.visibleClusters, &B.splitTriangles, &B.partTriangles, &B.genVertices,
+                      &B.tempInstanceIDs, &B.tempInstantiations, &B.tempClusterAddresses, &B.tempClusterSizes,
+                      &B.blasBuildInfos, &B.blasBuildSizes, &B.blasClusterAddresses, &B.genClusterData})
+    *f += base;
+  r->splitBytes = sizeof(tc_tess_triangle_info) * (uint64_t)c.maxSplitTriangles;
+
+  tc_status s = TC_OK;
+  if (hipMemcpy(r->instances, instancesHost, sizeof(tc_render_instance) * numInstances, hipMemcpyHostToDevice) != hipSuccess ||
+      (numTextures &&
+       hipMemcpy(r->textures, texturesHost, sizeof(tc_displacement_texture) * numTextures, hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(r->buildTemplate, &B, sizeof(B), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(r->arena, 0, oInst) != hipSuccess ||
+      hipMemset(reinterpret_cast<void*>(B.blasBuildSizes), 0, 4ull * numInstances) != hipSuccess)
+    s = TC_ERROR_LAUNCH;
+  if (s == TC_OK) s = tc_table_upload(&r->table);
+  if (s == TC_OK && hipHostMalloc((void**)&r->pinnedView, sizeof(tc_frame_constants), hipHostMallocDefault) != hipSuccess)
+    s = TC_ERROR_OUT_OF_MEMORY;
+  if (s == TC_OK) {
+    tc_bindings b;
+    tc_renderer_get_bindings(reinterpret_cast<tc_renderer*>(r), &b);
+    s = tc_workspace_init(&b, config, nullptr);
+    if (s == TC_OK && hipDeviceSynchronize() != hipSuccess) s = TC_ERROR_LAUNCH;
+  }
+  if (s != TC_OK) {
+    tc_renderer_destroy(reinterpret_cast<tc_renderer*>(r));
+    return s;
+  }
+  *out = reinterpret_cast<tc_renderer*>(r);
+  return TC_OK;
+}
+
+tc_status tc_renderer_destroy(tc_renderer* rr) {
+  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  if (r->table) tc_table_free(r->table);
+  if (r->pinnedView) (void)hipHostFree(r->pinnedView);
+  if (r->arena) (void)hipFree(r->arena);
+  delete r;
+  return TC_OK;
+}
+
+tc_status tc_renderer_get_bindings(tc_renderer* rr, tc_bindings* out) {
+  if (!rr || !out) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  out->view = r->view;
+  out->tessTable = r->table;
+  out->readback = r->readback;
+  out->instances = r->instances;
+  out->build = r->build;
+  out->displacementTextures = r->numTextures ? r->textures : nullptr;
+  out->numRenderInstances = r->numInstances;
+  out->numDisplacementTextures = r->numTextures;
+  out->workspace = r->workspace;
+  out->workspaceBytes = r->workspaceBytes;
+  return TC_OK;
+}
+
+tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {
+  if (!rr || !out) return TC_ERROR_INVALID_ARGUMENT;
+  *out = R(rr)->hostBuild;
+  return TC_OK;
+}
+
+// Frame start: frame constants upload (from pinned host memory) and
+// per-frame state reset.
+tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
+  if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  hipStream_t st = (hipStream_t)stream;
+  memcpy(r->pinnedView, frameHost, sizeof(tc_frame_constants));
+  if (hipMemcpyAsync(r->view, r->pinnedView, sizeof(tc_frame_constants), hipMemcpyHostToDevice, st) != hipSuccess)
+    return TC_ERROR_LAUNCH;
+  hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
+                     (const tc_frame_constants*)r->view, r->readback, reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles),
+                     r->splitBytes / 4);
+  return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+// Everything from instances_classify to blas_clusters_insert
+// (renderer_raytrace_clusters_tess.cpp:433-686).
+tc_status tc_renderer_tessellate(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
+  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  tc_status s;
+  if ((s = tc_renderer_begin_frame(rr, frameHost, stream)) != TC_OK) return s;
+  tc_bindings b;
+  tc_renderer_get_bindings(rr, &b);
+  if ((s = tc_instances_classify(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_clusters_cull(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_frame_tessellate(&b, r->cfg, 0, stream)) != TC_OK) return s;
+  return TC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,27 @@
+// tc_sort.h -- device radix sort and BLAS insertion kernels.
+#pragma once
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+struct SortBuffers {
+  uint64_t* keys[2];
+  uint32_t* vals[2];
+  uint32_t* hist;         // kSortRadix x maxTiles
+  uint32_t* digitTotals;  // kSortRadix
+  uint32_t capacity;      // max element count
+  uint32_t maxTiles;      // ceil(capacity / kSortTile)
+};
+
+// Sorts keys[0]/vals[0] by key bits [bit0, bit0 + bits) (whole passes of 8
+// bits); *outIndex receives the ping-pong slot holding the result.
+void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
+                      uint32_t* outIndex);
+
+__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity);
+__global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity);
+__global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
+                                 uint32_t capacity);
+
+}  // namespace tc
