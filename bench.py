@@ -1,0 +1,275 @@
+#!/usr/bin/env python3
+"""Frame benchmark of the tessellated-cluster path on one MI355X.
+
+Workload (default): 64 instances of a 256 x 256-quad grid patch (131072
+triangles each, FP32 positions) with seeded rigid transforms and frustum
+culling, a 2048^2 R16_UNORM fBm height map (seed 1, 6 octaves, bilinear),
+1920 x 1080, 4 px per segment, at most 11 segments per edge.
+
+A step is one whole frame with every input resident in HBM:
+instances_classify -> clusters_cull -> build_setup -> cluster_classify ->
+4 triangle_split passes -> triangle_tess_template_instantiate -> CLAS build
+-> blas_setup_insertion -> blas_clusters_insert -> BLAS + TLAS build ->
+one primary ray per pixel (back faces culled) and one shadow ray per hit.
+`value` is displaced triangles per second of whole frames.  Further frames
+are run stage by stage with HIP events on the same stream to time every
+stage; the slowest stage is the roofline kernel, priced with the
+algorithmic bytes of its work.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bench_config(abi):
+    return abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
+                              maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
+                              maxGeneratedVertices=1 << 25)
+
+
+def make_scene(name):
+    from vk_tessellated_clusters_amd import scene
+    if name == "grid64":
+        return scene.scene_grid_instances(), "64x grid 256x256 quads (8.4M tris), 2048^2 R16 fBm, 1920x1080 @ 4px"
+    if name == "icosphere":
+        return scene.scene_icosphere(), "icosphere 1280 tris, 4-oct fBm amp 0.05, 1280x720 @ 8px"
+    raise SystemExit("unknown workload " + name)
+
+
+class StageTimer:
+    """Records HIP events around stage calls on one stream."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream, self.marks = torch, stream, []
+
+    def mark(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.stream)
+        self.marks.append((name, e))
+
+    def durations(self):
+        self.torch.cuda.synchronize()
+        out = {}
+        for (n0, e0), (n1, e1) in zip(self.marks[:-1], self.marks[1:]):
+            out[n1] = out.get(n1, 0.0) + e0.elapsed_time(e1)
+        return out
+
+
+def staged_frame(r, fc, stream, timer, trace_params=None):
+    """One frame through the stage entry points, with an event after each."""
+    from vk_tessellated_clusters_amd import abi
+    L, b, cfg = r.L, C.byref(r.bindings), r.config
+    s = C.c_void_p(stream.cuda_stream)
+    timer.mark("start")
+    abi.check(L.tc_renderer_begin_frame(r.handle, C.byref(fc), s))
+    abi.check(L.tc_instances_classify(b, cfg, s))
+    abi.check(L.tc_clusters_cull(b, cfg, s))
+    timer.mark("frame_setup_cull")
+    abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_CLASSIFY, s))
+    abi.check(L.tc_cluster_classify(b, cfg, s))
+    timer.mark("cluster_classify")
+    abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_SPLIT, s))
+    passes = L.tc_num_split_passes(cfg.maxSplitFactor)
+    for p in range(passes):
+        abi.check(L.tc_triangle_split(b, cfg, s))
+        if p + 1 < passes:
+            abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_SPLIT_PASS, s))
+    timer.mark("triangle_split")
+    abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_INSTANTIATE_TESS, s))
+    abi.check(L.tc_triangle_tess_template_instantiate(b, cfg, s))
+    timer.mark("triangle_tess_template_instantiate")
+    abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_BUILD_BLAS, s))
+    abi.check(L.tc_clas_instantiate(b, cfg, s))
+    timer.mark("clas_instantiate")
+    abi.check(L.tc_blas_setup_insertion(b, cfg, s))
+    abi.check(L.tc_blas_clusters_insert(b, cfg, 0, s))
+    timer.mark("blas_insert")
+    accel, xf = r.accel()
+    abi.check(L.tc_accel_build(b, cfg, C.byref(accel), C.c_void_p(xf), s))
+    timer.mark("blas_tlas_build")
+    if trace_params is not None:
+        abi.check(L.tc_renderer_trace(r.handle, C.byref(trace_params), s))
+        timer.mark("trace_rays")
+
+
+def algorithmic_bytes(r, host, cfg):
+    """Compulsory bytes of each stage of the last frame (reads + writes of
+    the records each stage must touch; height map counted once per launch
+    up to its size)."""
+    from vk_tessellated_clusters_amd import abi
+    sb = r.scene_building()
+    rb = r.readback()
+    geo_of = [host.geometries[i.geometry] for i in host.instances]
+    nvis = sb.visibleClusterCounter
+    vis = r.copy(sb.visibleClusters, 8 * nvis).view(np.uint32).reshape(-1, 2)
+    V = np.array([int(geo_of[i].clusters["numVertices"][c]) for i, c in vis], np.int64)
+    T = np.array([int(geo_of[i].clusters["numTriangles"][c]) for i, c in vis], np.int64)
+    ntemp = sb.tempInstantiateCounter
+    ti = r.copy(sb.tempInstantiations, 32 * ntemp).view(abi.TEMPLATE_INSTANTIATE_INFO)
+    full_mask = (ti["clusterIdOffset"] >> 30) == 0
+    nparts = sb.partTriangleCounter
+    parts = r.copy(sb.partTriangles, 24 * nparts).view(abi.TESS_TRIANGLE_INFO)
+    lk = np.zeros(4096, abi.TABLE_ENTRY_DT)
+    abi.check(abi.lib().tc_table_lookup_host(lk.ctypes.data))
+    cfgs = (parts["triangleID_config"] >> 16) & 0x7FFF
+    nv_parts = lk["numVertices"][cfgs].astype(np.int64)
+    nt_parts = lk["numTriangles"][cfgs].astype(np.int64)
+    sizes = r.copy(sb.tempClusterSizes, 4 * ntemp).view(np.uint32).astype(np.int64)
+    tex_bytes = sum(a.nbytes for a, _ in host.textures)
+    full_verts = int(sb.genVertexCounter - nv_parts.sum())
+    n_split_items = sb.splitWriteCounter
+    classify = int((8 + 16 + 4 + 12 * V + 3 * T).sum() + (12 + 8 + 12) * full_verts + 44 * int(full_mask.sum())
+                   + 24 * (nparts + min(sb.splitWriteCounter, cfg.maxSplitTriangles)) + min(tex_bytes, 8 * full_verts))
+    split = int((24 + 36 + 3 + 16) * n_split_items + 24 * n_split_items * 8)
+    instantiate = int((24 + 96 + 3 + 16 + 44) * nparts + 12 * nv_parts.sum() + min(tex_bytes, 8 * int(nv_parts.sum())))
+    clas = int((32 + 32 + 4 + 8) * ntemp + sizes.sum() + 12 * sb.genVertexCounter + 4 * (nt_parts.sum()))
+    insert = int(ntemp * (4 + 8 + 4 + 8) + 2 * 12 * ntemp * 2)
+    # keys from CLAS headers, 5 radix passes of (8 B key + 4 B value) read and
+    # written, hierarchy + refit writing (N - 1) nodes of 64 B + 24 B boxes
+    accel = int(ntemp * (32 + 12) + 5 * 2 * 12 * ntemp + ntemp * (64 + 24 + 16 + 8))
+    return {"cluster_classify": classify, "triangle_split": split, "triangle_tess_template_instantiate": instantiate,
+            "clas_instantiate": clas, "blas_insert": insert, "blas_tlas_build": accel}, rb, sb
+
+
+def cpu_baseline(host, cfg, sample_instances, budget_s):
+    """The oracle (C++ port of the reference stages) on a bounded sample of
+    the workload: the first `sample_instances` instances, all host threads."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import harness
+    from vk_tessellated_clusters_amd.scene import HostScene
+    sub = HostScene(host.geometries, host.instances[:sample_instances], host.textures, host.camera,
+                    host.tess_rate_pixels, host.name)
+    threads = min(16, os.cpu_count() or 1)
+    of = harness.OracleFrame(sub, cfg)
+    of.run(sub.frame(), nthreads=threads, stages="tess")  # warm-up
+    times, tris = [], 0
+    t_end = time.time() + budget_s
+    while time.time() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        of.run(sub.frame(), nthreads=threads, stages="all")
+        times.append(time.perf_counter() - t0)
+        tris = of.rb.numTotalTriangles
+        if len(times) >= 20:
+            break
+    med = float(np.median(times))
+    return {"value": tris / med, "unit": "displaced tris/s", "cores": threads, "kind": "port",
+            "sample": "first %d of %d instances, whole frame (classify..insert), median of %d frames, %d tris/frame"
+                      % (sample_instances, len(host.instances), len(times), tris)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="grid64")
+    ap.add_argument("--cpu-sample-instances", type=int, default=8)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
+              "see the sharding notes); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    from vk_tessellated_clusters_amd import abi
+    from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
+
+    host, desc = make_scene(args.workload)
+    cfg = bench_config(abi)
+    ds = DeviceScene(host)
+    r = Renderer(ds, cfg)
+    fc = host.frame()
+    stream = torch.cuda.current_stream()
+
+    W, H = host.camera.width, host.camera.height
+    hits = torch.empty(W * H * abi.HIT_DT.itemsize, dtype=torch.uint8, device="cuda")
+    dec = torch.empty(W * H * abi.HIT_DECODED_DT.itemsize, dtype=torch.uint8, device="cuda")
+    tp = abi.TraceParams()
+    tp.width, tp.height, tp.flags = W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW
+    tp.hits, tp.decoded = hits.data_ptr(), dec.data_ptr()
+
+    def frame():
+        r.tessellate(fc, stream)
+        r.build_accel(stream)
+        r.trace(tp, stream)
+
+    for _ in range(args.warmup):
+        frame()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        frame()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+
+    # stage-by-stage frames for the stage times and the roofline kernel
+    stage_ms = {}
+    nstage = max(3, min(args.steps, 10))
+    for _ in range(nstage):
+        t = StageTimer(torch, stream)
+        staged_frame(r, fc, stream, t, tp)
+        for k, v in t.durations().items():
+            stage_ms[k] = stage_ms.get(k, 0.0) + v / nstage
+    alg, rb, sb = algorithmic_bytes(r, host, cfg)
+    tris = rb.numTotalTriangles
+    torch.cuda.synchronize()
+    dec_np = dec.cpu().numpy().view(abi.HIT_DECODED_DT)
+    n_primary = W * H
+    n_shadow = int((dec_np["instanceID"] != 0xFFFFFFFF).sum())
+    bvh_ms = stage_ms.get("clas_instantiate", 0) + stage_ms.get("blas_insert", 0) + stage_ms.get("blas_tlas_build", 0)
+    trace_ms = stage_ms.get("trace_rays", 0.0)
+    dom = max(alg, key=lambda k: stage_ms.get(k, 0.0))
+    dom_ms = stage_ms[dom]
+    achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
+    frame_bytes = sum(alg.values())
+
+    out = {
+        "metric": "displaced tris/s, cluster-BVH build ms/frame, Mrays/s, HBM GB/s (1 GPU)",
+        "value": tris / (ms * 1e-3),
+        "unit": "displaced tris/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
+                   "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),
+                   "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
+                   "clas_per_frame": int(sb.tempInstantiateCounter), "gen_vertices": int(sb.genVertexCounter)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "cluster_bvh_build_ms_per_frame": bvh_ms,
+        "mrays_per_s": (n_primary + n_shadow) / (trace_ms * 1e-3) / 1e6 if trace_ms else None,
+        "rays_per_frame": {"primary": n_primary, "shadow": n_shadow},
+        "frame_hbm_gbs": frame_bytes / (ms * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": alg[dom],
+                     "launch_ms": dom_ms},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_sample_instances, args.cpu_budget_s)
+    print(json.dumps(out))
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

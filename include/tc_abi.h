@@ -469,7 +469,7 @@ typedef struct tc_config {
   uint32_t maxGeneratedClusterMegs; /* MAX_GENERATED_CLUSTER_MEGS  */
   uint32_t maxGeneratedVertices;    /* MAX_GENERATED_VERTICES      */
   uint32_t clusterVertexCount;      /* CLUSTER_VERTEX_COUNT  (<=255) */
-  uint32_t clusterTriangleCount;    /* CLUSTER_TRIANGLE_COUNT (<=255) */
+  uint32_t clusterTriangleCount;    /* CLUSTER_TRIANGLE_COUNT (<=128) */
   uint32_t maxSplitFactor;          /* TESS_MAX_SPLIT_FACTOR       */
 } tc_config;
 
@@ -567,6 +567,59 @@ tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t n
  * tempClusterSizes[i]. */
 tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config, tc_stream stream);
 
+/* ---- cluster acceleration structures (replace the NV BLAS build and the
+ *      TLAS build, renderer_raytrace_clusters_tess.cpp:695-745) ---- */
+
+/* Device arrays of the two-level structure built each frame.  blasNodes and
+ * blasLeaves are indexed per instance from the instance's first reference
+ * (tc_tlas_instance.blasNodes / blasLeaves point at that offset). */
+typedef struct tc_accel {
+  tc_bvh_node* blasNodes;
+  uint64_t* blasLeaves; /* CLAS address per sorted reference */
+  tc_tlas_instance* tlasInstances;
+  tc_bvh_node* tlasNodes;
+  uint32_t* tlasRoot; /* [0] root child word */
+  uint32_t numInstances;
+  uint32_t capacity; /* CLAS references (>= config.maxGeneratedClusters) */
+  void* memory;
+  uint64_t memoryBytes;
+} tc_accel;
+
+/* Bytes of device memory for tc_accel_bind. */
+uint64_t tc_accel_size(uint32_t capacity, uint32_t numInstances);
+/* Carves the arrays of a tc_accel out of caller-owned device memory. */
+tc_status tc_accel_bind(void* memory, uint64_t bytes, uint32_t capacity, uint32_t numInstances, tc_accel* out);
+/* Per-instance LBVH over the CLAS of build.blasBuildInfos (after
+ * tc_blas_clusters_insert) and the TLAS over the instances.
+ * instanceXforms: device array, per instance objectToWorld then
+ * worldToObject, each 3x4 row-major (24 floats). */
+tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel* accel, const float* instanceXforms,
+                         tc_stream stream);
+
+/* ---- ray tracing (vkCmdTraceRaysKHR with render_raytrace.rgen.glsl,
+ *      render_raytrace_clusters.rchit.glsl, render_shading.glsl:263-274) ---- */
+
+#define TC_TRACE_CULL_BACK (1u << 0) /* gl_RayFlagsCullBackFacingTrianglesEXT on primary rays */
+#define TC_TRACE_SHADOW (1u << 1)    /* one shadow ray toward skyParams.sunDirection per hit */
+
+/* Rays are either explicit (rays != NULL, numRays) or the primary rays of
+ * the frame camera, one per pixel of width x height (rgen:52-86).  Hits are
+ * closest hits with ties broken by (instance, cluster, primitive).  Front
+ * faces are counter-clockwise seen from the ray origin (right-hand normal
+ * toward the origin). */
+typedef struct tc_trace_params {
+  const tc_ray* rays; /* device, or NULL for camera rays */
+  uint32_t numRays;
+  uint32_t width;
+  uint32_t height;
+  uint32_t flags; /* TC_TRACE_* */
+  tc_hit* hits;            /* device, per ray (t < 0: miss), may be NULL */
+  tc_hit_decoded* decoded; /* device, per ray, may be NULL */
+} tc_trace_params;
+
+tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* params,
+                        tc_stream stream);
+
 /* ---- frame driver (renderer_raytrace_clusters_tess.cpp:166-310, 410-780) ---- */
 
 typedef struct tc_renderer tc_renderer;
@@ -587,6 +640,19 @@ tc_status tc_renderer_get_building(tc_renderer* renderer, tc_scene_building* out
 tc_status tc_renderer_begin_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
 /* begin_frame, instances_classify, clusters_cull and tc_frame_tessellate. */
 tc_status tc_renderer_tessellate(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
+
+/* Accel arrays of the renderer and its per-instance transforms (device). */
+tc_status tc_renderer_get_accel(tc_renderer* renderer, tc_accel* out, const float** instanceXforms);
+/* tc_clas_instantiate, tc_blas_setup_insertion, tc_blas_clusters_insert,
+ * tc_accel_build over the renderer's buffers. */
+tc_status tc_renderer_build_accel(tc_renderer* renderer, tc_stream stream);
+tc_status tc_renderer_trace(tc_renderer* renderer, const tc_trace_params* params, tc_stream stream);
+/* tessellate + build_accel + trace (params may be NULL). */
+tc_status tc_renderer_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, const tc_trace_params* params,
+                            tc_stream stream);
+/* objectToWorld and worldToObject (3x4 row-major, 24 floats) per instance,
+ * the transforms the renderer uploads; worldToObject is inverted in FP64. */
+tc_status tc_instance_xforms_host(const tc_render_instance* instances, uint32_t n, float* out);
 
 /* Number of split passes the reference records for a split factor
  * (renderer_raytrace_clusters_tess.cpp:516-543). */

@@ -936,6 +936,240 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
   }
 }
 
+/* ------------------------------------------------------------ ray tracing */
+
+namespace {
+
+struct Ray {
+  float o[3], d[3], inv[3];
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+/* Woop, Benthin, Wald: Watertight Ray/Triangle Intersection (JCGT 2013) */
+void setup_ray(Ray& r, const float o[3], const float d[3]) {
+  for (int k = 0; k < 3; k++) r.o[k] = o[k], r.d[k] = d[k], r.inv[k] = 1.0f / d[k];
+  float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  if (d[r.kz] < 0.0f) std::swap(r.kx, r.ky);
+  r.Sx = d[r.kx] / d[r.kz];
+  r.Sy = d[r.ky] / d[r.kz];
+  r.Sz = 1.0f / d[r.kz];
+}
+
+float box_hit(const Ray& r, const float lo[3], const float hi[3], float tmin, float tmax) {
+  float tn = tmin, tf = tmax;
+  for (int k = 0; k < 3; k++) {
+    float t0 = (lo[k] - r.o[k]) * r.inv[k], t1 = (hi[k] - r.o[k]) * r.inv[k];
+    tn = fmaxf(tn, fminf(t0, t1));
+    tf = fminf(tf, fmaxf(t0, t1));
+  }
+  return tn <= tf * 1.0000005f ? tn : INFINITY;
+}
+
+bool tri_hit(const Ray& r, const float* p0, const float* p1, const float* p2, bool cull, float tmin, float tmax, float& t,
+             float& u, float& v) {
+  float A[3] = {p0[0] - r.o[0], p0[1] - r.o[1], p0[2] - r.o[2]};
+  float B[3] = {p1[0] - r.o[0], p1[1] - r.o[1], p1[2] - r.o[2]};
+  float Cc[3] = {p2[0] - r.o[0], p2[1] - r.o[1], p2[2] - r.o[2]};
+  float Ax = A[r.kx] - r.Sx * A[r.kz], Ay = A[r.ky] - r.Sy * A[r.kz];
+  float Bx = B[r.kx] - r.Sx * B[r.kz], By = B[r.ky] - r.Sy * B[r.kz];
+  float Cx = Cc[r.kx] - r.Sx * Cc[r.kz], Cy = Cc[r.ky] - r.Sy * Cc[r.kz];
+  float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = (U + V) + W;
+  if (det == 0.0f || (cull && det < 0.0f)) return false;
+  float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * Cc[r.kz];
+  float T = (U * Az + V * Bz) + W * Cz;
+  float tt = T / det;
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  t = tt, u = V / det, v = W / det;
+  return true;
+}
+
+void xform34(const float* m, const float v[3], float w, float out[3]) {
+  for (int r = 0; r < 3; r++) out[r] = ((m[4 * r] * v[0] + m[4 * r + 1] * v[1]) + m[4 * r + 2] * v[2]) + m[4 * r + 3] * w;
+}
+
+struct Best {
+  tc_hit h;
+  const char* clas;
+  uint32_t word;
+};
+
+bool less_key(float t, uint32_t i, uint32_t c, uint32_t p, const tc_hit& b) {
+  if (t != b.t) return t < b.t;
+  if (i != b.instanceID) return i < b.instanceID;
+  if (c != b.clusterID) return c < b.clusterID;
+  return p < b.primitiveID;
+}
+
+const float* clas_verts(const char* clas, uint32_t nt, uint32_t ng) {
+  return reinterpret_cast<const float*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u));
+}
+
+/* every CLAS of every instance, tested in CLAS-list order */
+bool trace_one(const oracle_ctx* c, const float* xf, const float o[3], const float d[3], float tmin, float tmax, bool cull,
+               bool anyHit, Best& best) {
+  best.h.t = tmax;
+  best.h.u = best.h.v = 0.0f;
+  best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
+  best.clas = nullptr;
+  best.word = 0;
+  const tc_blas_build_info* infos = P<const tc_blas_build_info>(c->build->blasBuildInfos);
+  for (uint32_t i = 0; i < c->numRenderInstances; i++) {
+    const tc_blas_build_info& bi = infos[i];
+    if (!bi.clusterReferencesCount) continue;
+    float ol[3], dl[3];
+    xform34(xf + 24 * i + 12, o, 1.0f, ol);
+    xform34(xf + 24 * i + 12, d, 0.0f, dl);
+    Ray r;
+    setup_ray(r, ol, dl);
+    const uint64_t* refs = P<const uint64_t>(bi.clusterReferences);
+    for (uint32_t k = 0; k < bi.clusterReferencesCount; k++) {
+      const char* clas = P<const char>(refs[k]);
+      tc_clas_header hdr;
+      memcpy(&hdr, clas, sizeof(hdr));
+      if (box_hit(r, hdr.lo, hdr.hi, tmin, best.h.t) == INFINITY) continue;
+      uint32_t nt = hdr.counts & 0xFF, ng = (hdr.counts >> 16) & 0xFF;
+      const tc_aabb* groups = reinterpret_cast<const tc_aabb*>(clas + sizeof(tc_clas_header));
+      const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
+      const float* verts = clas_verts(clas, nt, ng);
+      for (uint32_t g = 0; g < ng; g++) {
+        if (box_hit(r, groups[g].lo, groups[g].hi, tmin, best.h.t) == INFINITY) continue;
+        for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < std::min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES); t++) {
+          uint32_t w = tris[t];
+          float th, tu, tv;
+          if (!tri_hit(r, verts + 3 * (w & 0xFF), verts + 3 * ((w >> 8) & 0xFF), verts + 3 * ((w >> 16) & 0xFF), cull, tmin,
+                       best.h.t, th, tu, tv))
+            continue;
+          if (anyHit) {
+            best.h.t = th;
+            return true;
+          }
+          uint32_t prim = w >> 24;
+          if (less_key(th, i, hdr.clusterID, prim, best.h)) {
+            best.h = tc_hit{th, tu, tv, i, hdr.clusterID, prim};
+            best.clas = clas;
+            best.word = w;
+          }
+        }
+      }
+    }
+  }
+  return best.h.instanceID != 0xFFFFFFFFu;
+}
+
+void mul44(const float* m, const float v[4], float out[4]) {
+  for (int r = 0; r < 4; r++) out[r] = ((m[r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
+}
+void normalize4(float v[4]) {
+  float l = sqrtf(((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3]);
+  float s = 1.0f / l;
+  for (int k = 0; k < 4; k++) v[k] = v[k] * s;
+}
+
+/* render_raytrace.rgen.glsl:52-86 */
+void primary_ray(const tc_frame_constants* vw, uint32_t x, uint32_t y, uint32_t W, uint32_t H, float o[3], float d[3]) {
+  float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;
+  float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
+  float org[4] = {0, 0, 0, 1}, origin[4], ndc[4] = {dx, dy, 1.0f, 1.0f}, target[4], dir[4];
+  mul44(vw->viewMatrixI, org, origin);
+  mul44(vw->projMatrixI, ndc, target);
+  normalize4(target);
+  float tv[4] = {target[0], target[1], target[2], 0.0f};
+  mul44(vw->viewMatrixI, tv, dir);
+  normalize4(dir);
+  for (int k = 0; k < 3; k++) o[k] = origin[k], d[k] = dir[k];
+}
+
+/* render_raytrace_clusters.rchit.glsl:128-236, 338-339 */
+void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded& out) {
+  const tc_hit& h = b.h;
+  out.instanceID = h.instanceID;
+  out.shadowed = 0;
+  if (h.instanceID == 0xFFFFFFFFu) {
+    out.clusterID = out.triangleID = out.subTriangleID = 0xFFFFFFFFu;
+    out.baryBase[0] = out.baryBase[1] = out.baryBase[2] = 0.0f;
+    out.depth = 1.0f;
+    out.wPos[0] = out.wPos[1] = out.wPos[2] = 0.0f;
+    return;
+  }
+  uint32_t mode = h.clusterID >> 30, clusterID = h.clusterID & 0x3FFFFFFFu, triangleID = h.primitiveID;
+  v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v), base = bw;
+  if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
+    const tc_tess_triangle_info& ti = P<const tc_tess_triangle_info>(c->build->partTriangles)[clusterID];
+    triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
+    uint32_t cfg = ti.subTriangle.triangleID_config >> 16;
+    clusterID = ti.cluster.clusterID;
+    uint32_t idx[3];
+    cfg_tri_vertices(cfg, h.primitiveID, idx);
+    v3 vb[3] = {cfg_vertex_bary(cfg, idx[0]), cfg_vertex_bary(cfg, idx[1]), cfg_vertex_bary(cfg, idx[2])};
+    v3 nb = interp(vb, bw);
+    v3 bb[3] = {decode_bary(ti.subTriangle.vtxEncoded[0]), decode_bary(ti.subTriangle.vtxEncoded[1]),
+                decode_bary(ti.subTriangle.vtxEncoded[2])};
+    base = interp(bb, nb);
+  }
+  out.clusterID = clusterID;
+  out.triangleID = triangleID;
+  out.subTriangleID = h.primitiveID;
+  out.baryBase[0] = base.x, out.baryBase[1] = base.y, out.baryBase[2] = base.z;
+  tc_clas_header hdr;
+  memcpy(&hdr, b.clas, sizeof(hdr));
+  const float* verts = clas_verts(b.clas, hdr.counts & 0xFF, (hdr.counts >> 16) & 0xFF);
+  const float* p0 = verts + 3 * (b.word & 0xFF);
+  const float* p1 = verts + 3 * ((b.word >> 8) & 0xFF);
+  const float* p2 = verts + 3 * ((b.word >> 16) & 0xFF);
+  v3 pv[3] = {mk(p0[0], p0[1], p0[2]), mk(p1[0], p1[1], p1[2]), mk(p2[0], p2[1], p2[2])};
+  v3 op = interp(pv, bw);
+  float opv[3] = {op.x, op.y, op.z}, wp[3];
+  xform34(xf + 24 * h.instanceID, opv, 1.0f, wp);
+  out.wPos[0] = wp[0], out.wPos[1] = wp[1], out.wPos[2] = wp[2];
+  float w4[4] = {wp[0], wp[1], wp[2], 1.0f}, pp[4];
+  mul44(c->view->viewProjMatrix, w4, pp);
+  out.depth = pp[2] / pp[3];
+}
+
+}  // namespace
+
+extern "C" void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_ray* rays, uint32_t numRays, uint32_t width,
+                             uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded, uint32_t nthreads) {
+  uint32_t n = rays ? numRays : width * height;
+  bool cull = (flags & 1u) != 0, shadows = (flags & 2u) != 0;
+  parallel_for(n, nthreads, [&](uint32_t r) {
+    float o[3], d[3], tmin, tmax;
+    if (rays) {
+      for (int k = 0; k < 3; k++) o[k] = rays[r].origin[k], d[k] = rays[r].direction[k];
+      tmin = rays[r].tMin, tmax = rays[r].tMax;
+    } else {
+      primary_ray(c->view, r % width, r / width, width, height, o, d);
+      tmin = c->view->nearPlane, tmax = c->view->farPlane;
+    }
+    Best b;
+    trace_one(c, xforms, o, d, tmin, tmax, cull, false, b);
+    tc_hit hit = b.h;
+    if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
+    if (hits) hits[r] = hit;
+    if (!decoded) return;
+    tc_hit_decoded dec;
+    decode(c, xforms, b, dec);
+    if (shadows && hit.instanceID != 0xFFFFFFFFu) {
+      const float* sun = c->view->skyParams.sunDirection;
+      float sd[3] = {sun[0], sun[1], sun[2]};
+      Best sb;
+      dec.shadowed = trace_one(c, xforms, dec.wPos, sd, 0.001f, 10000000.0f, false, true, sb) ? 1u : 0u;
+    }
+    decoded[r] = dec;
+  });
+}
+
 uint32_t oracle_num_split_passes(uint32_t splitFactor) {
   /* renderer_raytrace_clusters_tess.cpp:516-520 */
   uint32_t coord = 32768, passes = 0;

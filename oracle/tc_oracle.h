@@ -82,6 +82,15 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg);
 
 uint32_t oracle_num_split_passes(uint32_t splitFactor);
 
+/* Closest-hit ray casting over every CLAS of every instance (after
+ * oracle_blas_clusters_insert), no hierarchy: the reference's rgen / rchit /
+ * shadow-ray semantics with a watertight triangle test.  rays == NULL casts
+ * the camera's width x height primary rays.  flags: 1 cull back faces,
+ * 2 shadow rays.  xforms: 24 floats per instance (objectToWorld,
+ * worldToObject, 3x4 row-major). */
+void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_ray* rays, uint32_t numRays, uint32_t width,
+                  uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded, uint32_t nthreads);
+
 /* The chain of renderer_raytrace_clusters_tess.cpp:467-582 from
  * build_setup CLASSIFY through build_setup BUILD_BLAS (visible clusters and
  * instance states are inputs).  CLAS build and BLAS insertion follow as

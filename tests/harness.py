@@ -58,6 +58,8 @@ def oracle():
         L.oracle_sample_height.argtypes = [P, C.c_float, C.c_float]
         L.oracle_sample_height.restype = C.c_float
         L.oracle_table_lookup.argtypes = [P]
+        L.oracle_trace.argtypes = [ctx, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
+        L.oracle_trace.restype = None
         for n in ("oracle_table_num_vertices", "oracle_table_num_triangles", "oracle_table_num_configs"):
             getattr(L, n).restype = C.c_uint32
         for n, t in (("oracle_table_vertices", C.c_uint32), ("oracle_table_triangles", C.c_uint32),
@@ -278,6 +280,17 @@ class OracleFrame:
         O.oracle_blas_setup_insertion(ctx, self.cfg)
         O.oracle_blas_clusters_insert(ctx, self.cfg, 0)
 
+    def trace(self, width, height, flags, nthreads=8, rays=None):
+        xf = np.zeros(24 * len(self.instances), np.float32)
+        abi.check(abi.lib().tc_instance_xforms_host(C.addressof(self.instances), len(self.instances), xf.ctypes.data))
+        n = len(rays) if rays is not None else width * height
+        hits = np.zeros(n, abi.HIT_DT)
+        dec = np.zeros(n, abi.HIT_DECODED_DT)
+        rp = rays.ctypes.data if rays is not None else None
+        oracle().oracle_trace(C.byref(self.ctx), xf.ctypes.data, rp, n if rays is not None else 0, width, height, flags,
+                              hits.ctypes.data, dec.ctypes.data, nthreads)
+        return hits, dec
+
     def results(self) -> Results:
         def fetch(addr, nbytes):
             return np.frombuffer(C.string_at(addr, nbytes), np.uint8).copy() if nbytes else np.zeros(0, np.uint8)
@@ -305,9 +318,42 @@ class GpuFrame:
 
     def run(self, fc: abi.FrameConstants, stages: str = "all"):
         self.r.tessellate(fc)
-        if stages != "tess":
+        if stages == "accel":
+            self.r.build_accel()
+        elif stages != "tess":
             self.r.clas_and_blas()
         abi.check(abi.lib().tc_device_synchronize(), "synchronize")
+
+    def trace(self, width, height, flags, rays=None):
+        import torch
+        n = len(rays) if rays is not None else width * height
+        hits = torch.zeros(n * abi.HIT_DT.itemsize, dtype=torch.uint8, device="cuda")
+        dec = torch.zeros(n * abi.HIT_DECODED_DT.itemsize, dtype=torch.uint8, device="cuda")
+        p = abi.TraceParams()
+        if rays is not None:
+            rd = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
+            p.rays, p.numRays = rd.data_ptr(), n
+        p.width, p.height, p.flags = width, height, flags
+        p.hits, p.decoded = hits.data_ptr(), dec.data_ptr()
+        self.r.trace(p)
+        torch.cuda.synchronize()
+        return hits.cpu().numpy().view(abi.HIT_DT), dec.cpu().numpy().view(abi.HIT_DECODED_DT)
+
+    def accel_arrays(self):
+        """Host copies of the built acceleration structure."""
+        a, _ = self.r.accel()
+        sb = self.r.scene_building()
+        n = a.numInstances
+        out = {"accel": a, "sb": sb}
+        out["tlasInstances"] = self.r.copy(a.tlasInstances, 128 * n).view(abi.TLAS_INSTANCE_DT)
+        out["tlasNodes"] = self.r.copy(a.tlasNodes, 64 * n).view(abi.BVH_NODE_DT)
+        out["tlasRoot"] = int(self.r.copy(a.tlasRoot, 4).view(np.uint32)[0])
+        total = sb.blasClusterCounter
+        out["blasNodes"] = self.r.copy(a.blasNodes, 64 * total).view(abi.BVH_NODE_DT)
+        out["blasLeaves"] = self.r.copy(a.blasLeaves, 8 * total).view(np.uint64)
+        out["refs"] = self.r.copy(sb.blasClusterAddresses, 8 * total).view(np.uint64)
+        out["infos"] = self.r.copy(sb.blasBuildInfos, 16 * n).view(abi.BLAS_BUILD_INFO)
+        return out
 
     def results(self) -> Results:
         sb = self.r.scene_building()

@@ -157,6 +157,48 @@ class ClasTemplate(C.Structure):
                 ("triangles", u64), ("_pad", u64)]
 
 
+class Ray(C.Structure):
+    _fields_ = [("origin", f32 * 3), ("tMin", f32), ("direction", f32 * 3), ("tMax", f32)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", f32), ("u", f32), ("v", f32), ("instanceID", u32), ("clusterID", u32), ("primitiveID", u32)]
+
+
+class HitDecoded(C.Structure):
+    _fields_ = [("instanceID", u32), ("clusterID", u32), ("triangleID", u32), ("subTriangleID", u32),
+                ("baryBase", f32 * 3), ("depth", f32), ("wPos", f32 * 3), ("shadowed", u32)]
+
+
+class Accel(C.Structure):
+    _fields_ = [("blasNodes", C.c_void_p), ("blasLeaves", C.c_void_p), ("tlasInstances", C.c_void_p),
+                ("tlasNodes", C.c_void_p), ("tlasRoot", C.c_void_p), ("numInstances", u32), ("capacity", u32),
+                ("memory", C.c_void_p), ("memoryBytes", u64)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("rays", C.c_void_p), ("numRays", u32), ("width", u32), ("height", u32), ("flags", u32),
+                ("hits", C.c_void_p), ("decoded", C.c_void_p)]
+
+
+TRACE_CULL_BACK = 1
+TRACE_SHADOW = 2
+
+RAY_DT = np.dtype([("origin", "<f4", (3,)), ("tMin", "<f4"), ("direction", "<f4", (3,)), ("tMax", "<f4")])
+HIT_DT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instanceID", "<u4"), ("clusterID", "<u4"),
+                   ("primitiveID", "<u4")])
+HIT_DECODED_DT = np.dtype([("instanceID", "<u4"), ("clusterID", "<u4"), ("triangleID", "<u4"),
+                           ("subTriangleID", "<u4"), ("baryBase", "<f4", (3,)), ("depth", "<f4"),
+                           ("wPos", "<f4", (3,)), ("shadowed", "<u4")])
+BVH_NODE_DT = np.dtype([("lo0", "<f4", (3,)), ("child0", "<u4"), ("hi0", "<f4", (3,)), ("child1", "<u4"),
+                        ("lo1", "<f4", (3,)), ("pad0", "<u4"), ("hi1", "<f4", (3,)), ("pad1", "<u4")])
+TLAS_INSTANCE_DT = np.dtype([("objectToWorld", "<f4", (12,)), ("worldToObject", "<f4", (12,)), ("blasNodes", "<u8"),
+                             ("blasLeaves", "<u8"), ("blasRoot", "<u4"), ("numClusters", "<u4"), ("flags", "<u4"),
+                             ("pad", "<u4")])
+CLAS_HEADER_DT = np.dtype([("lo", "<f4", (3,)), ("clusterID", "<u4"), ("hi", "<f4", (3,)), ("counts", "<u4")])
+BVH_LEAF = 0x80000000
+
+
 class Config(C.Structure):
     _fields_ = [("flags", u32), ("maxVisibleClusters", u32), ("maxPartTriangles", u32), ("maxSplitTriangles", u32),
                 ("maxGeneratedClusters", u32), ("maxGeneratedClusterMegs", u32), ("maxGeneratedVertices", u32),
@@ -199,7 +241,9 @@ EXPORTS = [
     "tc_blas_setup_insertion", "tc_blas_clusters_insert", "tc_frame_tessellate", "tc_renderer_create",
     "tc_renderer_destroy", "tc_renderer_get_bindings", "tc_renderer_get_building", "tc_renderer_begin_frame",
     "tc_renderer_tessellate", "tc_num_split_passes", "tc_memcpy", "tc_device_synchronize", "tc_clas_instantiate",
-    "tc_table_templates_host",
+    "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
+    "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
+    "tc_instance_xforms_host",
 ]
 
 
@@ -241,6 +285,15 @@ def lib():
         "tc_device_synchronize": (i32, []),
         "tc_clas_instantiate": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_table_templates_host": (i32, [P, P, P, C.POINTER(u32)]),
+        "tc_accel_size": (u64, [u32, u32]),
+        "tc_accel_bind": (i32, [P, u64, u32, u32, C.POINTER(Accel)]),
+        "tc_accel_build": (i32, [C.POINTER(Bindings), Config, C.POINTER(Accel), P, P]),
+        "tc_trace_rays": (i32, [C.POINTER(Bindings), Config, C.POINTER(Accel), C.POINTER(TraceParams), P]),
+        "tc_renderer_get_accel": (i32, [P, C.POINTER(Accel), C.POINTER(P)]),
+        "tc_renderer_build_accel": (i32, [P, P]),
+        "tc_renderer_trace": (i32, [P, C.POINTER(TraceParams), P]),
+        "tc_renderer_frame": (i32, [P, C.POINTER(FrameConstants), C.POINTER(TraceParams), P]),
+        "tc_instance_xforms_host": (i32, [P, u32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

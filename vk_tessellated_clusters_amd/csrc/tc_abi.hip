@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "tc_sort.h"
+#include "tc_stage.h"
 
 namespace tc {
 
@@ -412,8 +413,9 @@ tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream 
   if (s != TC_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)0);
-  uint32_t grid = std::min<uint32_t>(config.maxVisibleClusters, (uint32_t)num_cu() * 16);
-  hipLaunchKernelGGL(k_cluster_classify, dim3(grid), dim3(64), 0, st, a);
+  uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 5);
+  hipLaunchKernelGGL(k_cluster_classify, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
   return launched();
 }
 
@@ -435,12 +437,12 @@ tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config 
   if (s != TC_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)2);
-  uint32_t tiles = (config.maxPartTriangles + kWave - 1) / kWave;
-  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 16);
+  uint32_t tiles = (config.maxPartTriangles + kInstTile - 1) / kInstTile;
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 3);
   if (config.flags & TC_FLAG_PN)
-    hipLaunchKernelGGL(k_instantiate<true>, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_instantiate<true>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
   else
-    hipLaunchKernelGGL(k_instantiate<false>, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_instantiate<false>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
   return launched();
 }
 

@@ -1,0 +1,47 @@
+// tc_bvh.h -- BLAS / TLAS build arguments.
+#pragma once
+
+#include "tc_sort.h"
+
+namespace tc {
+
+struct BvhArgs {
+  tc_scene_building* build;
+  const tc_render_instance* instances;
+  uint32_t numInstances;
+  uint32_t capacity;  // CLAS references the arrays hold
+  // BLAS
+  uint64_t* keys;  // sort input
+  uint32_t* vals;
+  const uint64_t* sortedKeys;
+  const uint32_t* sortedVals;
+  uint64_t* leafAddr;  // CLAS address per sorted position (blasLeaves)
+  uint32_t* child;     // 2 per internal node, relative to the instance start
+  uint32_t* parent;    // [0, capacity) internal nodes, [capacity, 2 capacity) leaves
+  uint32_t* counter;   // refit arrival counters
+  tc_bvh_node* nodes;  // blasNodes
+  tc_aabb* nodeBox;    // union box of every internal node
+  // TLAS
+  tc_tlas_instance* tlasInst;
+  tc_aabb* tlasLeafBox;
+  uint64_t* tlasKeys;
+  uint32_t* tlasVals;
+  const uint64_t* tlasSortedKeys;
+  const uint32_t* tlasSortedVals;
+  uint32_t* tlasChild;
+  uint32_t* tlasParent;
+  uint32_t* tlasCounter;
+  tc_bvh_node* tlasNodes;
+  tc_aabb* tlasNodeBox;
+  uint32_t* tlasRoot;  // [0] root word, [1] instance count
+};
+
+struct AccelLayout {
+  uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter;
+  uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasRoot;
+  uint64_t total;
+};
+
+AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances);
+
+}  // namespace tc

@@ -1,0 +1,462 @@
+// tc_bvh.hip -- per-instance BLAS and the TLAS over instances.
+//
+// Replaces the NV cluster BLAS build and the TLAS update
+// (renderer_raytrace_clusters_tess.cpp:695-745, renderer.cpp:318-337):
+//   keys    30-bit Morton code of every CLAS centre inside its instance's
+//           geometry bounds, above it the instance index (one 64-bit key);
+//   sort    the LDS radix sort of tc_sort.hip (stable, CLAS order breaks ties);
+//   build   Karras' binary radix tree over each instance's key segment
+//           (internal nodes of instance i live at [s_i, s_i + n_i - 1));
+//   refit   bottom-up with one arrival counter per internal node;
+//   TLAS    the same builder over the instances' world-space boxes.
+#include <algorithm>
+
+#include "tc_bvh.h"
+
+namespace tc {
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton code of the box centre, normalised to [lo, hi]
+__host__ __device__ __forceinline__ uint32_t morton_of(const float* blo, const float* bhi, const float* lo,
+                                                       const float* hi) {
+  uint32_t code = 0;
+  for (int k = 0; k < 3; k++) {
+    const float c = (blo[k] + bhi[k]) * 0.5f;
+    const float ext = hi[k] - lo[k];
+    float n = ext > 0.0f ? (c - lo[k]) / ext : 0.5f;
+    n = fminf(fmaxf(n, 0.0f), 1.0f);
+    uint32_t q = (uint32_t)(n * 1023.0f + 0.5f);
+    q = (q | (q << 16)) & 0x030000FFu;
+    q = (q | (q << 8)) & 0x0300F00Fu;
+    q = (q | (q << 4)) & 0x030C30C3u;
+    q = (q | (q << 2)) & 0x09249249u;
+    code |= q << k;
+  }
+  return code;
+}
+
+// instance owning global reference position k (lists are contiguous in
+// instance order after blas_setup_insertion)
+__device__ __forceinline__ uint32_t owner_instance(const tc_blas_build_info* infos, uint32_t numInstances, uint64_t base,
+                                                   uint32_t k) {
+  uint32_t lo = 0, hi = numInstances;  // last i with start(i) <= k
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    const uint64_t s = (infos[mid].clusterReferences - base) / 8;
+    if (s <= k)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// BLAS keys over every inserted CLAS reference
+__global__ void k_blas_keys(BvhArgs a) {
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  const tc_blas_build_info* infos = ptr<const tc_blas_build_info>(b->blasBuildInfos);
+  const uint64_t* refs = ptr<const uint64_t>(b->blasClusterAddresses);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t inst = owner_instance(infos, a.numInstances, b->blasClusterAddresses, k);
+    const tc_clas_header h = *ptr<const tc_clas_header>(refs[k]);
+    const tc_render_instance& ri = a.instances[inst];
+    const uint32_t m = morton_of(h.lo, h.hi, ri.geoLo, ri.geoHi);
+    a.keys[k] = ((uint64_t)inst << 32) | m;
+    a.vals[k] = k;
+  }
+}
+
+// length of the common prefix of keys i and j (ties broken by index);
+// -1 outside [first, last]
+__device__ __forceinline__ int delta(const uint64_t* keys, int64_t i, int64_t j, int64_t first, int64_t last) {
+  if (j < first || j > last) return -1;
+  const uint64_t ki = keys[i], kj = keys[j];
+  if (ki == kj) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll(ki ^ kj);
+}
+
+// Karras (2012) radix tree for the segment [first, last] of sorted keys;
+// thread handles internal node `g` (global index first + local j).
+__device__ __forceinline__ void karras_node(const uint64_t* keys, int64_t first, int64_t last, int64_t i, uint32_t* child,
+                                            uint32_t* parent, uint32_t leafParentBase) {
+  const int d = (delta(keys, i, i + 1, first, last) - delta(keys, i, i - 1, first, last)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, i, i - d, first, last);
+  int64_t lmax = 2;
+  while (delta(keys, i, i + lmax * d, first, last) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, i, i + (l + t) * d, first, last) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = delta(keys, i, j, first, last);
+  int64_t s = 0;
+  int64_t t = l;
+  do {
+    t = (t + 1) / 2;
+    if (delta(keys, i, i + (s + t) * d, first, last) > dnode) s += t;
+  } while (t > 1);
+  const int64_t gamma = i + s * d + min(d, 0);
+  const int64_t lo = min(i, j), hi = max(i, j);
+  // children: leaf flag + position relative to `first`, or internal node
+  const uint32_t c0 = (lo == gamma) ? (TC_BVH_LEAF | (uint32_t)(gamma - first)) : (uint32_t)(gamma - first);
+  const uint32_t c1 = (hi == gamma + 1) ? (TC_BVH_LEAF | (uint32_t)(gamma + 1 - first)) : (uint32_t)(gamma + 1 - first);
+  const int64_t node = i - first;
+  child[2 * (first + node) + 0] = c0;
+  child[2 * (first + node) + 1] = c1;
+  // parents (global indices), leaves offset by leafParentBase
+  if (c0 & TC_BVH_LEAF)
+    parent[leafParentBase + first + (c0 & ~TC_BVH_LEAF)] = (uint32_t)(first + node);
+  else
+    parent[first + c0] = (uint32_t)(first + node);
+  if (c1 & TC_BVH_LEAF)
+    parent[leafParentBase + first + (c1 & ~TC_BVH_LEAF)] = (uint32_t)(first + node);
+  else
+    parent[first + c1] = (uint32_t)(first + node);
+}
+
+// segment [first, last] of instance inst in the sorted reference list
+__device__ __forceinline__ void blas_segment(const BvhArgs& a, uint32_t inst, uint32_t n, int64_t& first, int64_t& last) {
+  const tc_blas_build_info& bi = ptr<const tc_blas_build_info>(a.build->blasBuildInfos)[inst];
+  first = (int64_t)((bi.clusterReferences - a.build->blasClusterAddresses) / 8);
+  last = first + (int64_t)bi.clusterReferencesCount - 1;
+}
+
+// One thread per sorted position k: internal node k of its instance (when
+// k < last) and the sorted CLAS address / leaf box.
+__global__ void k_blas_hierarchy(BvhArgs a) {
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  const uint64_t* refs = ptr<const uint64_t>(b->blasClusterAddresses);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
+    int64_t first, last;
+    blas_segment(a, inst, n, first, last);
+    a.leafAddr[k] = refs[a.sortedVals[k]];
+    a.counter[k] = 0;
+    if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity);
+  }
+}
+
+__device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[3], float hi[3]) {
+  const tc_clas_header h = *ptr<const tc_clas_header>(a.leafAddr[k]);
+  for (int c = 0; c < 3; c++) lo[c] = h.lo[c], hi[c] = h.hi[c];
+}
+
+// Bottom-up refit: each leaf climbs; the second arrival at a node writes it.
+__global__ void k_blas_refit(BvhArgs a) {
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
+    int64_t first, last;
+    blas_segment(a, inst, n, first, last);
+    if (first == last) continue;  // single CLAS: the root is the leaf
+    uint32_t node = a.parent[a.capacity + k];
+    while (true) {
+      __threadfence();
+      if (atomicAdd(&a.counter[node], 1u) == 0) break;  // first arrival stops
+      __threadfence();
+      tc_bvh_node out;
+      float lo[2][3], hi[2][3];
+      for (int c = 0; c < 2; c++) {
+        const uint32_t ch = a.child[2 * node + c];
+        if (ch & TC_BVH_LEAF) {
+          leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
+        } else {
+          const tc_aabb bx = a.nodeBox[(uint32_t)first + ch];
+          for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+        }
+      }
+      for (int q = 0; q < 3; q++) {
+        out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
+      }
+      out.child0 = a.child[2 * node + 0];
+      out.child1 = a.child[2 * node + 1];
+      out._pad0 = out._pad1 = 0;
+      a.nodes[node] = out;
+      tc_aabb self;
+      for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
+      a.nodeBox[node] = self;
+      if ((int64_t)node == first) break;  // instance root
+      node = a.parent[node];
+    }
+  }
+}
+
+// Per-instance TLAS records, world boxes and TLAS keys (one thread per
+// instance).
+__global__ void k_tlas_prep(BvhArgs a, const float* xforms /* 24 floats per instance */) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.numInstances) return;
+  const tc_blas_build_info bi = ptr<const tc_blas_build_info>(a.build->blasBuildInfos)[i];
+  const uint32_t first = (uint32_t)((bi.clusterReferences - a.build->blasClusterAddresses) / 8);
+  const uint32_t cnt = bi.clusterReferencesCount;
+  tc_tlas_instance t;
+  for (int q = 0; q < 12; q++) t.objectToWorld[q] = xforms[24 * i + q], t.worldToObject[q] = xforms[24 * i + 12 + q];
+  t.blasNodes = reinterpret_cast<uint64_t>(a.nodes + first);
+  t.blasLeaves = reinterpret_cast<uint64_t>(a.leafAddr + first);
+  t.blasRoot = cnt == 1 ? TC_BVH_LEAF : 0u;
+  t.numClusters = cnt;
+  t.flags = 1;
+  t._pad = 0;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (cnt == 1) {
+    const tc_clas_header h = *ptr<const tc_clas_header>(a.leafAddr[first]);
+    for (int q = 0; q < 3; q++) lo[q] = h.lo[q], hi[q] = h.hi[q];
+  } else if (cnt > 1) {
+    const tc_aabb bx = a.nodeBox[first];
+    for (int q = 0; q < 3; q++) lo[q] = bx.lo[q], hi[q] = bx.hi[q];
+  }
+  a.tlasInst[i] = t;
+  // world box of the object box corners
+  float wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (cnt > 0)
+    for (int c = 0; c < 8; c++) {
+      const float p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+      for (int r = 0; r < 3; r++) {
+        const float* m = t.objectToWorld + 4 * r;
+        const float w = ((m[0] * p[0] + m[1] * p[1]) + m[2] * p[2]) + m[3];
+        wlo[r] = fminf(wlo[r], w);
+        whi[r] = fmaxf(whi[r], w);
+      }
+    }
+  // widen by a few ULP-scale steps so rounding of the transformed corners
+  // never cuts off a hit found in object space
+  tc_aabb wb;
+  for (int q = 0; q < 3; q++) {
+    const float e = cnt > 0 ? 1e-6f * fmaxf(fmaxf(fabsf(wlo[q]), fabsf(whi[q])), 1e-3f) : 0.0f;
+    wb.lo[q] = wlo[q] - e, wb.hi[q] = whi[q] + e;
+  }
+  a.tlasLeafBox[i] = wb;
+}
+
+// Scene bounds and TLAS keys over instances (one workgroup).
+__global__ __launch_bounds__(256) void k_tlas_keys(BvhArgs a) {
+  __shared__ float s_lo[3][256], s_hi[3][256];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = threadIdx.x; i < a.numInstances; i += blockDim.x) {
+    const tc_aabb bx = a.tlasLeafBox[i];
+    if (!(bx.lo[0] <= bx.hi[0])) continue;
+    for (int q = 0; q < 3; q++) lo[q] = fminf(lo[q], (bx.lo[q] + bx.hi[q]) * 0.5f), hi[q] = fmaxf(hi[q], (bx.lo[q] + bx.hi[q]) * 0.5f);
+  }
+  for (int q = 0; q < 3; q++) s_lo[q][threadIdx.x] = lo[q], s_hi[q][threadIdx.x] = hi[q];
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 3; q++) {
+        s_lo[q][threadIdx.x] = fminf(s_lo[q][threadIdx.x], s_lo[q][threadIdx.x + s]);
+        s_hi[q][threadIdx.x] = fmaxf(s_hi[q][threadIdx.x], s_hi[q][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  const float slo[3] = {s_lo[0][0], s_lo[1][0], s_lo[2][0]}, shi[3] = {s_hi[0][0], s_hi[1][0], s_hi[2][0]};
+  for (uint32_t i = threadIdx.x; i < a.numInstances; i += blockDim.x) {
+    const tc_aabb bx = a.tlasLeafBox[i];
+    const bool empty = !(bx.lo[0] <= bx.hi[0]);
+    // empty instances sort last and are never entered
+    a.tlasKeys[i] = empty ? (0xFFFFFFFFull << 32) | i : ((uint64_t)morton_of(bx.lo, bx.hi, slo, shi) << 32) | i;
+    a.tlasVals[i] = i;
+  }
+}
+
+// TLAS over the (sorted) instances: one thread per sorted position.
+__global__ void k_tlas_hierarchy(BvhArgs a) {
+  const uint32_t n = a.numInstances;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    a.tlasCounter[k] = 0;
+    if (k + 1 < n) karras_node(a.tlasSortedKeys, 0, (int64_t)n - 1, k, a.tlasChild, a.tlasParent, n);
+  }
+}
+
+__global__ void k_tlas_refit(BvhArgs a) {
+  const uint32_t n = a.numInstances;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (n == 1) continue;
+    uint32_t node = a.tlasParent[n + k];
+    while (true) {
+      __threadfence();
+      if (atomicAdd(&a.tlasCounter[node], 1u) == 0) break;
+      __threadfence();
+      tc_bvh_node out;
+      float lo[2][3], hi[2][3];
+      for (int c = 0; c < 2; c++) {
+        const uint32_t ch = a.tlasChild[2 * node + c];
+        tc_aabb bx = (ch & TC_BVH_LEAF) ? a.tlasLeafBox[a.tlasSortedVals[ch & ~TC_BVH_LEAF]] : a.tlasNodeBox[ch];
+        for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+      }
+      for (int q = 0; q < 3; q++) out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
+      const uint32_t c0 = a.tlasChild[2 * node], c1 = a.tlasChild[2 * node + 1];
+      // leaves of the TLAS name instances directly
+      out.child0 = (c0 & TC_BVH_LEAF) ? (TC_BVH_LEAF | a.tlasSortedVals[c0 & ~TC_BVH_LEAF]) : c0;
+      out.child1 = (c1 & TC_BVH_LEAF) ? (TC_BVH_LEAF | a.tlasSortedVals[c1 & ~TC_BVH_LEAF]) : c1;
+      out._pad0 = out._pad1 = 0;
+      a.tlasNodes[node] = out;
+      tc_aabb self;
+      for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
+      a.tlasNodeBox[node] = self;
+      if (node == 0) break;
+      node = a.tlasParent[node];
+    }
+  }
+}
+
+// TLAS root word (internal node 0, or the single instance) and the TLAS
+// element count read by the sort
+__global__ void k_tlas_root(BvhArgs a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    a.tlasRoot[0] = a.numInstances == 1 ? (TC_BVH_LEAF | 0u) : 0u;
+    a.tlasRoot[1] = a.numInstances;
+  }
+}
+
+}  // namespace tc
+
+// ------------------------------------------------------------ host entry
+
+namespace tc {
+
+static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
+
+AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances) {
+  AccelLayout L{};
+  uint64_t o = 0;
+  auto put = [&](uint64_t& f, uint64_t bytes) {
+    f = o;
+    o = a256(o + bytes);
+  };
+  const uint64_t c = capacity, n = std::max(numInstances, 1u);
+  put(L.blasNodes, c * sizeof(tc_bvh_node));
+  put(L.blasLeaves, c * 8);
+  put(L.nodeBox, c * sizeof(tc_aabb));
+  put(L.child, 2 * c * 4);
+  put(L.parent, 2 * c * 4);
+  put(L.counter, c * 4);
+  put(L.tlasInstances, n * sizeof(tc_tlas_instance));
+  put(L.tlasNodes, n * sizeof(tc_bvh_node));
+  put(L.tlasNodeBox, n * sizeof(tc_aabb));
+  put(L.tlasLeafBox, n * sizeof(tc_aabb));
+  put(L.tlasChild, 2 * n * 4);
+  put(L.tlasParent, 2 * n * 4);
+  put(L.tlasCounter, n * 4);
+  put(L.tlasRoot, 256);
+  L.total = o;
+  return L;
+}
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" {
+
+uint64_t tc_accel_size(uint32_t capacity, uint32_t numInstances) { return accel_layout(capacity, numInstances).total; }
+
+tc_status tc_accel_bind(void* memory, uint64_t bytes, uint32_t capacity, uint32_t numInstances, tc_accel* out) {
+  if (!memory || !out || !capacity || !numInstances) return TC_ERROR_INVALID_ARGUMENT;
+  AccelLayout L = accel_layout(capacity, numInstances);
+  if (bytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
+  char* m = static_cast<char*>(memory);
+  out->blasNodes = reinterpret_cast<tc_bvh_node*>(m + L.blasNodes);
+  out->blasLeaves = reinterpret_cast<uint64_t*>(m + L.blasLeaves);
+  out->tlasInstances = reinterpret_cast<tc_tlas_instance*>(m + L.tlasInstances);
+  out->tlasNodes = reinterpret_cast<tc_bvh_node*>(m + L.tlasNodes);
+  out->tlasRoot = reinterpret_cast<uint32_t*>(m + L.tlasRoot);
+  out->numInstances = numInstances;
+  out->capacity = capacity;
+  out->memory = memory;
+  out->memoryBytes = bytes;
+  return TC_OK;
+}
+
+tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel* accel, const float* instanceXforms,
+                         tc_stream stream) {
+  KArgs ka;
+  tc_status s = make_args_public(b, config, ka);
+  if (s != TC_OK) return s;
+  if (!accel || !accel->memory || !instanceXforms || accel->numInstances != b->numRenderInstances) return TC_ERROR_INVALID_ARGUMENT;
+  const uint32_t capacity = accel->capacity;
+  if (capacity < config.maxGeneratedClusters) return TC_ERROR_INVALID_ARGUMENT;
+  AccelLayout L = accel_layout(capacity, accel->numInstances);
+  if (accel->memoryBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
+  WsLayout W = ws_layout(config, b->numRenderInstances);
+  if (W.sortCapacity < capacity) return TC_ERROR_INVALID_ARGUMENT;
+  char* m = static_cast<char*>(accel->memory);
+  char* w = static_cast<char*>(b->workspace);
+  hipStream_t st = (hipStream_t)stream;
+
+  BvhArgs a;
+  a.build = b->build;
+  a.instances = b->instances;
+  a.numInstances = b->numRenderInstances;
+  a.capacity = capacity;
+  SortBuffers sb;
+  for (int i = 0; i < 2; i++) {
+    sb.keys[i] = reinterpret_cast<uint64_t*>(w + W.sortKeys[i]);
+    sb.vals[i] = reinterpret_cast<uint32_t*>(w + W.sortVals[i]);
+  }
+  sb.hist = reinterpret_cast<uint32_t*>(w + W.sortHist);
+  sb.digitTotals = reinterpret_cast<uint32_t*>(w + W.sortAux);
+  sb.capacity = W.sortCapacity;
+  sb.maxTiles = W.sortMaxTiles;
+  a.keys = sb.keys[0];
+  a.vals = sb.vals[0];
+  a.leafAddr = reinterpret_cast<uint64_t*>(m + L.blasLeaves);
+  a.child = reinterpret_cast<uint32_t*>(m + L.child);
+  a.parent = reinterpret_cast<uint32_t*>(m + L.parent);
+  a.counter = reinterpret_cast<uint32_t*>(m + L.counter);
+  a.nodes = reinterpret_cast<tc_bvh_node*>(m + L.blasNodes);
+  a.nodeBox = reinterpret_cast<tc_aabb*>(m + L.nodeBox);
+  a.tlasInst = reinterpret_cast<tc_tlas_instance*>(m + L.tlasInstances);
+  a.tlasNodes = reinterpret_cast<tc_bvh_node*>(m + L.tlasNodes);
+  a.tlasNodeBox = reinterpret_cast<tc_aabb*>(m + L.tlasNodeBox);
+  a.tlasLeafBox = reinterpret_cast<tc_aabb*>(m + L.tlasLeafBox);
+  a.tlasChild = reinterpret_cast<uint32_t*>(m + L.tlasChild);
+  a.tlasParent = reinterpret_cast<uint32_t*>(m + L.tlasParent);
+  a.tlasCounter = reinterpret_cast<uint32_t*>(m + L.tlasCounter);
+  a.tlasRoot = reinterpret_cast<uint32_t*>(m + L.tlasRoot);
+
+  const uint32_t grid = std::min<uint32_t>((capacity + 255) / 256, (uint32_t)num_cu_public() * 8);
+  // BLAS: keys, sort by (instance, Morton), hierarchy, refit
+  hipLaunchKernelGGL(k_blas_keys, dim3(grid), dim3(256), 0, st, a);
+  uint32_t instBits = 1;
+  while ((1u << instBits) < a.numInstances) instBits++;
+  uint32_t res = 0;
+  radix_sort_pairs(sb, &b->build->blasClusterCounter, 0, 32 + instBits, st, &res);
+  a.sortedKeys = sb.keys[res];
+  a.sortedVals = sb.vals[res];
+  hipLaunchKernelGGL(k_blas_hierarchy, dim3(grid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_blas_refit, dim3(grid), dim3(256), 0, st, a);
+  // TLAS over instances; its sort reuses the other ping-pong slot
+  a.tlasKeys = sb.keys[res ^ 1];
+  a.tlasVals = sb.vals[res ^ 1];
+  hipLaunchKernelGGL(k_tlas_prep, dim3((a.numInstances + 63) / 64), dim3(64), 0, st, a, instanceXforms);
+  hipLaunchKernelGGL(k_tlas_keys, dim3(1), dim3(256), 0, st, a);
+  // the TLAS sort runs on the second ping-pong slot of the workspace; its
+  // scratch slot is the TLAS child / parent arrays, written only afterwards
+  // (32 key bits = 4 passes, so the result lands back in slot 0)
+  SortBuffers tb = sb;
+  tb.keys[0] = sb.keys[res ^ 1];
+  tb.vals[0] = sb.vals[res ^ 1];
+  tb.keys[1] = reinterpret_cast<uint64_t*>(m + L.tlasChild);
+  tb.vals[1] = reinterpret_cast<uint32_t*>(m + L.tlasParent);
+  tb.capacity = a.numInstances;
+  tb.maxTiles = (a.numInstances + kSortTile - 1) / kSortTile;
+  hipLaunchKernelGGL(k_tlas_root, dim3(1), dim3(64), 0, st, a);
+  uint32_t tres = 0;
+  radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);
+  a.tlasSortedKeys = tb.keys[tres];
+  a.tlasSortedVals = tb.vals[tres];
+  const uint32_t tgrid = (a.numInstances + 255) / 256;
+  hipLaunchKernelGGL(k_tlas_hierarchy, dim3(tgrid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_tlas_refit, dim3(tgrid), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+}  // extern "C"

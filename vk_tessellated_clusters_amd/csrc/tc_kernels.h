@@ -6,7 +6,7 @@
 namespace tc {
 
 constexpr uint32_t kMaxClusterVerts = 256;
-constexpr uint32_t kMaxClusterTris = 256;
+constexpr uint32_t kMaxClusterTris = 128;
 
 // Everything a stage kernel reads: the bindings (device pointers), the
 // configuration and the look-back workspace.

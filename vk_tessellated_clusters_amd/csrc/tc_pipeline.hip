@@ -7,13 +7,11 @@
 // offsets a serial execution would produce.  Output order is therefore the
 // canonical serial order: (visible cluster, triangle) for classify,
 // (split item, child) for split, part index for instantiate.
-#include "tc_kernels.h"
+#include "tc_stage.h"
 
 namespace tc {
 
 // ------------------------------------------------------------ prologue
-
-enum Stage : uint32_t { kStageClassify = 0, kStageSplit = 1, kStageInstantiate = 2 };
 
 // Opens a new look-back launch: bumps the epoch (clearing stale status
 // words when it wraps), resets the ticket and snapshots the counters the
@@ -39,8 +37,8 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
   h->epoch = e;
   h->ticket = 0;
   if (stage == kStageClassify) {
-    const uint32_t n = min(min(b->visibleClusterCounter, a.cfg.maxVisibleClusters), a.maxTiles);
-    h->numTiles = n;
+    const uint32_t n = min(b->visibleClusterCounter, a.cfg.maxVisibleClusters);
+    h->numTiles = min((n + kClassifyTile - 1) / kClassifyTile, a.maxTiles);
     h->itemBegin = 0;
     h->itemEnd = n;
     h->base[0] = (uint32_t)b->splitTriangleCounter;
@@ -52,14 +50,14 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
   } else if (stage == kStageSplit) {
     uint32_t s = min(b->splitPassStart, a.cfg.maxSplitTriangles), en = min(b->splitPassEnd, a.cfg.maxSplitTriangles);
     uint32_t n = en > s ? en - s : 0;
-    h->numTiles = (n + kWave - 1) / kWave;
+    h->numTiles = min((n + kWave - 1) / kWave, a.maxTiles);
     h->itemBegin = s;
     h->itemEnd = s + n;
     h->base[0] = b->splitWriteCounter;
     h->base[1] = b->partTriangleCounter;
   } else {
     uint32_t n = min(b->partTriangleCounter, a.cfg.maxPartTriangles);
-    h->numTiles = (n + kWave - 1) / kWave;
+    h->numTiles = min((n + kInstTile - 1) / kInstTile, a.maxTiles);
     h->itemBegin = 0;
     h->itemEnd = n;
     h->base[0] = b->genClusterCounter;
@@ -128,214 +126,6 @@ __global__ void k_build_setup(KArgs a, int mode) {
   }
 }
 
-// ------------------------------------------------------ cluster_classify
-
-__device__ __forceinline__ ViewParams view_params(const KArgs& a) {
-  ViewParams vp;
-  vp.eye = mk(a.build->viewPos[0], a.build->viewPos[1], a.build->viewPos[2]);
-  vp.nearPlane = a.view->nearPlane;
-  vp.vpy = a.view->viewportf[1];
-  vp.rate = a.view->tessRate;
-  return vp;
-}
-
-__device__ __forceinline__ bool has_disp(const KArgs& a, int32_t idx) {
-  return (a.cfg.flags & TC_FLAG_DISPLACEMENT) && idx >= 0 && (uint32_t)idx < a.numTextures;
-}
-
-__device__ __forceinline__ float displaced_height(const KArgs& a, const tc_displacement_texture& tex, float scale,
-                                                  float offset, v2 uv) {
-  float h = sample_height(tex, uv.x, uv.y);
-  return (((h * scale) * a.view->displacementScale) + offset) + a.view->displacementOffset;
-}
-
-__device__ __forceinline__ uint64_t max_data_bytes(const tc_config& c) {
-  return (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull;
-}
-
-// cluster_classify.comp.glsl:150-666, one visible cluster per tile, one
-// wave per workgroup.  Counters: 0 split, 1 part, 2 genCluster,
-// 3 genVertex, 4 genData.
-__global__ __launch_bounds__(64) void k_cluster_classify(KArgs a) {
-  __shared__ float s_pos[kMaxClusterVerts * 3];
-  __shared__ uint32_t s_fac[kMaxClusterTris * 3];
-
-  WsHeader* h = a.ws;
-  const uint32_t lane = lane_id();
-  const uint32_t epoch = h->epoch, numTiles = h->numTiles;
-  const uint64_t B0 = h->base[0], B1 = h->base[1], B2 = h->base[2], B3 = h->base[3], B4 = h->base[4], B5 = h->base[5];
-  const ViewParams vp = view_params(a);
-  tc_scene_building* b = a.build;
-  const tc_config cfg = a.cfg;
-
-  while (true) {
-    const uint32_t tile = take_ticket(h);
-    if (tile >= numTiles) break;
-
-    tc_cluster_info ci = ptr<const tc_cluster_info>(b->visibleClusters)[tile];
-    // malformed entries are reduced to an in-bounds cluster
-    if (ci.instanceID >= a.numInstances) ci.instanceID = 0;
-    const tc_render_instance& inst = a.instances[ci.instanceID];
-    if (ci.clusterID >= inst.numClusters) ci.clusterID = 0;
-    const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ci.clusterID];
-    const uint32_t state = ptr<const uint32_t>(b->instanceStates)[ci.instanceID];
-    const uint32_t nv = min((uint32_t)cl.numVertices, kMaxClusterVerts), nt = min((uint32_t)cl.numTriangles, kMaxClusterTris);
-    const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
-
-    // :177-195 stage object-space vertices
-    for (uint32_t v = lane; v < nv; v += kWave) {
-      v3 p = ld3(inst.positions, cl.firstLocalVertex + v);
-      s_pos[v * 3 + 0] = p.x;
-      s_pos[v * 3 + 1] = p.y;
-      s_pos[v * 3 + 2] = p.z;
-    }
-    __syncthreads();
-
-    // :200-246 tessellation factors on world-space undisplaced vertices
-    uint32_t simple = 0;
-    const bool invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
-    if (invisible) {
-      simple = nt;
-    } else {
-      float m[16];
-#pragma unroll
-      for (int i = 0; i < 16; i++) m[i] = inst.worldMatrix[i];
-      for (uint32_t t0 = 0; t0 < nt; t0 += kWave) {
-        uint32_t t = t0 + lane;
-        bool valid = t < nt;
-        uint32_t f[3] = {1, 1, 1};
-        if (valid) {
-          uint32_t i0 = lt[t * 3 + 0], i1 = lt[t * 3 + 1], i2 = lt[t * 3 + 2];
-          v3 w0 = xform_point(m, mk(s_pos[i0 * 3], s_pos[i0 * 3 + 1], s_pos[i0 * 3 + 2]));
-          v3 w1 = xform_point(m, mk(s_pos[i1 * 3], s_pos[i1 * 3 + 1], s_pos[i1 * 3 + 2]));
-          v3 w2 = xform_point(m, mk(s_pos[i2 * 3], s_pos[i2 * 3 + 1], s_pos[i2 * 3 + 2]));
-          tess_factors(w0, w1, w2, vp, f);
-          s_fac[t * 3 + 0] = f[0];
-          s_fac[t * 3 + 1] = f[1];
-          s_fac[t * 3 + 2] = f[2];
-        }
-        simple += __popcll(ballot(valid && umax3(f[0], f[1], f[2]) == 1));
-      }
-    }
-    const bool full = simple == nt;
-
-    // tile aggregates
-    uint64_t agg[5] = {0, 0, 0, 0, 0};
-    uint64_t dataSize = 0;
-    if (full) {
-      dataSize = ptr<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID];
-      agg[2] = 1;
-      agg[3] = nv;
-      agg[4] = dataSize;
-    } else {
-      uint32_t ns = 0, np = 0;
-      for (uint32_t t0 = 0; t0 < nt; t0 += kWave) {
-        uint32_t t = t0 + lane;
-        bool valid = t < nt;
-        uint32_t mx = valid ? umax3(s_fac[t * 3], s_fac[t * 3 + 1], s_fac[t * 3 + 2]) : 0;
-        ns += __popcll(ballot(valid && mx > TC_TESSTABLE_SIZE));
-        np += __popcll(ballot(valid && mx <= TC_TESSTABLE_SIZE));
-      }
-      agg[0] = ns;
-      agg[1] = np;
-    }
-    uint64_t excl[5];
-    lookback<5>(a.status, a.maxTiles, epoch, tile, agg, excl);
-
-    if (full) {
-      // :300-445 template instantiation of the whole cluster
-      const uint32_t genOffset = (uint32_t)(B2 + excl[2]);
-      const uint32_t vertexOffset = (uint32_t)(B3 + excl[3]);
-      const uint64_t dataOffset = B4 + excl[4];
-      const bool ok = !((uint64_t)vertexOffset + nv > cfg.maxGeneratedVertices ||
-                        (uint64_t)genOffset + 1 > cfg.maxGeneratedClusters || dataOffset + dataSize > max_data_bytes(cfg));
-      if (lane == 0) {
-        atomicAdd(&a.readback->numFullClusters, 1u);
-        if (ok) {
-          // accepted clusters form a prefix of the canonical order, so the
-          // template slot is the launch's first slot plus the cluster rank
-          const uint32_t tempOffset = (uint32_t)(B5 + (genOffset - B2));
-          atomicAdd(&b->tempInstantiateCounter, 1u);
-          tc_template_instantiate_info ti;
-          ti.clusterIdOffset = 0;
-          ti.geometryIndexOffset = 0;
-          ti.clusterTemplateAddress = ptr<const uint64_t>(inst.clusterTemplateAdresses)[ci.clusterID];
-          ti.vertexBufferAddress = b->genVertices + (uint64_t)vertexOffset * 12ull;
-          ti.vertexBufferStride = 12;
-          ptr<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
-          ptr<uint32_t>(b->tempInstanceIDs)[tempOffset] = ci.instanceID;
-          ptr<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
-          atomicAdd(&ptr<tc_blas_build_info>(b->blasBuildInfos)[ci.instanceID].clusterReferencesCount, 1u);
-          atomicAdd(&a.readback->numTotalTriangles, simple);
-        }
-      }
-      if (ok) {
-        // :459-488 displaced cluster vertices
-        const bool disp = has_disp(a, inst.displacementIndex);
-        tc_displacement_texture tex;
-        if (disp) tex = a.textures[inst.displacementIndex];
-        for (uint32_t v = lane; v < nv; v += kWave) {
-          v3 p = mk(s_pos[v * 3], s_pos[v * 3 + 1], s_pos[v * 3 + 2]);
-          if (disp) {
-            uint32_t vi = cl.firstLocalVertex + v;
-            float hgt = displaced_height(a, tex, inst.displacementScale, inst.displacementOffset, ld2(inst.texcoords, vi));
-            p = add(p, mul(normalize(ld3(inst.normals, vi)), hgt));
-          }
-          st3(b->genVertices, (uint64_t)vertexOffset + v, p);
-        }
-      }
-    } else {
-      // :543-666 split / part emission in triangle order
-      uint32_t runSplit = (uint32_t)(B0 + excl[0]);
-      uint32_t runPart = (uint32_t)(B1 + excl[1]);
-      for (uint32_t t0 = 0; t0 < nt; t0 += kWave) {
-        uint32_t t = t0 + lane;
-        bool valid = t < nt;
-        uint32_t f[3] = {0, 0, 0};
-        if (valid) f[0] = s_fac[t * 3], f[1] = s_fac[t * 3 + 1], f[2] = s_fac[t * 3 + 2];
-        uint32_t mx = umax3(f[0], f[1], f[2]);
-        bool needSplit = valid && mx > TC_TESSTABLE_SIZE;
-        bool needPart = valid && mx <= TC_TESSTABLE_SIZE;
-        uint64_t vs = ballot(needSplit), vpm = ballot(needPart);
-        uint32_t offSplit = runSplit + mask_prefix(vs);
-        uint32_t offPart = runPart + mask_prefix(vpm);
-        runSplit += __popcll(vs);
-        runPart += __popcll(vpm);
-        tc_tess_triangle_info ti;
-        ti.cluster = ci;
-        uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
-        if (needSplit && offSplit < cfg.maxSplitTriangles) {
-          f[0] = split_factor(f[0], cfg.maxSplitFactor);
-          f[1] = split_factor(f[1], cfg.maxSplitFactor);
-          f[2] = split_factor(f[2], cfg.maxSplitFactor);
-          uint32_t cf = get_config(f, vtx);
-          ti.subTriangle.vtxEncoded[0] = vtx[0];
-          ti.subTriangle.vtxEncoded[1] = vtx[1];
-          ti.subTriangle.vtxEncoded[2] = vtx[2];
-          ti.subTriangle.triangleID_config = t | (cf << 16);
-          ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-        } else if (needPart && offPart < cfg.maxPartTriangles) {
-          uint32_t cf = get_config(f, vtx);
-          ti.subTriangle.vtxEncoded[0] = vtx[0];
-          ti.subTriangle.vtxEncoded[1] = vtx[1];
-          ti.subTriangle.vtxEncoded[2] = vtx[2];
-          ti.subTriangle.triangleID_config = t | (cf << 16);
-          ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
-        }
-      }
-    }
-
-    if (tile == numTiles - 1 && lane == 0) {
-      b->splitTriangleCounter = (int32_t)(uint32_t)(B0 + excl[0] + agg[0]);
-      b->partTriangleCounter = (uint32_t)(B1 + excl[1] + agg[1]);
-      b->genClusterCounter = (uint32_t)(B2 + excl[2] + agg[2]);
-      b->genVertexCounter = (uint32_t)(B3 + excl[3] + agg[3]);
-      b->genClusterDataCounter = B4 + excl[4] + agg[4];
-    }
-    __syncthreads();
-  }
-}
-
 // -------------------------------------------------------- triangle_split
 
 struct SplitItem {
@@ -366,13 +156,6 @@ __device__ __forceinline__ void split_child(const TableView& T, const SplitItem&
   tess_factors(wp[0], wp[1], wp[2], vp, f);
 }
 
-__device__ __forceinline__ uint32_t find_owner(const uint32_t* s_start, uint32_t t) {
-  uint32_t lo = 0;
-#pragma unroll
-  for (uint32_t step = 32; step > 0; step >>= 1)
-    if (lo + step < (uint32_t)kWave && s_start[lo + step] <= t) lo += step;
-  return lo;
-}
 
 // triangle_split.comp.glsl:598-632 + processAllSubTasks :350-472, one tile
 // of 64 split items per wave; children packed densely over the wave.
@@ -498,162 +281,6 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
     __syncthreads();
   }
 }
-
-// ---------------------------------------- triangle_tess_template_instantiate
-
-struct PartBase {
-  v3 bary[3];
-  v3 pos[3];
-  v3 nrm[3];
-  v2 uv[3];
-  uint32_t cfg;
-  uint32_t firstVertex;
-  uint32_t vertexOffset;
-  uint32_t start;  // first virtual thread of this part
-  int32_t dispIndex;
-  float dispScale;
-  float dispOffset;
-  uint32_t _pad;
-};
-
-// triangle_tess_template_instantiate.comp.glsl:126-373, 64 part triangles
-// per tile.  Counters: 0 genCluster, 1 genVertex, 2 genData.
-template <bool PN>
-__global__ __launch_bounds__(64) void k_instantiate(KArgs a) {
-  __shared__ PartBase s_parts[kWave];
-  __shared__ PNBasis s_pn[PN ? kWave : 1];
-  __shared__ uint32_t s_start[kWave];
-
-  WsHeader* h = a.ws;
-  const uint32_t lane = lane_id();
-  const uint32_t epoch = h->epoch, numTiles = h->numTiles, numParts = h->itemEnd;
-  const uint64_t G0 = h->base[0], V0 = h->base[1], D0 = h->base[2], T0 = h->base[3];
-  const TableView T = load_table(a.tessTable);
-  tc_scene_building* b = a.build;
-  const tc_config cfg = a.cfg;
-
-  while (true) {
-    const uint32_t tile = take_ticket(h);
-    if (tile >= numTiles) break;
-
-    const uint32_t p = tile * kWave + lane;
-    const bool inRange = p < numParts;
-    tc_tess_triangle_info ti;
-    uint32_t cf = 0, nv = 0, nt = 0;
-    uint64_t dataSize = 0;
-    uint32_t firstVertex = 0;
-    if (inRange) ti = ptr<const tc_tess_triangle_info>(b->partTriangles)[p];
-    // a malformed record allocates like a (1,1,1) part but generates nothing
-    const bool wellFormed = inRange && ti.cluster.instanceID < a.numInstances &&
-                            cfg_index(ti.subTriangle.triangleID_config >> 16) < TC_TESSTABLE_NUM_LOOKUP &&
-                            ti.cluster.clusterID < a.instances[ti.cluster.instanceID].numClusters;
-    const bool valid = inRange;
-    if (valid) {
-      if (!wellFormed) ti.subTriangle.triangleID_config = 0, ti.cluster.instanceID = 0, ti.cluster.clusterID = 0;
-      cf = ti.subTriangle.triangleID_config >> 16;
-      tc_tess_table_entry e = cfg_entry(T, cf);
-      nv = e.numVertices;
-      nt = e.numTriangles;
-      firstVertex = e.firstVertex;
-      dataSize = T.templateSizes[cfg_index(cf)];
-    }
-    // :155-160 allocation in part order
-    const uint64_t validMask = ballot(valid);
-    const uint32_t vIncl = wave_incl_scan(nv);
-    const uint64_t dIncl = wave_incl_scan64(dataSize);
-    uint64_t agg[3] = {(uint64_t)__popcll(validMask), (uint64_t)shfl(vIncl, kWave - 1), shfl64(dIncl, kWave - 1)};
-    uint64_t excl[3];
-    lookback<3>(a.status, a.maxTiles, epoch, tile, agg, excl);
-
-    const uint32_t genOffset = (uint32_t)(G0 + excl[0] + mask_prefix(validMask));
-    const uint32_t vertexOffset = (uint32_t)(V0 + excl[1] + (vIncl - nv));
-    const uint64_t dataOffset = D0 + excl[2] + (dIncl - dataSize);
-    const bool ok = valid && !((uint64_t)vertexOffset + nv > cfg.maxGeneratedVertices ||
-                               (uint64_t)genOffset + 1 > cfg.maxGeneratedClusters || dataOffset + dataSize > max_data_bytes(cfg));
-    const uint64_t okMask = ballot(ok);
-    if (ok) {
-      // :172-198; accepted parts are a prefix of the canonical order
-      const uint32_t tempOffset = (uint32_t)(T0 + (genOffset - G0));
-      tc_template_instantiate_info info;
-      info.clusterIdOffset = p | (TC_RT_CLUSTER_MODE_SINGLE_TESSELLATED << 30);
-      info.geometryIndexOffset = 0;
-      info.clusterTemplateAddress = T.templateAddresses[cfg_index(cf)];
-      info.vertexBufferAddress = b->genVertices + (uint64_t)vertexOffset * 12ull;
-      info.vertexBufferStride = 12;
-      ptr<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = info;
-      ptr<uint32_t>(b->tempInstanceIDs)[tempOffset] = ti.cluster.instanceID;
-      ptr<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
-      atomicAdd(&ptr<tc_blas_build_info>(b->blasBuildInfos)[ti.cluster.instanceID].clusterReferencesCount, 1u);
-    }
-    const uint32_t triSum = wave_sum(ok ? nt : 0u);
-    if (lane == 0 && okMask) {
-      atomicAdd(&b->tempInstantiateCounter, (uint32_t)__popcll(okMask));
-      atomicAdd(&a.readback->numTotalTriangles, triSum);
-    }
-
-    // stage the base triangle of every accepted part (:295-337)
-    const uint32_t gen = ok ? nv : 0u;
-    const uint32_t gIncl = wave_incl_scan(gen);
-    const uint32_t totalVerts = shfl(gIncl, kWave - 1);
-    s_start[lane] = ok ? gIncl - gen : 0xFFFFFFFFu;
-    if (ok) {
-      const tc_render_instance& inst = a.instances[ti.cluster.instanceID];
-      const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
-      const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
-      const uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
-      PartBase pb;
-#pragma unroll
-      for (int v = 0; v < 3; v++) {
-        uint32_t vi = lt[tri * 3 + v] + cl.firstLocalVertex;
-        pb.bary[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
-        pb.pos[v] = ld3(inst.positions, vi);
-        pb.nrm[v] = normalize(ld3(inst.normals, vi));
-        pb.uv[v] = ld2(inst.texcoords, vi);
-      }
-      pb.cfg = cf;
-      pb.firstVertex = firstVertex;
-      pb.vertexOffset = vertexOffset;
-      pb.start = gIncl - gen;
-      pb.dispIndex = has_disp(a, inst.displacementIndex) ? inst.displacementIndex : -1;
-      pb.dispScale = inst.displacementScale;
-      pb.dispOffset = inst.displacementOffset;
-      s_parts[lane] = pb;
-      if (PN) setup_pn(s_pn[lane], pb.pos[0], pb.pos[1], pb.pos[2], pb.nrm[0], pb.nrm[1], pb.nrm[2]);
-    }
-    __syncthreads();
-
-    // :346-372 one generated vertex per virtual thread; output indices of
-    // consecutive virtual threads are consecutive
-    for (uint32_t t0 = 0; t0 < totalVerts; t0 += kWave) {
-      const uint32_t t = t0 + lane;
-      if (t >= totalVerts) break;
-      const uint32_t owner = find_owner(s_start, t);
-      const PartBase& pb = s_parts[owner];
-      const uint32_t vert = t - pb.start;
-      v3 w = cfg_vertex_bary(T, pb.cfg, pb.firstVertex, vert);
-      w = interp(pb.bary[0], pb.bary[1], pb.bary[2], w);
-      v3 pos = PN ? get_pn(s_pn[owner], w) : interp(pb.pos[0], pb.pos[1], pb.pos[2], w);
-      if (pb.dispIndex >= 0) {
-        const tc_displacement_texture tex = a.textures[pb.dispIndex];
-        v3 n = interp(pb.nrm[0], pb.nrm[1], pb.nrm[2], w);
-        v2 uv = interp2(pb.uv[0], pb.uv[1], pb.uv[2], w);
-        float hgt = displaced_height(a, tex, pb.dispScale, pb.dispOffset, uv);
-        pos = add(pos, mul(normalize(n), hgt));
-      }
-      st3(b->genVertices, (uint64_t)pb.vertexOffset + vert, pos);
-    }
-
-    if (tile == numTiles - 1 && lane == 0) {
-      b->genClusterCounter = (uint32_t)(G0 + excl[0] + agg[0]);
-      b->genVertexCounter = (uint32_t)(V0 + excl[1] + agg[1]);
-      b->genClusterDataCounter = D0 + excl[2] + agg[2];
-    }
-    __syncthreads();
-  }
-}
-
-template __global__ void k_instantiate<false>(KArgs);
-template __global__ void k_instantiate<true>(KArgs);
 
 // ------------------------------------------------- blas_setup_insertion
 

@@ -28,9 +28,36 @@ struct Renderer {
   tc_scene_building hostBuild{};
   tc_frame_constants* pinnedView = nullptr;
   uint64_t splitBytes = 0;
+  float* xforms = nullptr;  // per instance objectToWorld | worldToObject (3x4 row-major)
+  tc_accel accel{};
 };
 
 static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
+
+// inverse of a 4x4 row-major matrix in double precision (cofactors)
+static bool invert4(const double m[16], double out[16]) {
+  double inv[16];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  const double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  if (det == 0.0) return false;
+  for (int i = 0; i < 16; i++) out[i] = inv[i] / det;
+  return true;
+}
 
 // Resets the per-frame state: SceneBuilding from its template (with the
 // eye position of the frame), Readback to zero and the split list to ~0
@@ -53,6 +80,26 @@ __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building*
 using namespace tc;
 
 extern "C" {
+
+// objectToWorld (the instance's column-major world matrix as 3x4 row-major)
+// and its inverse, per instance
+tc_status tc_instance_xforms_host(const tc_render_instance* instances, uint32_t n, float* out) {
+  if (!instances || !out) return TC_ERROR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; i++) {
+    const float* m = instances[i].worldMatrix;  // m[col * 4 + row]
+    double rm[16];
+    for (int r = 0; r < 4; r++)
+      for (int c = 0; c < 4; c++) rm[r * 4 + c] = m[c * 4 + r];
+    double inv[16];
+    if (!invert4(rm, inv)) return TC_ERROR_INVALID_ARGUMENT;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 4; c++) {
+        out[24 * i + r * 4 + c] = (float)rm[r * 4 + c];
+        out[24 * i + 12 + r * 4 + c] = (float)inv[r * 4 + c];
+      }
+  }
+  return TC_OK;
+}
 
 tc_status tc_table_upload(tc_tessellation_table** tableDev);
 tc_status tc_table_free(tc_tessellation_table* tableDev);
@@ -83,7 +130,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs;
+  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
       {&oRb, sizeof(tc_readback)},
@@ -105,6 +152,8 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&B.blasClusterAddresses, 8ull * c.maxGeneratedClusters},
       {&B.genClusterData, (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull},
       {&oWs, wsBytes},
+      {&oXf, 24ull * 4 * numInstances},
+      {&oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances)},
   };
   uint64_t o = 0;
   for (auto& s : slots) {
@@ -126,6 +175,11 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   r->buildTemplate = reinterpret_cast<tc_scene_building*>(mem + oTmpl);
   r->workspace = mem + oWs;
   r->workspaceBytes = wsBytes;
+  r->xforms = reinterpret_cast<float*>(mem + oXf);
+  tc_accel_bind(mem + oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances), c.maxGeneratedClusters, numInstances,
+                &r->accel);
+  std::vector<float> xf(24ull * numInstances);
+  tc_instance_xforms_host(instancesHost, numInstances, xf.data());
   for (uint64_t* f : {&B.instanceStates, &B.visibleClusters, &B.splitTriangles, &B.partTriangles, &B.genVertices,
                       &B.tempInstanceIDs, &B.tempInstantiations, &B.tempClusterAddresses, &B.tempClusterSizes,
                       &B.blasBuildInfos, &B.blasBuildSizes, &B.blasClusterAddresses, &B.genClusterData})
@@ -137,6 +191,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       (numTextures &&
        hipMemcpy(r->textures, texturesHost, sizeof(tc_displacement_texture) * numTextures, hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(r->buildTemplate, &B, sizeof(B), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(r->xforms, xf.data(), xf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(r->arena, 0, oInst) != hipSuccess ||
       hipMemset(reinterpret_cast<void*>(B.blasBuildSizes), 0, 4ull * numInstances) != hipSuccess)
     s = TC_ERROR_LAUNCH;
@@ -181,6 +236,43 @@ tc_status tc_renderer_get_bindings(tc_renderer* rr, tc_bindings* out) {
   out->workspace = r->workspace;
   out->workspaceBytes = r->workspaceBytes;
   return TC_OK;
+}
+
+tc_status tc_renderer_get_accel(tc_renderer* rr, tc_accel* out, const float** instanceXforms) {
+  if (!rr || !out) return TC_ERROR_INVALID_ARGUMENT;
+  *out = R(rr)->accel;
+  if (instanceXforms) *instanceXforms = R(rr)->xforms;
+  return TC_OK;
+}
+
+// CLAS build, BLAS insertion, BLAS + TLAS build
+// (renderer_raytrace_clusters_tess.cpp:592-745).
+tc_status tc_renderer_build_accel(tc_renderer* rr, tc_stream stream) {
+  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  tc_bindings b;
+  tc_renderer_get_bindings(rr, &b);
+  tc_status s;
+  if ((s = tc_clas_instantiate(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_blas_setup_insertion(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_blas_clusters_insert(&b, r->cfg, 0, stream)) != TC_OK) return s;
+  return tc_accel_build(&b, r->cfg, &r->accel, r->xforms, stream);
+}
+
+tc_status tc_renderer_trace(tc_renderer* rr, const tc_trace_params* params, tc_stream stream) {
+  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  tc_bindings b;
+  tc_renderer_get_bindings(rr, &b);
+  return tc_trace_rays(&b, R(rr)->cfg, &R(rr)->accel, params, stream);
+}
+
+// One whole frame: tessellation, cluster acceleration structures, rays.
+tc_status tc_renderer_frame(tc_renderer* rr, const tc_frame_constants* frameHost, const tc_trace_params* params,
+                            tc_stream stream) {
+  tc_status s;
+  if ((s = tc_renderer_tessellate(rr, frameHost, stream)) != TC_OK) return s;
+  if ((s = tc_renderer_build_accel(rr, stream)) != TC_OK) return s;
+  return params ? tc_renderer_trace(rr, params, stream) : TC_OK;
 }
 
 tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {

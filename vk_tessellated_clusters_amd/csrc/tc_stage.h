@@ -1,0 +1,67 @@
+// tc_stage.h -- helpers shared by the stage kernels.
+#pragma once
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+enum Stage : uint32_t { kStageClassify = 0, kStageSplit = 1, kStageInstantiate = 2 };
+
+constexpr uint32_t kClassifyWaves = 4;   // waves per classify workgroup
+constexpr uint32_t kClassifyTile = 64;   // visible clusters per classify tile
+constexpr uint32_t kInstWaves = 4;       // waves per instantiate workgroup
+constexpr uint32_t kInstTile = kInstWaves * 64;  // part triangles per instantiate tile
+
+__device__ __forceinline__ ViewParams view_params(const KArgs& a) {
+  ViewParams vp;
+  vp.eye = mk(a.build->viewPos[0], a.build->viewPos[1], a.build->viewPos[2]);
+  vp.nearPlane = a.view->nearPlane;
+  vp.vpy = a.view->viewportf[1];
+  vp.rate = a.view->tessRate;
+  return vp;
+}
+
+__device__ __forceinline__ bool has_disp(const KArgs& a, int32_t idx) {
+  return (a.cfg.flags & TC_FLAG_DISPLACEMENT) && idx >= 0 && (uint32_t)idx < a.numTextures;
+}
+
+// cluster_classify.comp.glsl:477-479
+__device__ __forceinline__ float displaced_height(const KArgs& a, const tc_displacement_texture& tex, float scale,
+                                                  float offset, v2 uv) {
+  float h = sample_height(tex, uv.x, uv.y);
+  return (((h * scale) * a.view->displacementScale) + offset) + a.view->displacementOffset;
+}
+
+__device__ __forceinline__ uint64_t max_data_bytes(const tc_config& c) {
+  return (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull;
+}
+
+// last index j of the sorted s_start[0..n) with s_start[j] <= t (n a power of two)
+template <uint32_t N>
+__device__ __forceinline__ uint32_t find_owner_n(const uint32_t* s_start, uint32_t t) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = N / 2; step > 0; step >>= 1)
+    if (s_start[lo + step] <= t) lo += step;
+  return lo;
+}
+__device__ __forceinline__ uint32_t find_owner(const uint32_t* s_start, uint32_t t) { return find_owner_n<64>(s_start, t); }
+
+// Adds `count` of every active lane to infos[instance].clusterReferencesCount
+// with one atomic per distinct instance in the wave (neighbouring lanes
+// usually share the instance).
+__device__ __forceinline__ void add_blas_reference_counts(tc_blas_build_info* infos, uint32_t key, uint32_t count,
+                                                          bool active) {
+  uint64_t pending = ballot(active && count != 0);
+  while (pending) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t lkey = shfl(key, leader);
+    const bool same = ((pending >> lane_id()) & 1ull) && key == lkey;
+    const uint64_t sameMask = ballot(same);
+    const uint32_t sum = wave_sum(same ? count : 0u);
+    if (lane_id() == leader) atomicAdd(&infos[lkey].clusterReferencesCount, sum);
+    pending &= ~sameMask;
+  }
+}
+
+}  // namespace tc

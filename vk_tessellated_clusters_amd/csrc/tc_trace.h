@@ -1,0 +1,21 @@
+// tc_trace.h -- traversal kernel arguments.
+#pragma once
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+struct TraceArgs {
+  const tc_frame_constants* view;
+  const tc_tessellation_table* tessTable;
+  const tc_scene_building* build;
+  const tc_tlas_instance* tlasInstances;
+  const tc_bvh_node* tlasNodes;
+  const uint32_t* tlasRoot;
+  const tc_ray* rays;
+  uint32_t width, height, numRays, flags;
+  tc_hit* hits;
+  tc_hit_decoded* decoded;
+};
+
+}  // namespace tc

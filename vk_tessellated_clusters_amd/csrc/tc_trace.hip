@@ -1,0 +1,377 @@
+// tc_trace.hip -- primary / shadow ray traversal and the closest-hit decode.
+//
+// Stands for vkCmdTraceRaysKHR with render_raytrace.rgen.glsl:52-86 (rays),
+// the hardware traversal of the cluster BVH, render_raytrace_clusters.rchit.
+// glsl:128-236, 338-339 (decode, depth) and render_shading.glsl:263-274
+// (shadow rays).  One ray per lane; a per-lane LDS stack holds TLAS and BLAS
+// entries (bit 30 marks BLAS entries; BLAS entries above a TLAS entry belong
+// to the instance entered last).  Triangles use the watertight test of Woop,
+// Benthin and Wald (JCGT 2013) with an FP64 recomputation of zero edge
+// functions; closest-hit ties are broken by (instance, cluster, primitive)
+// so the result does not depend on traversal order.
+#include <algorithm>
+
+#include "tc_trace.h"
+
+namespace tc {
+
+constexpr uint32_t kTraceThreads = 128;
+constexpr uint32_t kStackDepth = 64;
+constexpr uint32_t kBlasEntry = 1u << 30;
+
+struct WoopRay {
+  float o[3], d[3], inv[3];
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ void setup_ray(WoopRay& r, const float o[3], const float d[3]) {
+  for (int k = 0; k < 3; k++) r.o[k] = o[k], r.d[k] = d[k], r.inv[k] = 1.0f / d[k];
+  const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  if (d[r.kz] < 0.0f) {
+    const int t = r.kx;
+    r.kx = r.ky;
+    r.ky = t;
+  }
+  r.Sx = d[r.kx] / d[r.kz];
+  r.Sy = d[r.ky] / d[r.kz];
+  r.Sz = 1.0f / d[r.kz];
+}
+
+// conservative slab test; returns entry distance or +inf
+__device__ __forceinline__ float box_hit(const WoopRay& r, const float lo[3], const float hi[3], float tmin, float tmax) {
+  float tn = tmin, tf = tmax;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float t0 = (lo[k] - r.o[k]) * r.inv[k];
+    const float t1 = (hi[k] - r.o[k]) * r.inv[k];
+    tn = fmaxf(tn, fminf(t0, t1));
+    tf = fminf(tf, fmaxf(t0, t1));
+  }
+  return tn <= tf * 1.0000005f ? tn : INFINITY;
+}
+
+struct TriHit {
+  float t, u, v;
+};
+
+// Woop et al. 2013 with barycentrics (u: vertex 1, v: vertex 2)
+__device__ __forceinline__ bool tri_hit(const WoopRay& r, const float* p0, const float* p1, const float* p2, bool cullBack,
+                                        float tmin, float tmax, TriHit& h) {
+  const float A[3] = {p0[0] - r.o[0], p0[1] - r.o[1], p0[2] - r.o[2]};
+  const float B[3] = {p1[0] - r.o[0], p1[1] - r.o[1], p1[2] - r.o[2]};
+  const float Cc[3] = {p2[0] - r.o[0], p2[1] - r.o[1], p2[2] - r.o[2]};
+  const float Ax = A[r.kx] - r.Sx * A[r.kz], Ay = A[r.ky] - r.Sy * A[r.kz];
+  const float Bx = B[r.kx] - r.Sx * B[r.kz], By = B[r.ky] - r.Sy * B[r.kz];
+  const float Cx = Cc[r.kx] - r.Sx * Cc[r.kz], Cy = Cc[r.ky] - r.Sy * Cc[r.kz];
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = (U + V) + W;
+  if (det == 0.0f) return false;
+  if (cullBack && det < 0.0f) return false;
+  const float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * Cc[r.kz];
+  const float T = (U * Az + V * Bz) + W * Cz;
+  const float t = T / det;
+  if (!(t >= tmin && t <= tmax)) return false;
+  h.t = t;
+  h.u = V / det;
+  h.v = W / det;
+  return true;
+}
+
+struct HitState {
+  tc_hit h;
+  const char* clas;  // CLAS block of the best hit
+  uint32_t word;     // its triangle word
+};
+
+__device__ __forceinline__ bool key_less(float t, uint32_t i, uint32_t c, uint32_t p, const tc_hit& best) {
+  if (t != best.t) return t < best.t;
+  if (i != best.instanceID) return i < best.instanceID;
+  if (c != best.clusterID) return c < best.clusterID;
+  return p < best.primitiveID;
+}
+
+__device__ __forceinline__ void xform34(const float* m, const float v[3], float w, float out[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) out[r] = ((m[4 * r] * v[0] + m[4 * r + 1] * v[1]) + m[4 * r + 2] * v[2]) + m[4 * r + 3] * w;
+}
+
+__device__ __forceinline__ const float* clas_vertices(const char* clas, uint32_t nt, uint32_t ng) {
+  return reinterpret_cast<const float*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u));
+}
+
+// Intersects one CLAS block; updates `best` (closest) or returns true on
+// the first hit (any-hit).
+__device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uint32_t instanceID, bool cullBack, bool anyHit,
+                                         float tmin, HitState& best) {
+  const tc_clas_header hdr = *reinterpret_cast<const tc_clas_header*>(clas);
+  const uint32_t nt = hdr.counts & 0xFF, ng = (hdr.counts >> 16) & 0xFF;
+  const tc_aabb* groups = reinterpret_cast<const tc_aabb*>(clas + sizeof(tc_clas_header));
+  const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
+  const float* verts = clas_vertices(clas, nt, ng);
+  for (uint32_t g = 0; g < ng; g++) {
+    const tc_aabb gb = groups[g];
+    if (box_hit(r, gb.lo, gb.hi, tmin, best.h.t) == INFINITY) continue;
+    const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
+    for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
+      const uint32_t w = tris[t];
+      TriHit th;
+      if (!tri_hit(r, verts + 3 * (w & 0xFF), verts + 3 * ((w >> 8) & 0xFF), verts + 3 * ((w >> 16) & 0xFF), cullBack, tmin,
+                   best.h.t, th))
+        continue;
+      const uint32_t prim = w >> 24;
+      if (anyHit) {
+        best.h.t = th.t;
+        return true;
+      }
+      if (key_less(th.t, instanceID, hdr.clusterID, prim, best.h)) {
+        best.h.t = th.t;
+        best.h.u = th.u;
+        best.h.v = th.v;
+        best.h.instanceID = instanceID;
+        best.h.clusterID = hdr.clusterID;
+        best.h.primitiveID = prim;
+        best.clas = clas;
+        best.word = w;
+      }
+    }
+  }
+  return false;
+}
+
+// Full two-level traversal of one ray.  `stack` is this lane's LDS stack
+// (stride kTraceThreads).  Returns true if anything was hit.
+__device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3], float tmin, float tmax, bool cullBack,
+                          bool anyHit, uint32_t* stack, HitState& best) {
+  best.h.t = tmax;
+  best.h.u = best.h.v = 0.0f;
+  best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
+  best.clas = nullptr;
+  best.word = 0;
+  WoopRay world, ray;
+  setup_ray(world, o, d);
+  ray = world;
+  stack[0] = *a.tlasRoot;  // TLAS entry
+  uint32_t sp = 1;
+  uint32_t inst = 0xFFFFFFFFu;
+  const tc_bvh_node* blasNodes = nullptr;
+  const uint64_t* blasLeaves = nullptr;
+  while (sp > 0) {
+    const uint32_t e = stack[--sp * kTraceThreads];
+    if (!(e & kBlasEntry)) {
+      if (inst != 0xFFFFFFFFu) {
+        ray = world;
+        inst = 0xFFFFFFFFu;
+      }
+      if (e & TC_BVH_LEAF) {
+        const uint32_t i = e & 0x3FFFFFFFu;
+        const tc_tlas_instance ti = a.tlasInstances[i];
+        if (ti.numClusters == 0) continue;
+        float ol[3], dl[3];
+        xform34(ti.worldToObject, o, 1.0f, ol);
+        xform34(ti.worldToObject, d, 0.0f, dl);
+        setup_ray(ray, ol, dl);
+        inst = i;
+        blasNodes = reinterpret_cast<const tc_bvh_node*>(ti.blasNodes);
+        blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+        if (sp < kStackDepth) stack[sp++ * kTraceThreads] = kBlasEntry | ti.blasRoot;
+        continue;
+      }
+      const tc_bvh_node n = a.tlasNodes[e];
+      const float t0 = box_hit(ray, n.lo0, n.hi0, tmin, best.h.t);
+      const float t1 = box_hit(ray, n.lo1, n.hi1, tmin, best.h.t);
+      const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY;
+      if (h0 && h1) {
+        const bool near0 = t0 <= t1;
+        if (sp + 1 < kStackDepth) {
+          stack[sp++ * kTraceThreads] = near0 ? n.child1 : n.child0;
+          stack[sp++ * kTraceThreads] = near0 ? n.child0 : n.child1;
+        }
+      } else if (h0 || h1) {
+        if (sp < kStackDepth) stack[sp++ * kTraceThreads] = h0 ? n.child0 : n.child1;
+      }
+      continue;
+    }
+    const uint32_t be = e & ~kBlasEntry;
+    if (be & TC_BVH_LEAF) {
+      const char* clas = reinterpret_cast<const char*>(blasLeaves[be & ~TC_BVH_LEAF]);
+      if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
+      continue;
+    }
+    const tc_bvh_node n = blasNodes[be];
+    const float t0 = box_hit(ray, n.lo0, n.hi0, tmin, best.h.t);
+    const float t1 = box_hit(ray, n.lo1, n.hi1, tmin, best.h.t);
+    const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY;
+    if (h0 && h1) {
+      const bool near0 = t0 <= t1;
+      if (sp + 1 < kStackDepth) {
+        stack[sp++ * kTraceThreads] = kBlasEntry | (near0 ? n.child1 : n.child0);
+        stack[sp++ * kTraceThreads] = kBlasEntry | (near0 ? n.child0 : n.child1);
+      }
+    } else if (h0 || h1) {
+      if (sp < kStackDepth) stack[sp++ * kTraceThreads] = kBlasEntry | (h0 ? n.child0 : n.child1);
+    }
+  }
+  return best.h.instanceID != 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void mul44(const float* m, const float v[4], float out[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) out[r] = ((m[r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
+}
+
+__device__ __forceinline__ void normalize4(float v[4]) {
+  const float l = sqrtf(((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3]);
+  const float s = 1.0f / l;
+  for (int k = 0; k < 4; k++) v[k] = v[k] * s;
+}
+
+// render_raytrace.rgen.glsl:52-86
+__device__ __forceinline__ void primary_ray(const tc_frame_constants* view, uint32_t x, uint32_t y, uint32_t W, uint32_t H,
+                                            float o[3], float d[3]) {
+  const float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;
+  const float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
+  const float org[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+  float origin[4];
+  mul44(view->viewMatrixI, org, origin);
+  const float ndc[4] = {dx, dy, 1.0f, 1.0f};
+  float target[4];
+  mul44(view->projMatrixI, ndc, target);
+  normalize4(target);
+  const float tv[4] = {target[0], target[1], target[2], 0.0f};
+  float dir[4];
+  mul44(view->viewMatrixI, tv, dir);
+  normalize4(dir);
+  for (int k = 0; k < 3; k++) o[k] = origin[k], d[k] = dir[k];
+}
+
+// render_raytrace_clusters.rchit.glsl:128-236 and 338-339: base cluster,
+// triangle and barycentrics, world position and depth of a hit
+__device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decoded& out) {
+  const tc_hit& h = hs.h;
+  out.instanceID = h.instanceID;
+  out.shadowed = 0;
+  if (h.instanceID == 0xFFFFFFFFu) {
+    out.clusterID = out.triangleID = out.subTriangleID = 0xFFFFFFFFu;
+    out.baryBase[0] = out.baryBase[1] = out.baryBase[2] = 0.0f;
+    out.depth = 1.0f;
+    out.wPos[0] = out.wPos[1] = out.wPos[2] = 0.0f;
+    return;
+  }
+  const uint32_t mode = h.clusterID >> 30;
+  uint32_t clusterID = h.clusterID & 0x3FFFFFFFu;
+  uint32_t triangleID = h.primitiveID;
+  const v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v);
+  v3 base = bw;
+  if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
+    const tc_tess_triangle_info ti = ptr<const tc_tess_triangle_info>(a.build->partTriangles)[clusterID];
+    triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
+    const uint32_t cfg = ti.subTriangle.triangleID_config >> 16;
+    clusterID = ti.cluster.clusterID;
+    const TableView T = load_table(a.tessTable);
+    const tc_tess_table_entry e = cfg_entry(T, cfg);
+    uint32_t idx[3];
+    cfg_tri_vertices(T, cfg, e.firstTriangle, h.primitiveID, idx);
+    const v3 nb = interp(cfg_vertex_bary(T, cfg, e.firstVertex, idx[0]), cfg_vertex_bary(T, cfg, e.firstVertex, idx[1]),
+                         cfg_vertex_bary(T, cfg, e.firstVertex, idx[2]), bw);
+    base = interp(decode_bary(ti.subTriangle.vtxEncoded[0]), decode_bary(ti.subTriangle.vtxEncoded[1]),
+                  decode_bary(ti.subTriangle.vtxEncoded[2]), nb);
+  }
+  out.clusterID = clusterID;
+  out.triangleID = triangleID;
+  out.subTriangleID = h.primitiveID;
+  out.baryBase[0] = base.x, out.baryBase[1] = base.y, out.baryBase[2] = base.z;
+  // :248-249 object-space hit point from the hit triangle, then world
+  const tc_clas_header hdr = *reinterpret_cast<const tc_clas_header*>(hs.clas);
+  const float* verts = clas_vertices(hs.clas, hdr.counts & 0xFF, (hdr.counts >> 16) & 0xFF);
+  const float* p0 = verts + 3 * (hs.word & 0xFF);
+  const float* p1 = verts + 3 * ((hs.word >> 8) & 0xFF);
+  const float* p2 = verts + 3 * ((hs.word >> 16) & 0xFF);
+  const v3 op = interp(mk(p0[0], p0[1], p0[2]), mk(p1[0], p1[1], p1[2]), mk(p2[0], p2[1], p2[2]), bw);
+  const float opv[3] = {op.x, op.y, op.z};
+  float wp[3];
+  xform34(a.tlasInstances[h.instanceID].objectToWorld, opv, 1.0f, wp);
+  out.wPos[0] = wp[0], out.wPos[1] = wp[1], out.wPos[2] = wp[2];
+  const float w4[4] = {wp[0], wp[1], wp[2], 1.0f};
+  float pp[4];
+  mul44(a.view->viewProjMatrix, w4, pp);
+  out.depth = pp[2] / pp[3];
+}
+
+__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
+  __shared__ uint32_t s_stack[kStackDepth * kTraceThreads];
+  uint32_t* stack = s_stack + threadIdx.x;
+  const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
+  const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
+  for (uint32_t r = blockIdx.x * kTraceThreads + threadIdx.x; r < a.numRays; r += gridDim.x * kTraceThreads) {
+    float o[3], d[3], tmin, tmax;
+    if (a.rays) {
+      const tc_ray ray = a.rays[r];
+      for (int k = 0; k < 3; k++) o[k] = ray.origin[k], d[k] = ray.direction[k];
+      tmin = ray.tMin;
+      tmax = ray.tMax;
+    } else {
+      primary_ray(a.view, r % a.width, r / a.width, a.width, a.height, o, d);
+      tmin = a.view->nearPlane;
+      tmax = a.view->farPlane;
+    }
+    HitState hs;
+    trace_ray(a, o, d, tmin, tmax, cull, false, stack, hs);
+    tc_hit hit = hs.h;
+    if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
+    if (a.hits) a.hits[r] = hit;
+    if (!a.decoded) continue;
+    tc_hit_decoded dec;
+    decode_hit(a, hs, dec);
+    if (shadows && hit.instanceID != 0xFFFFFFFFu) {
+      // render_shading.glsl:263-274: opaque, terminate on first hit
+      const float* sun = a.view->skyParams.sunDirection;
+      const float sd[3] = {sun[0], sun[1], sun[2]};
+      HitState sh;
+      dec.shadowed = trace_ray(a, dec.wPos, sd, 0.001f, 10000000.0f, false, true, stack, sh) ? 1u : 0u;
+    }
+    a.decoded[r] = dec;
+  }
+}
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* p,
+                                   tc_stream stream) {
+  KArgs ka;
+  tc_status s = make_args_public(b, config, ka);
+  if (s != TC_OK) return s;
+  if (!accel || !p || !accel->tlasInstances || (!p->hits && !p->decoded)) return TC_ERROR_INVALID_ARGUMENT;
+  TraceArgs a;
+  a.view = b->view;
+  a.tessTable = b->tessTable;
+  a.build = b->build;
+  a.tlasInstances = accel->tlasInstances;
+  a.tlasNodes = accel->tlasNodes;
+  a.tlasRoot = accel->tlasRoot;
+  a.rays = p->rays;
+  a.width = p->width;
+  a.height = p->height;
+  a.numRays = p->rays ? p->numRays : p->width * p->height;
+  a.hits = p->hits;
+  a.decoded = p->decoded;
+  a.flags = p->flags;
+  if (!p->rays && (!p->width || !p->height)) return TC_ERROR_INVALID_ARGUMENT;
+  if (a.numRays == 0) return TC_OK;
+  const uint32_t grid = std::min<uint32_t>((a.numRays + kTraceThreads - 1) / kTraceThreads, (uint32_t)num_cu_public() * 16);
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kTraceThreads), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}

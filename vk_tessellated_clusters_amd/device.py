@@ -152,6 +152,18 @@ class Renderer:
         abi.check(self.L.tc_renderer_tessellate(self.handle, C.byref(fc), self.stream_ptr(stream)),
                   "tc_renderer_tessellate")
 
+    def build_accel(self, stream=None):
+        abi.check(self.L.tc_renderer_build_accel(self.handle, self.stream_ptr(stream)), "tc_renderer_build_accel")
+
+    def accel(self):
+        a = abi.Accel()
+        xf = C.c_void_p()
+        abi.check(self.L.tc_renderer_get_accel(self.handle, C.byref(a), C.byref(xf)))
+        return a, xf.value
+
+    def trace(self, params: abi.TraceParams, stream=None):
+        abi.check(self.L.tc_renderer_trace(self.handle, C.byref(params), self.stream_ptr(stream)), "tc_renderer_trace")
+
     def clas_and_blas(self, stream=None):
         s = self.stream_ptr(stream)
         b = C.byref(self.bindings)
