@@ -44,7 +44,48 @@ def make_scene(name):
         return scene.scene_grid_instances(), "64x grid 256x256 quads (8.4M tris), 2048^2 R16 fBm, 1920x1080 @ 4px"
     if name == "icosphere":
         return scene.scene_icosphere(), "icosphere 1280 tris, 4-oct fBm amp 0.05, 1280x720 @ 8px"
+    if name == "batched500":
+        return scene.scene_batched(), ("500 meshes log-uniform 200..200000 tris (seed 2, 14.0M tris), grids + "
+                                       "icospheres, own 1024^2 R16 fBm each, 1920x1080 @ 4px")
     raise SystemExit("unknown workload " + name)
+
+
+def per_mesh_ms(torch, ds, cfg, fc, stream, reps):
+    """Frame time of the per-mesh form: one renderer sized for one mesh,
+    re-pointed at each mesh in turn, the whole tessellation + CLAS + BLAS
+    chain launched once per mesh (eager launches, no graph)."""
+    from vk_tessellated_clusters_amd import abi
+    from vk_tessellated_clusters_amd.device import Renderer
+    L = abi.lib()
+    n = len(ds.instances)
+    one = (abi.RenderInstance * 1)(ds.instances[0])
+
+    class One:
+        instances, textures, num_textures = one, ds.textures, ds.num_textures
+
+    r1 = Renderer(One, cfg)
+    xf = np.zeros(24 * n, np.float32)
+    abi.check(L.tc_instance_xforms_host(C.addressof(ds.instances), n, xf.ctypes.data))
+    inst_dev = torch.from_numpy(np.frombuffer(bytes(ds.instances), np.uint8).copy()).cuda()
+    xf_dev = torch.from_numpy(xf.view(np.uint8)).cuda()
+    s = C.c_void_p(stream.cuda_stream)
+
+    def frame():
+        for i in range(n):
+            abi.check(L.tc_renderer_rebind_instances(r1.handle, C.c_void_p(inst_dev.data_ptr() + 192 * i),
+                                                     C.c_void_p(xf_dev.data_ptr() + 96 * i), 1))
+            abi.check(L.tc_renderer_frame(r1.handle, C.byref(fc), None, s))
+
+    frame()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        frame()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    r1.close()
+    return e0.elapsed_time(e1) / reps
 
 
 class StageTimer:
@@ -177,6 +218,8 @@ def main():
     ap.add_argument("--cpu-sample-instances", type=int, default=8)
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", dest="graph", action="store_false")
+    ap.add_argument("--per-mesh-reps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
@@ -209,6 +252,15 @@ def main():
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
+    if args.graph:
+        abi.check(r.L.tc_renderer_graph_capture(r.handle, C.byref(fc), C.byref(tp), C.c_void_p(stream.cuda_stream)),
+                  "graph capture")
+
+        def frame():  # noqa: F811
+            abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)), "graph launch")
+
+        frame()
+        torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(args.steps):
@@ -264,6 +316,22 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": alg[dom],
                      "launch_ms": dom_ms},
     }
+    if args.workload == "batched500":
+        # tessellation + CLAS + BLAS + TLAS of all meshes: one graph replay vs
+        # per-mesh launches (rays excluded from both)
+        abi.check(r.L.tc_renderer_graph_capture(r.handle, C.byref(fc), None, C.c_void_p(stream.cuda_stream)))
+        for _ in range(2):
+            abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)))
+        torch.cuda.synchronize()
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b0.record(stream)
+        for _ in range(args.steps):
+            abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)))
+        b1.record(stream)
+        torch.cuda.synchronize()
+        batched = b0.elapsed_time(b1) / args.steps
+        pm = per_mesh_ms(torch, ds, cfg, fc, stream, args.per_mesh_reps)
+        out["batched_vs_per_mesh"] = {"batched_graph_ms": batched, "per_mesh_ms": pm, "speedup": pm / batched}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_sample_instances, args.cpu_budget_s)
     print(json.dumps(out))

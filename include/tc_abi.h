@@ -389,7 +389,9 @@ typedef struct tc_clas_header {
   float lo[3];
   uint32_t clusterID; /* mode << 30 | id, the value of gl_ClusterIDNV */
   float hi[3];
-  uint32_t counts; /* numTriangles | numVertices << 8 | numGroups << 16 */
+  uint32_t counts;     /* numTriangles | numVertices << 8 | numGroups << 16 */
+  uint32_t vertexBase; /* genVertices index of the source of vertex 0 */
+  uint32_t _pad[3];
 } tc_clas_header;
 
 /* Two-child BVH node used by the per-instance BLAS and the TLAS.  A child
@@ -615,9 +617,18 @@ typedef struct tc_trace_params {
   uint32_t flags; /* TC_TRACE_* */
   tc_hit* hits;            /* device, per ray (t < 0: miss), may be NULL */
   tc_hit_decoded* decoded; /* device, per ray, may be NULL */
+  uint32_t* hitVertices;   /* device, 4 per ray: genVertices indices of the hit
+                              triangle's vertices (~0 on a miss), may be NULL */
 } tc_trace_params;
 
 tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* params,
+                        tc_stream stream);
+
+/* Synthetic workload input: a size x size tileable value-noise fBm height
+ * map (octaves from baseFreq, integer lattice hash) written to device
+ * memory as R16_UNORM or R32_FLOAT.  No reference counterpart: the
+ * reference loads height maps from glTF assets. */
+tc_status tc_fbm_height(void* texelsDev, uint32_t size, uint32_t seed, uint32_t octaves, uint32_t baseFreq, uint32_t format,
                         tc_stream stream);
 
 /* ---- frame driver (renderer_raytrace_clusters_tess.cpp:166-310, 410-780) ---- */
@@ -650,6 +661,18 @@ tc_status tc_renderer_trace(tc_renderer* renderer, const tc_trace_params* params
 /* tessellate + build_accel + trace (params may be NULL). */
 tc_status tc_renderer_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, const tc_trace_params* params,
                             tc_stream stream);
+/* Points the renderer at n instance records and 24-float transforms already
+ * in device memory (n = the count given at creation); NULL restores its own
+ * copies.  Lets one small renderer process many meshes one after another. */
+tc_status tc_renderer_rebind_instances(tc_renderer* renderer, const tc_render_instance* devInstances, const float* devXforms,
+                                       uint32_t n);
+
+/* Captures one whole frame into a HIP graph on `stream` (not the null
+ * stream); tc_renderer_graph_launch replays it with new frame constants. */
+tc_status tc_renderer_graph_capture(tc_renderer* renderer, const tc_frame_constants* frameHost,
+                                    const tc_trace_params* params, tc_stream stream);
+tc_status tc_renderer_graph_launch(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
+
 /* objectToWorld and worldToObject (3x4 row-major, 24 floats) per instance,
  * the transforms the renderer uploads; worldToObject is inverted in FP64. */
 tc_status tc_instance_xforms_host(const tc_render_instance* instances, uint32_t n, float* out);
@@ -675,7 +698,7 @@ static_assert(offsetof(tc_scene_building, positionTruncateBitCount) == 208, "Sce
 static_assert(offsetof(tc_scene_building, numBlasReservedSizes) == 360, "SceneBuilding layout");
 static_assert(sizeof(tc_scene_building) == 368, "SceneBuilding");
 static_assert(offsetof(tc_readback, numGenDatas) == 48, "Readback layout");
-static_assert(sizeof(tc_clas_header) == 32, "clas header");
+static_assert(sizeof(tc_clas_header) == 48, "clas header");
 static_assert(sizeof(tc_bvh_node) == 64, "bvh node");
 static_assert(sizeof(tc_clas_template) == 32, "clas template");
 #endif

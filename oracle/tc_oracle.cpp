@@ -931,6 +931,9 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
     for (int a = 0; a < 3; a++) h.lo[a] = lo[a], h.hi[a] = hi[a];
     h.clusterID = T.clusterID + info.clusterIdOffset;
     h.counts = nt | (nv << 8) | (ng << 16);
+    h.vertexBase = info.vertexBufferAddress >= b->genVertices ? (uint32_t)((info.vertexBufferAddress - b->genVertices) / 12u)
+                                                               : 0xFFFFFFFFu;
+    h._pad[0] = h._pad[1] = h._pad[2] = 0;
     memcpy(dst, &h, sizeof(h));
     P<uint32_t>(b->tempClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
   }

@@ -209,8 +209,10 @@ class OracleFrame:
                                    "clusterLocalTriangles": keep(geo.local_triangles).ctypes.data,
                                    "clusterBboxes": keep(geo.bboxes).ctypes.data, "clusterTemplateAdresses": ra.ctypes.data,
                                    "clusterTemplateInstantiatonSizes": s.ctypes.data})
+        from vk_tessellated_clusters_amd.scene import FbmTexture
         self.textures = (abi.DisplacementTexture * max(1, len(host_scene.textures)))()
-        for i, (arr, fmt) in enumerate(host_scene.textures):
+        for i, tex in enumerate(host_scene.textures):
+            arr, fmt = (tex.host(), tex.format) if isinstance(tex, FbmTexture) else tex
             a = keep(arr)
             self.textures[i].texels = a.ctypes.data
             self.textures[i].height, self.textures[i].width = a.shape

@@ -68,7 +68,7 @@ def test_clas_size_formula():
     for nt in (1, 4, 5, 64, 121, 128):
         for nv in (3, 45, 78, 81, 128):
             ng = (nt + 3) // 4
-            assert L.tc_clas_size(nt, nv) == 32 + r16(24 * ng) + r16(4 * nt) + r16(12 * nv)
+            assert L.tc_clas_size(nt, nv) == 48 + r16(24 * ng) + r16(4 * nt) + r16(12 * nv)
 
 
 def test_argument_errors_without_gpu_work():

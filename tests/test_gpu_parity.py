@@ -82,3 +82,21 @@ def test_repeat_frames_are_identical():
     finally:
         g.close()
     harness.compare(r2, r1, max_ulp=0)
+
+
+def test_device_fbm_matches_numpy():
+    """The device height-map generator reproduces the numpy generator texel
+    for texel (R32 and R16)."""
+    import numpy as np
+    import torch
+    L = abi.lib()
+    for size, seed, octaves, fmt in ((64, 1, 6, abi.TEXTURE_R32_FLOAT), (256, 7, 4, abi.TEXTURE_R16_UNORM),
+                                     (100, 3, 5, abi.TEXTURE_R32_FLOAT)):
+        ref = scene.fbm_height(size, seed, octaves)
+        if fmt == abi.TEXTURE_R16_UNORM:
+            ref = scene.to_r16(ref)
+        out = torch.zeros(size * size * ref.itemsize, dtype=torch.uint8, device="cuda")
+        abi.check(L.tc_fbm_height(out.data_ptr(), size, seed, octaves, 8, fmt, None))
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().view(ref.dtype).reshape(size, size)
+        assert np.array_equal(got, ref)

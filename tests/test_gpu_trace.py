@@ -22,7 +22,7 @@ def small_scene():
 def clas_boxes(g, addrs):
     out = {}
     for a in np.unique(addrs):
-        h = g.r.copy(int(a), 32).view(abi.CLAS_HEADER_DT)[0]
+        h = g.r.copy(int(a), 48).view(abi.CLAS_HEADER_DT)[0]
         out[int(a)] = (h["lo"].copy(), h["hi"].copy())
     return out
 

@@ -178,7 +178,7 @@ class Accel(C.Structure):
 
 class TraceParams(C.Structure):
     _fields_ = [("rays", C.c_void_p), ("numRays", u32), ("width", u32), ("height", u32), ("flags", u32),
-                ("hits", C.c_void_p), ("decoded", C.c_void_p)]
+                ("hits", C.c_void_p), ("decoded", C.c_void_p), ("hitVertices", C.c_void_p)]
 
 
 TRACE_CULL_BACK = 1
@@ -195,7 +195,8 @@ BVH_NODE_DT = np.dtype([("lo0", "<f4", (3,)), ("child0", "<u4"), ("hi0", "<f4", 
 TLAS_INSTANCE_DT = np.dtype([("objectToWorld", "<f4", (12,)), ("worldToObject", "<f4", (12,)), ("blasNodes", "<u8"),
                              ("blasLeaves", "<u8"), ("blasRoot", "<u4"), ("numClusters", "<u4"), ("flags", "<u4"),
                              ("pad", "<u4")])
-CLAS_HEADER_DT = np.dtype([("lo", "<f4", (3,)), ("clusterID", "<u4"), ("hi", "<f4", (3,)), ("counts", "<u4")])
+CLAS_HEADER_DT = np.dtype([("lo", "<f4", (3,)), ("clusterID", "<u4"), ("hi", "<f4", (3,)), ("counts", "<u4"),
+                           ("vertexBase", "<u4"), ("pad", "<u4", (3,))])
 BVH_LEAF = 0x80000000
 
 
@@ -243,7 +244,8 @@ EXPORTS = [
     "tc_renderer_tessellate", "tc_num_split_passes", "tc_memcpy", "tc_device_synchronize", "tc_clas_instantiate",
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
-    "tc_instance_xforms_host",
+    "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
+    "tc_renderer_rebind_instances",
 ]
 
 
@@ -294,6 +296,10 @@ def lib():
         "tc_renderer_trace": (i32, [P, C.POINTER(TraceParams), P]),
         "tc_renderer_frame": (i32, [P, C.POINTER(FrameConstants), C.POINTER(TraceParams), P]),
         "tc_instance_xforms_host": (i32, [P, u32, P]),
+        "tc_renderer_graph_capture": (i32, [P, C.POINTER(FrameConstants), C.POINTER(TraceParams), P]),
+        "tc_renderer_graph_launch": (i32, [P, C.POINTER(FrameConstants), P]),
+        "tc_fbm_height": (i32, [P, u32, u32, u32, u32, u32, P]),
+        "tc_renderer_rebind_instances": (i32, [P, P, P, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

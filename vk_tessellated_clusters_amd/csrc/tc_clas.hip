@@ -94,6 +94,9 @@ __global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
       h.hi[0] = hi[0], h.hi[1] = hi[1], h.hi[2] = hi[2];
       h.clusterID = T.clusterID + info.clusterIdOffset;
       h.counts = nt | (nv << 8) | (ng << 16);
+      h.vertexBase = info.vertexBufferAddress >= b->genVertices ? (uint32_t)((info.vertexBufferAddress - b->genVertices) / 12u)
+                                                                 : 0xFFFFFFFFu;
+      h._pad[0] = h._pad[1] = h._pad[2] = 0;
       *reinterpret_cast<tc_clas_header*>(dst) = h;
       sizes[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
     }

@@ -30,6 +30,11 @@ struct Renderer {
   uint64_t splitBytes = 0;
   float* xforms = nullptr;  // per instance objectToWorld | worldToObject (3x4 row-major)
   tc_accel accel{};
+  tc_render_instance* ownInstances = nullptr;
+  float* ownXforms = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  tc_trace_params graphParams{};
 };
 
 static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
@@ -176,6 +181,8 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   r->workspace = mem + oWs;
   r->workspaceBytes = wsBytes;
   r->xforms = reinterpret_cast<float*>(mem + oXf);
+  r->ownInstances = r->instances;
+  r->ownXforms = r->xforms;
   tc_accel_bind(mem + oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances), c.maxGeneratedClusters, numInstances,
                 &r->accel);
   std::vector<float> xf(24ull * numInstances);
@@ -215,6 +222,8 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
 tc_status tc_renderer_destroy(tc_renderer* rr) {
   if (!rr) return TC_ERROR_INVALID_ARGUMENT;
   Renderer* r = R(rr);
+  if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
   if (r->table) tc_table_free(r->table);
   if (r->pinnedView) (void)hipHostFree(r->pinnedView);
   if (r->arena) (void)hipFree(r->arena);
@@ -273,6 +282,59 @@ tc_status tc_renderer_frame(tc_renderer* rr, const tc_frame_constants* frameHost
   if ((s = tc_renderer_tessellate(rr, frameHost, stream)) != TC_OK) return s;
   if ((s = tc_renderer_build_accel(rr, stream)) != TC_OK) return s;
   return params ? tc_renderer_trace(rr, params, stream) : TC_OK;
+}
+
+// Points the renderer at `n` instance records and transforms already in
+// device memory (n equal to the count given at creation); NULL arguments
+// restore the renderer's own copies.
+tc_status tc_renderer_rebind_instances(tc_renderer* rr, const tc_render_instance* devInstances, const float* devXforms,
+                                       uint32_t n) {
+  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  if (!devInstances || !devXforms) {
+    r->instances = r->ownInstances;
+    r->xforms = r->ownXforms;
+    return TC_OK;
+  }
+  if (n != r->numInstances) return TC_ERROR_INVALID_ARGUMENT;
+  r->instances = const_cast<tc_render_instance*>(devInstances);
+  r->xforms = const_cast<float*>(devXforms);
+  return TC_OK;
+}
+
+// Captures one whole frame (tc_renderer_frame) into a graph on `stream`.
+// The frame constants are read at replay time from the renderer's pinned
+// buffer, which tc_renderer_graph_launch fills.
+tc_status tc_renderer_graph_capture(tc_renderer* rr, const tc_frame_constants* frameHost, const tc_trace_params* params,
+                                    tc_stream stream) {
+  if (!rr || !frameHost || !stream) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  hipStream_t st = (hipStream_t)stream;
+  if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec), r->graphExec = nullptr;
+  if (r->graph) (void)hipGraphDestroy(r->graph), r->graph = nullptr;
+  if (params) r->graphParams = *params;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return TC_ERROR_LAUNCH;
+  tc_status s = tc_renderer_frame(rr, frameHost, params ? &r->graphParams : nullptr, stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != TC_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return s != TC_OK ? s : TC_ERROR_LAUNCH;
+  }
+  r->graph = g;
+  if (hipGraphInstantiate(&r->graphExec, g, nullptr, nullptr, 0) != hipSuccess) return TC_ERROR_LAUNCH;
+  return TC_OK;
+}
+
+// Replays the captured frame with new frame constants.  The pinned buffer
+// is rewritten on the host, so a caller changing the camera between
+// replays must let the previous replay finish its upload first (the frame
+// constants copy is the graph's first node).
+tc_status tc_renderer_graph_launch(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
+  if (!rr || !R(rr)->graphExec) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  if (frameHost) memcpy(r->pinnedView, frameHost, sizeof(tc_frame_constants));
+  return hipGraphLaunch(r->graphExec, (hipStream_t)stream) == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 
 tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {

@@ -16,6 +16,7 @@ struct TraceArgs {
   uint32_t width, height, numRays, flags;
   tc_hit* hits;
   tc_hit_decoded* decoded;
+  uint32_t* hitVertices;
 };
 
 }  // namespace tc

@@ -331,6 +331,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     tc_hit hit = hs.h;
     if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
     if (a.hits) a.hits[r] = hit;
+    if (a.hitVertices) {
+      uint4 hv = make_uint4(~0u, ~0u, ~0u, ~0u);
+      if (hs.clas) {
+        const uint32_t base = reinterpret_cast<const tc_clas_header*>(hs.clas)->vertexBase;
+        hv = make_uint4(base + (hs.word & 0xFF), base + ((hs.word >> 8) & 0xFF), base + ((hs.word >> 16) & 0xFF), 0u);
+      }
+      reinterpret_cast<uint4*>(a.hitVertices)[r] = hv;
+    }
     if (!a.decoded) continue;
     tc_hit_decoded dec;
     decode_hit(a, hs, dec);
@@ -354,7 +362,7 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   KArgs ka;
   tc_status s = make_args_public(b, config, ka);
   if (s != TC_OK) return s;
-  if (!accel || !p || !accel->tlasInstances || (!p->hits && !p->decoded)) return TC_ERROR_INVALID_ARGUMENT;
+  if (!accel || !p || !accel->tlasInstances || (!p->hits && !p->decoded && !p->hitVertices)) return TC_ERROR_INVALID_ARGUMENT;
   TraceArgs a;
   a.view = b->view;
   a.tessTable = b->tessTable;
@@ -368,6 +376,7 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   a.numRays = p->rays ? p->numRays : p->width * p->height;
   a.hits = p->hits;
   a.decoded = p->decoded;
+  a.hitVertices = p->hitVertices;
   a.flags = p->flags;
   if (!p->rays && (!p->width || !p->height)) return TC_ERROR_INVALID_ARGUMENT;
   if (a.numRays == 0) return TC_OK;

@@ -101,12 +101,21 @@ class DeviceScene:
             a["clusterTemplateAdresses"] = up(np.uint64(raddr) + np.arange(geo.num_clusters, dtype=np.uint64) *
                                               np.uint64(32))
             self.geo_addrs.append(a)
+        from .scene import FbmTexture
         self.textures = (abi.DisplacementTexture * max(1, len(host.textures)))()
-        for i, (arr, fmt) in enumerate(host.textures):
+        for i, tex in enumerate(host.textures):
             t = self.textures[i]
-            t.texels = up(arr)
-            t.height, t.width = arr.shape
-            t.format = fmt
+            if isinstance(tex, FbmTexture):
+                buf = torch.empty(tex.nbytes, dtype=torch.uint8, device=dev)
+                self._keep.append(buf)
+                abi.check(L.tc_fbm_height(buf.data_ptr(), tex.size, tex.seed, tex.octaves, 8, tex.format, None),
+                          "tc_fbm_height")
+                t.texels, t.width, t.height, t.format = buf.data_ptr(), tex.size, tex.size, tex.format
+            else:
+                arr, fmt = tex
+                t.texels = up(arr)
+                t.height, t.width = arr.shape
+                t.format = fmt
         self.num_textures = len(host.textures)
         self.instances = (abi.RenderInstance * len(host.instances))()
         for i, inst in enumerate(host.instances):

@@ -161,7 +161,9 @@ def _hash2(x, y, seed):
 
 
 def fbm_height(size: int, seed: int, octaves: int, base_freq: int = 8) -> np.ndarray:
-    """Tileable value-noise fBm in [0, 1] on a size x size grid (float32)."""
+    """Tileable value-noise fBm in [0, 1] on a size x size grid (float32).
+    FP64 arithmetic in a fixed order; tc_fbm_height computes the same texels
+    on the device."""
     coords = (np.arange(size, dtype=np.float64) + 0.5) / size
     acc = np.zeros((size, size), np.float64)
     amp_sum = 0.0
@@ -169,10 +171,11 @@ def fbm_height(size: int, seed: int, octaves: int, base_freq: int = 8) -> np.nda
         f = base_freq << o
         amp = 0.5 ** o
         x = coords * f
-        xi = np.floor(x).astype(np.int64)
+        xi = np.floor(x)
         xf = x - xi
-        sx = xf * xf * (3 - 2 * xf)
-        x0, x1 = xi % f, (xi + 1) % f
+        sx = xf * xf * (3.0 - 2.0 * xf)
+        x0 = xi.astype(np.uint64) % f
+        x1 = (xi.astype(np.uint64) + 1) % f
         s = np.uint32((seed * 0x27D4EB2F + o * 0x165667B1) & 0xFFFFFFFF)
         # rows are y, columns are x; the lattice is square so both axes share x0 / x1
         v00 = _hash2(x0[None, :], x0[:, None], s) / 4294967296.0
@@ -181,10 +184,10 @@ def fbm_height(size: int, seed: int, octaves: int, base_freq: int = 8) -> np.nda
         v11 = _hash2(x1[None, :], x1[:, None], s) / 4294967296.0
         SX = sx[None, :]
         SY = sx[:, None]
-        top = v00 * (1 - SX) + v10 * SX
-        bot = v01 * (1 - SX) + v11 * SX
-        acc += amp * (top * (1 - SY) + bot * SY)
-        amp_sum += amp
+        top = v00 * (1.0 - SX) + v10 * SX
+        bot = v01 * (1.0 - SX) + v11 * SX
+        acc = acc + amp * (top * (1.0 - SY) + bot * SY)
+        amp_sum = amp_sum + amp
     return (acc / amp_sum).astype(np.float32)
 
 
@@ -298,7 +301,7 @@ class Instance:
 class HostScene:
     geometries: list
     instances: list
-    textures: list               # list of (array, format) with format abi.TEXTURE_*
+    textures: list               # (array, abi.TEXTURE_*) pairs or FbmTexture descriptors
     camera: Camera
     tess_rate_pixels: float
     name: str = ""
@@ -354,3 +357,66 @@ def scene_grid_instances(num_instances=64, quads=256, patch_size=16.0, tex_size=
     cam = Camera(eye=(span * 0.5, cam_height, -patch_size * 0.3), center=(span * 0.5, 0.0, span * 0.45),
                  width=width, height=height, near=0.05, far=4 * span)
     return HostScene([geo], insts, [(h, abi.TEXTURE_R16_UNORM)], cam, tess_rate_pixels, "grid_instances")
+
+
+# ---------------------------------------------------- batched 500-mesh frame
+
+
+@dataclass
+class FbmTexture:
+    """A height map generated on the device by tc_fbm_height (numpy's
+    fbm_height gives the same texels on the host)."""
+    size: int
+    seed: int
+    octaves: int
+    format: int
+
+    def host(self) -> np.ndarray:
+        h = fbm_height(self.size, self.seed, self.octaves)
+        return to_r16(h) if self.format == abi.TEXTURE_R16_UNORM else h
+
+    @property
+    def nbytes(self):
+        return self.size * self.size * (2 if self.format == abi.TEXTURE_R16_UNORM else 4)
+
+
+def _grid_sizes():
+    out, n = [], 10.0
+    while n < 330:
+        out.append(int(round(n)))
+        n *= 1.2
+    return out
+
+
+def scene_batched(num_meshes=500, seed=2, tex_size=1024, width=1920, height=1080, tess_rate_pixels=4.0,
+                  field=40.0):
+    """num_meshes meshes with triangle counts log-uniform over 200..200000:
+    grids (nearest of a 1.2-ratio ladder of quad resolutions) and
+    icospheres (20 * (8q)^2 triangles), random rigid transforms with a
+    uniform scale, each with its own tex_size^2 R16 fBm height map."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(200), math.log(200000), num_meshes))
+    kinds = rng.random(num_meshes)
+    ladder = _grid_sizes()
+    geos, key_of, insts, texs = [], {}, [], []
+    for i in range(num_meshes):
+        T = counts[i]
+        if T >= 1280 and kinds[i] < 0.5:
+            key = ("ico", max(1, int(round(math.sqrt(T / 1280.0)))))
+        else:
+            n = math.sqrt(T / 2.0)
+            key = ("grid", min(ladder, key=lambda q: abs(q - n)))
+        if key not in key_of:
+            key_of[key] = len(geos)
+            geos.append(icosphere(8 * key[1], radius=0.5) if key[0] == "ico" else grid_patch(key[1], key[1], 1.0, 1.0))
+        g = key_of[key]
+        s = float(rng.uniform(1.0, 3.0))
+        pos = [float(rng.uniform(-field / 2, field / 2)), float(rng.uniform(0.0, 4.0)), float(rng.uniform(0, field))]
+        M = rigid_transform(float(rng.uniform(0, 2 * math.pi)), pos, float(rng.uniform(-0.5, 0.5)),
+                            float(rng.uniform(-0.5, 0.5)))
+        M[:3, :3] *= s
+        insts.append(Instance(g, M, displacement_index=i, displacement_scale=0.05 * s))
+        texs.append(FbmTexture(tex_size, 1000 * seed + i, 6, abi.TEXTURE_R16_UNORM))
+    cam = Camera(eye=(0.0, 8.0, -12.0), center=(0.0, 1.0, field * 0.4), width=width, height=height, near=0.05,
+                 far=10 * field)
+    return HostScene(geos, insts, texs, cam, tess_rate_pixels, "batched")
