@@ -459,6 +459,11 @@ typedef struct tc_hit_decoded {
 #define TC_FLAG_1X_TRANSIENT (1u << 4)     /* TESS_USE_1X_TRANSIENTBUILDS */
 #define TC_FLAG_2X_TRANSIENT (1u << 5)     /* TESS_USE_2X_TRANSIENTBUILDS */
 #define TC_FLAG_PERSISTENT_SPLIT (1u << 6) /* TESS_USE_PERSISTENT_KERNEL */
+/* No reference counterpart: cluster_classify and
+   triangle_tess_template_instantiate also write one tc_vertex_tape record
+   per generated vertex (tc_bindings.vertexTape), the input of
+   tc_depth_adjoint. */
+#define TC_FLAG_VERTEX_TAPE (1u << 7)
 
 /* The reference's specialisation macros
  * (renderer_raytrace_clusters_tess.cpp:117-138). */
@@ -477,6 +482,22 @@ typedef struct tc_config {
 
 /* The reference's descriptor set (shaderio.h:48-58) as device pointers,
  * plus the caller-owned workspace that replaces dispatch-indirect state. */
+/* How a generated vertex was displaced (own record, no reference
+   counterpart): position = base + normal * (H(u, v) * heightScale +
+   heightOffset) with H the bilinear sample of displacementTextures[texture]
+   (displacement.glsl, cluster_classify.comp.glsl:477-479 and
+   triangle_tess_template_instantiate.comp.glsl, interpolated or cluster
+   attributes).  texture is ~0u for a vertex that was not displaced. */
+typedef struct tc_vertex_tape {
+  float base[3];
+  float normal[3]; /* normalised, object space */
+  float u, v;
+  float heightScale;  /* instance scale * frame scale */
+  float heightOffset; /* instance offset + frame offset */
+  uint32_t texture;
+  uint32_t _pad;
+} tc_vertex_tape;
+
 typedef struct tc_bindings {
   const tc_frame_constants* view;                      /* FRAME_UBO        */
   const tc_tessellation_table* tessTable;              /* TESSTABLE_UBO    */
@@ -488,6 +509,8 @@ typedef struct tc_bindings {
   uint32_t numDisplacementTextures;
   void* workspace;         /* tc_workspace_size() bytes, device memory */
   uint64_t workspaceBytes;
+  struct tc_vertex_tape* vertexTape; /* maxGeneratedVertices records, only read
+                                        with TC_FLAG_VERTEX_TAPE */
 } tc_bindings;
 
 /* ------------------------------------------------------------ entry points */
@@ -624,6 +647,36 @@ typedef struct tc_trace_params {
 tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* params,
                         tc_stream stream);
 
+/* Adjoint of hit distance with respect to height-map texels (no reference
+ * counterpart; the reference is not differentiable).  For every ray with
+ * a hit and gradT[r] != 0 it adds
+ *   gradT[r] * dt/dH(texel)
+ * to gradTexels[texture][texel], where t is the distance along the ray to
+ * the plane of the hit triangle, the triangle's vertices are the
+ * genVertices named by hitVertices, and each vertex moves along its tape
+ * normal by heightScale * H(u, v), H the bilinear REPEAT sample.  The
+ * tessellation pattern and the hit triangle are those of the forward
+ * pass (frozen).  With n the object-space normal of the hit triangle, d
+ * the object-space ray direction and b its barycentrics,
+ *   dt/dP_k = b_k n / (n . d),  dP_k/dH = heightScale * normal_k.
+ * Gradients are accumulated with float atomics (summation order varies). */
+typedef struct tc_adjoint_params {
+  const tc_ray* rays;          /* device, the traced rays (world space) */
+  const tc_hit* hits;          /* device, from tc_trace_rays */
+  const uint32_t* hitVertices; /* device, 4 per ray, from tc_trace_rays */
+  const float* gradT;          /* device, dL/dt per ray */
+  uint32_t numRays;
+  uint32_t numTextures;
+  const float* genVertices;                      /* device, SceneBuilding.genVertices */
+  const tc_vertex_tape* tape;                    /* device, tc_bindings.vertexTape */
+  const tc_tlas_instance* tlasInstances;         /* device, tc_accel.tlasInstances */
+  const tc_displacement_texture* textures;       /* device, numTextures records */
+  float* const* gradTexels; /* device array of numTextures device pointers
+                               (width * height floats each, NULL: skip) */
+} tc_adjoint_params;
+
+tc_status tc_depth_adjoint(const tc_adjoint_params* params, tc_stream stream);
+
 /* Synthetic workload input: a size x size tileable value-noise fBm height
  * map (octaves from baseFreq, integer lattice hash) written to device
  * memory as R16_UNORM or R32_FLOAT.  No reference counterpart: the
@@ -701,6 +754,9 @@ static_assert(offsetof(tc_readback, numGenDatas) == 48, "Readback layout");
 static_assert(sizeof(tc_clas_header) == 48, "clas header");
 static_assert(sizeof(tc_bvh_node) == 64, "bvh node");
 static_assert(sizeof(tc_clas_template) == 32, "clas template");
+static_assert(sizeof(tc_vertex_tape) == 48, "vertex tape");
+static_assert(sizeof(tc_bindings) == 80, "bindings");
+static_assert(sizeof(tc_adjoint_params) == 80, "adjoint params");
 #endif
 
 #endif /* TC_ABI_H */

@@ -101,6 +101,35 @@ void oracle_frame_tessellate(const oracle_ctx* c, tc_config cfg, uint32_t numSpl
  * clusters / items with a deterministic merge); identical output. */
 void oracle_frame_tessellate_mt(const oracle_ctx* c, tc_config cfg, uint32_t numSplitPasses, uint32_t nthreads);
 
+/* ---- depth adjoint checker (tc_oracle_adjoint.cpp; no reference
+ * counterpart).  Height maps hold the sampled (normalised) texel values;
+ * xforms holds 24 floats per instance (objectToWorld | worldToObject, 3x4
+ * row-major, as tc_instance_xforms_host writes them). ---- */
+typedef struct oracle_height_map {
+  const float* texels; /* NULL: not differentiated */
+  uint32_t width, height;
+} oracle_height_map;
+
+/* FP64 distance along each ray to the plane of its hit triangle with the
+ * triangle's vertices re-displaced from the tape, texel perturbTexel of
+ * map perturbMap offset by delta; -1 for rays without a hit. */
+void oracle_depth_t64(const tc_ray* rays, const tc_hit* hits, const uint32_t* hitVertices, uint32_t numRays,
+                      const tc_vertex_tape* tape, const oracle_height_map* maps, uint32_t numMaps, const float* xforms,
+                      uint32_t perturbMap, uint64_t perturbTexel, double delta, double* t);
+
+/* FP64 analytic adjoint: grads[map][texel] += gradT[r] * dt_r/dtexel. */
+void oracle_depth_adjoint64(const tc_ray* rays, const tc_hit* hits, const uint32_t* hitVertices, uint32_t numRays,
+                            const tc_vertex_tape* tape, const oracle_height_map* maps, uint32_t numMaps,
+                            const float* xforms, const double* gradT, double* const* grads);
+
+/* FP64 central differences (step eps) of
+ *   L = scale * sum over rays with a hit and target >= 0 of (t - target)^2
+ * with respect to the listed texels of one map. */
+void oracle_depth_loss_fd64(const tc_ray* rays, const tc_hit* hits, const uint32_t* hitVertices, uint32_t numRays,
+                            const tc_vertex_tape* tape, const oracle_height_map* maps, uint32_t numMaps,
+                            const float* xforms, const float* target, double scale, uint32_t map, const uint64_t* texels,
+                            uint32_t count, double eps, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,9 @@ def test_record_sizes():
         assert C.sizeof(getattr(abi, name)) == size, name
     assert abi.TESS_TRIANGLE_INFO.itemsize == 24 and abi.TEMPLATE_INSTANTIATE_INFO.itemsize == 32
     assert abi.BLAS_BUILD_INFO.itemsize == 16 and abi.CLUSTER_DT.itemsize == 16 and abi.BBOX_DT.itemsize == 32
-    assert C.sizeof(abi.Bindings) == 72
+    assert C.sizeof(abi.Bindings) == 80
+    assert C.sizeof(abi.AdjointParams) == 80
+    assert abi.VERTEX_TAPE_DT.itemsize == 48
 
 
 def test_header_compiles_as_c_and_cxx(tmp_path):

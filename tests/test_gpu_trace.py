@@ -130,5 +130,7 @@ def test_trace_parity_icosphere():
         gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
     finally:
         g.close()
-    assert (oh["t"] > 0).mean() > 0.2
+    # a radius-1 sphere at distance 3 under a 60 degree, 16:9 frustum covers
+    # about 17% of the pixels
+    assert 0.12 < (oh["t"] > 0).mean() < 0.25
     compare_hits(gh, oh, gd, od)

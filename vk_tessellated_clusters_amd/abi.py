@@ -44,6 +44,7 @@ FLAG_PN = 1 << 3
 FLAG_1X_TRANSIENT = 1 << 4
 FLAG_2X_TRANSIENT = 1 << 5
 FLAG_PERSISTENT_SPLIT = 1 << 6
+FLAG_VERTEX_TAPE = 1 << 7
 
 CLAS_GROUP_TRIANGLES = 4
 
@@ -209,7 +210,18 @@ class Config(C.Structure):
 class Bindings(C.Structure):
     _fields_ = [("view", C.c_void_p), ("tessTable", C.c_void_p), ("readback", C.c_void_p), ("instances", C.c_void_p),
                 ("build", C.c_void_p), ("displacementTextures", C.c_void_p), ("numRenderInstances", u32),
-                ("numDisplacementTextures", u32), ("workspace", C.c_void_p), ("workspaceBytes", u64)]
+                ("numDisplacementTextures", u32), ("workspace", C.c_void_p), ("workspaceBytes", u64),
+                ("vertexTape", C.c_void_p)]
+
+
+class AdjointParams(C.Structure):
+    _fields_ = [("rays", C.c_void_p), ("hits", C.c_void_p), ("hitVertices", C.c_void_p), ("gradT", C.c_void_p),
+                ("numRays", u32), ("numTextures", u32), ("genVertices", C.c_void_p), ("tape", C.c_void_p),
+                ("tlasInstances", C.c_void_p), ("textures", C.c_void_p), ("gradTexels", C.c_void_p)]
+
+
+VERTEX_TAPE_DT = np.dtype([("base", "<f4", (3,)), ("normal", "<f4", (3,)), ("u", "<f4"), ("v", "<f4"),
+                           ("heightScale", "<f4"), ("heightOffset", "<f4"), ("texture", "<u4"), ("pad", "<u4")])
 
 
 EXPECTED_SIZES = {"TessTableEntry": 8, "Cluster": 16, "RenderInstance": 192, "ClusterInfo": 8,
@@ -245,7 +257,7 @@ EXPORTS = [
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
     "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
-    "tc_renderer_rebind_instances",
+    "tc_renderer_rebind_instances", "tc_depth_adjoint",
 ]
 
 
@@ -299,6 +311,7 @@ def lib():
         "tc_renderer_graph_capture": (i32, [P, C.POINTER(FrameConstants), C.POINTER(TraceParams), P]),
         "tc_renderer_graph_launch": (i32, [P, C.POINTER(FrameConstants), P]),
         "tc_fbm_height": (i32, [P, u32, u32, u32, u32, u32, P]),
+        "tc_depth_adjoint": (i32, [C.POINTER(AdjointParams), P]),
         "tc_renderer_rebind_instances": (i32, [P, P, P, u32]),
     }
     for name, (res, args) in sig.items():

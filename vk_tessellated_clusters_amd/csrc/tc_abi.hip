@@ -71,7 +71,7 @@ static bool config_ok(const tc_config& c) {
   return true;
 }
 
-static constexpr uint32_t kSupportedFlags = TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN;
+static constexpr uint32_t kSupportedFlags = TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -79,6 +79,7 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   if (cfg.flags & ~kSupportedFlags) return TC_ERROR_UNSUPPORTED;
   if (b->numRenderInstances == 0 || b->numRenderInstances > kMaxInstances) return TC_ERROR_INVALID_ARGUMENT;
   if ((cfg.flags & TC_FLAG_DISPLACEMENT) && b->numDisplacementTextures && !b->displacementTextures) return TC_ERROR_INVALID_ARGUMENT;
+  if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && !b->vertexTape) return TC_ERROR_INVALID_ARGUMENT;
   WsLayout L = ws_layout(cfg, b->numRenderInstances);
   if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   a.view = b->view;
@@ -93,6 +94,7 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   a.status = reinterpret_cast<uint64_t*>(static_cast<char*>(b->workspace) + L.status);
   a.maxTiles = L.maxTiles;
   a._pad = 0;
+  a.tape = (cfg.flags & TC_FLAG_VERTEX_TAPE) ? b->vertexTape : nullptr;
   a.cfg = cfg;
   if (layout) *layout = L;
   return TC_OK;

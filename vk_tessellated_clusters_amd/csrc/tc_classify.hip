@@ -198,6 +198,9 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         for (uint32_t v = lane; v < s.nv; v += kWave) {
           const uint32_t vi = s.firstLocalVertex + v;
           v3 p = ld3(inst.positions, vi);
+          if (a.tape) write_tape(a, (uint64_t)s.vertexOffset + v, p, disp ? normalize(ld3(inst.normals, vi)) : mk(0.f, 0.f, 0.f),
+                                 disp ? ld2(inst.texcoords, vi) : v2{0.f, 0.f}, inst.displacementScale,
+                                 inst.displacementOffset, disp ? (uint32_t)inst.displacementIndex : ~0u);
           if (disp) {
             const float hgt = displaced_height(a, tex, inst.displacementScale, inst.displacementOffset, ld2(inst.texcoords, vi));
             p = add(p, mul(normalize(ld3(inst.normals, vi)), hgt));

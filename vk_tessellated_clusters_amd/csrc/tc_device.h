@@ -39,6 +39,9 @@ __device__ __forceinline__ float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y
 __device__ __forceinline__ float length(v3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ float distance(v3 a, v3 b) { return length(sub(a, b)); }
 __device__ __forceinline__ v3 normalize(v3 a) { return mul(a, 1.0f / length(a)); }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
 
 __device__ __forceinline__ v3 xform_point(const float* m, v3 p) {
   v3 r;
@@ -232,6 +235,23 @@ __device__ __forceinline__ float sample_height(const tc_displacement_texture& t,
   float top = t00 * (1.0f - fx) + t10 * fx;
   float bot = t01 * (1.0f - fx) + t11 * fx;
   return top * (1.0f - fy) + bot * fy;
+}
+
+// The four texels sample_height(t, u, v) reads and their bilinear weights
+// (same coordinate arithmetic), order (x0,y0) (x1,y0) (x0,y1) (x1,y1).
+__device__ __forceinline__ void bilinear_footprint(uint32_t width, uint32_t height, float u, float v, uint64_t idx[4],
+                                                   float w[4]) {
+  float x = u * float(width) - 0.5f;
+  float y = v * float(height) - 0.5f;
+  x = fminf(fmaxf(x, -16777216.0f), 16777216.0f);
+  y = fminf(fmaxf(y, -16777216.0f), 16777216.0f);
+  const float x0 = floorf(x), y0 = floorf(y);
+  const float fx = x - x0, fy = y - y0;
+  const int32_t ix = (int32_t)x0, iy = (int32_t)y0;
+  const uint64_t xa = (uint32_t)wrapi(ix, (int32_t)width), xb = (uint32_t)wrapi(ix + 1, (int32_t)width);
+  const uint64_t ya = (uint32_t)wrapi(iy, (int32_t)height), yb = (uint32_t)wrapi(iy + 1, (int32_t)height);
+  idx[0] = ya * width + xa, idx[1] = ya * width + xb, idx[2] = yb * width + xa, idx[3] = yb * width + xb;
+  w[0] = (1.0f - fx) * (1.0f - fy), w[1] = fx * (1.0f - fy), w[2] = (1.0f - fx) * fy, w[3] = fx * fy;
 }
 
 // displacement.glsl:30-104 (PN triangle), pow written as products

@@ -173,7 +173,12 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
         const v3 n = interp(pb.nrm[0], pb.nrm[1], pb.nrm[2], w);
         const v2 uv = interp2(pb.uv[0], pb.uv[1], pb.uv[2], w);
         const float hgt = displaced_height(a, tex, pb.dispScale, pb.dispOffset, uv);
+        if (a.tape)
+          write_tape(a, (uint64_t)pb.vertexOffset + vert, pos, normalize(n), uv, pb.dispScale, pb.dispOffset,
+                     (uint32_t)pb.dispIndex);
         pos = add(pos, mul(normalize(n), hgt));
+      } else if (a.tape) {
+        write_tape(a, (uint64_t)pb.vertexOffset + vert, pos, mk(0.f, 0.f, 0.f), v2{0.f, 0.f}, 0.f, 0.f, ~0u);
       }
       st3(b->genVertices, (uint64_t)pb.vertexOffset + vert, pos);
     }

@@ -23,6 +23,7 @@ struct KArgs {
   uint64_t* status;  // kMaxCounters x maxTiles words
   uint32_t maxTiles;
   uint32_t _pad;
+  tc_vertex_tape* tape;  // non-null with TC_FLAG_VERTEX_TAPE
   tc_config cfg;
 };
 

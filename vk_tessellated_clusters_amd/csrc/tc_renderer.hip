@@ -25,6 +25,7 @@ struct Renderer {
   tc_tessellation_table* table = nullptr;
   void* workspace = nullptr;
   uint64_t workspaceBytes = 0;
+  tc_vertex_tape* tape = nullptr;  // with TC_FLAG_VERTEX_TAPE
   tc_scene_building hostBuild{};
   tc_frame_constants* pinnedView = nullptr;
   uint64_t splitBytes = 0;
@@ -135,7 +136,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel;
+  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
       {&oRb, sizeof(tc_readback)},
@@ -159,6 +160,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&oWs, wsBytes},
       {&oXf, 24ull * 4 * numInstances},
       {&oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances)},
+      {&oTape, (c.flags & TC_FLAG_VERTEX_TAPE) ? sizeof(tc_vertex_tape) * (uint64_t)c.maxGeneratedVertices : 0},
   };
   uint64_t o = 0;
   for (auto& s : slots) {
@@ -181,6 +183,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   r->workspace = mem + oWs;
   r->workspaceBytes = wsBytes;
   r->xforms = reinterpret_cast<float*>(mem + oXf);
+  if (c.flags & TC_FLAG_VERTEX_TAPE) r->tape = reinterpret_cast<tc_vertex_tape*>(mem + oTape);
   r->ownInstances = r->instances;
   r->ownXforms = r->xforms;
   tc_accel_bind(mem + oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances), c.maxGeneratedClusters, numInstances,
@@ -244,6 +247,7 @@ tc_status tc_renderer_get_bindings(tc_renderer* rr, tc_bindings* out) {
   out->numDisplacementTextures = r->numTextures;
   out->workspace = r->workspace;
   out->workspaceBytes = r->workspaceBytes;
+  out->vertexTape = r->tape;
   return TC_OK;
 }
 

@@ -32,6 +32,15 @@ __device__ __forceinline__ float displaced_height(const KArgs& a, const tc_displ
   return (((h * scale) * a.view->displacementScale) + offset) + a.view->displacementOffset;
 }
 
+// Records how generated vertex `index` was displaced (tc_vertex_tape).
+__device__ __forceinline__ void write_tape(const KArgs& a, uint64_t index, v3 base, v3 normal, v2 uv, float scale,
+                                           float offset, uint32_t texture) {
+  float4* t = reinterpret_cast<float4*>(a.tape + index);
+  t[0] = make_float4(base.x, base.y, base.z, normal.x);
+  t[1] = make_float4(normal.y, normal.z, uv.x, uv.y);
+  t[2] = make_float4(scale * a.view->displacementScale, offset + a.view->displacementOffset, __uint_as_float(texture), 0.0f);
+}
+
 __device__ __forceinline__ uint64_t max_data_bytes(const tc_config& c) {
   return (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull;
 }

@@ -113,7 +113,14 @@ class DeviceScene:
                 t.texels, t.width, t.height, t.format = buf.data_ptr(), tex.size, tex.size, tex.format
             else:
                 arr, fmt = tex
-                t.texels = up(arr)
+                if isinstance(arr, torch.Tensor):
+                    # device-resident texels the caller owns (kept alive here)
+                    if not arr.is_cuda or not arr.is_contiguous() or arr.dim() != 2:
+                        raise ValueError("texture tensors must be contiguous 2-D device tensors")
+                    self._keep.append(arr)
+                    t.texels = arr.data_ptr()
+                else:
+                    t.texels = up(arr)
                 t.height, t.width = arr.shape
                 t.format = fmt
         self.num_textures = len(host.textures)
