@@ -282,6 +282,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", dest="graph", action="store_false")
     ap.add_argument("--per-mesh-reps", type=int, default=2)
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
+                    help="pick launch shapes by measurement first (default for batched500)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
@@ -320,6 +323,10 @@ def main():
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
+    from vk_tessellated_clusters_amd import autotune
+    autotuned = args.autotune if args.autotune is not None else args.workload == "batched500"
+    if autotuned:
+        autotune.autotune(r, fc, tp, stream, reps=3, log=lambda m: print(m, file=sys.stderr, flush=True))
     if args.graph:
         abi.check(r.L.tc_renderer_graph_capture(r.handle, C.byref(fc), C.byref(tp), C.c_void_p(stream.cuda_stream)),
                   "graph capture")
@@ -376,6 +383,7 @@ def main():
                    "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
                    "clas_per_frame": int(sb.tempInstantiateCounter), "gen_vertices": int(sb.genVertexCounter)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "tuning": dict(autotune.get_tuning(), autotuned=bool(autotuned)),
         "cluster_bvh_build_ms_per_frame": bvh_ms,
         "mrays_per_s": (n_primary + n_shadow) / (trace_ms * 1e-3) / 1e6 if trace_ms else None,
         "rays_per_frame": {"primary": n_primary, "shadow": n_shadow},

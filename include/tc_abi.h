@@ -408,6 +408,19 @@ typedef struct tc_bvh_node {
 } tc_bvh_node;
 #define TC_BVH_LEAF 0x80000000u
 
+/* Four-wide view of the same tree, one record per internal node of the
+ * two-child tree with the same index: the slots are the node's children,
+ * with every internal child replaced by that child's two children (leaf
+ * children stay leaves).  Traversal visits every other level of the binary
+ * tree.  Unused slots hold TC_BVH4_EMPTY. */
+typedef struct tc_bvh4_node {
+  float lo[3][4]; /* [axis][slot] */
+  float hi[3][4];
+  uint32_t child[4]; /* TC_BVH_LEAF | leaf, internal node index, or TC_BVH4_EMPTY */
+  uint32_t _pad[4];
+} tc_bvh4_node;
+#define TC_BVH4_EMPTY 0xFFFFFFFFu
+
 /* One TLAS instance (replaces VkAccelerationStructureInstanceKHR,
  * renderer.cpp:240-266). */
 typedef struct tc_tlas_instance {
@@ -518,6 +531,28 @@ typedef struct tc_bindings {
 /* Device count / basic probe.  Returns TC_ERROR_NO_DEVICE without a GPU. */
 tc_status tc_device_check(int* deviceCount);
 
+/* Launch shapes of the work-queue kernels (no reference counterpart: the
+ * reference's dispatch sizes are fixed by its shaders and indirect
+ * arguments).  Grids are given in workgroups per CU; the work-queue kernels
+ * take their tiles by ticket, so every setting produces identical results
+ * and only the speed changes.  traceThreads is the traversal workgroup
+ * size (64, 128 or 256).  Settings are process-wide and apply to later
+ * calls (a captured graph keeps the shapes it was captured with); a NULL
+ * argument to tc_set_tuning restores the defaults. */
+typedef struct tc_tuning {
+  uint32_t classifyBlocksPerCU;    /* cluster_classify, 256-thread workgroups */
+  uint32_t splitBlocksPerCU;       /* triangle_split, 64-thread workgroups */
+  uint32_t instantiateBlocksPerCU; /* template instantiate, 256-thread workgroups */
+  uint32_t clasWavesPerCU;         /* CLAS build, one wave per CLAS */
+  uint32_t traceBlocksPerCU;       /* traversal workgroups per CU */
+  uint32_t traceThreads;           /* traversal workgroup size */
+  uint32_t bvhBlocksPerCU;         /* BVH keys / hierarchy / collapse */
+  uint32_t _reserved;
+} tc_tuning;
+
+tc_status tc_get_tuning(tc_tuning* out);
+tc_status tc_set_tuning(const tc_tuning* tuning);
+
 /* Bytes of workspace the stage kernels need for `config`. */
 uint64_t tc_workspace_size(const tc_config* config);
 
@@ -608,6 +643,8 @@ typedef struct tc_accel {
   uint32_t capacity; /* CLAS references (>= config.maxGeneratedClusters) */
   void* memory;
   uint64_t memoryBytes;
+  tc_bvh4_node* blasNodes4; /* four-wide view of blasNodes, same indices */
+  tc_bvh4_node* tlasNodes4; /* four-wide view of tlasNodes */
 } tc_accel;
 
 /* Bytes of device memory for tc_accel_bind. */
@@ -626,6 +663,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
 
 #define TC_TRACE_CULL_BACK (1u << 0) /* gl_RayFlagsCullBackFacingTrianglesEXT on primary rays */
 #define TC_TRACE_SHADOW (1u << 1)    /* one shadow ray toward skyParams.sunDirection per hit */
+#define TC_TRACE_PRIVATE_STACK (1u << 2) /* keep the whole traversal stack in private memory
+                                            (slower; exercises the overflow part of the stack) */
 
 /* Rays are either explicit (rays != NULL, numRays) or the primary rays of
  * the frame camera, one per pixel of width x height (rgen:52-86).  Hits are
@@ -753,6 +792,7 @@ static_assert(sizeof(tc_scene_building) == 368, "SceneBuilding");
 static_assert(offsetof(tc_readback, numGenDatas) == 48, "Readback layout");
 static_assert(sizeof(tc_clas_header) == 48, "clas header");
 static_assert(sizeof(tc_bvh_node) == 64, "bvh node");
+static_assert(sizeof(tc_bvh4_node) == 128, "bvh4 node");
 static_assert(sizeof(tc_clas_template) == 32, "clas template");
 static_assert(sizeof(tc_vertex_tape) == 48, "vertex tape");
 static_assert(sizeof(tc_bindings) == 80, "bindings");

@@ -101,3 +101,26 @@ def test_cluster_template_order_is_a_permutation():
         assert sorted((w >> 24).tolist()) == list(range(nt))
         src = geo.local_triangles[3 * t0:3 * (t0 + nt)].reshape(-1, 3)[w >> 24]
         assert np.array_equal(w & 0xFF, src[:, 0]) and np.array_equal((w >> 8) & 0xFF, src[:, 1])
+
+
+def test_tuning_roundtrip_and_validation():
+    import ctypes as C
+    L = abi.lib()
+    t = abi.Tuning()
+    abi.check(L.tc_get_tuning(C.byref(t)))
+    default = t.as_dict()
+    assert default["traceThreads"] in (64, 128, 256) and all(v > 0 for v in default.values())
+    try:
+        t.classifyBlocksPerCU, t.traceThreads = 7, 256
+        abi.check(L.tc_set_tuning(C.byref(t)))
+        u = abi.Tuning()
+        abi.check(L.tc_get_tuning(C.byref(u)))
+        assert u.classifyBlocksPerCU == 7 and u.traceThreads == 256
+        bad = abi.Tuning(**dict(u.as_dict(), traceThreads=96))
+        assert L.tc_set_tuning(C.byref(bad)) == abi.TC_ERROR_INVALID_ARGUMENT
+        bad = abi.Tuning(**dict(u.as_dict(), splitBlocksPerCU=0))
+        assert L.tc_set_tuning(C.byref(bad)) == abi.TC_ERROR_INVALID_ARGUMENT
+    finally:
+        abi.check(L.tc_set_tuning(None))
+    abi.check(L.tc_get_tuning(C.byref(t)))
+    assert t.as_dict() == default

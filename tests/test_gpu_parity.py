@@ -84,6 +84,53 @@ def test_repeat_frames_are_identical():
     harness.compare(r2, r1, max_ulp=0)
 
 
+def test_launch_shapes_do_not_change_results():
+    """Work-queue kernels give identical frames, BVHs and hits for every
+    launch shape the tuner may pick."""
+    import numpy as np
+    from vk_tessellated_clusters_amd import autotune
+    host = small_grid()
+    cfg = harness.small_config()
+    shapes = [None, dict(classifyBlocksPerCU=1, splitBlocksPerCU=1, instantiateBlocksPerCU=1, clasWavesPerCU=4,
+                         traceBlocksPerCU=2, traceThreads=64, bvhBlocksPerCU=1),
+              dict(classifyBlocksPerCU=12, splitBlocksPerCU=16, instantiateBlocksPerCU=8, clasWavesPerCU=64,
+                   traceBlocksPerCU=32, traceThreads=256, bvhBlocksPerCU=16)]
+    out = []
+    try:
+        for shape in shapes:
+            autotune.set_tuning(shape)
+            g = harness.GpuFrame(host, cfg)
+            try:
+                g.run(host.frame(), stages="accel")
+                res = g.results()
+                arr = g.accel_arrays()
+                hits, dec = g.trace(96, 54, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+            finally:
+                g.close()
+            out.append((res, arr, hits, dec))
+    finally:
+        autotune.set_tuning(None)
+    def written(a):
+        """Node records that the build writes (n - 1 internal nodes per
+        instance) and leaves as offsets into the CLAS pool."""
+        sb, n = a["sb"], len(a["infos"])
+        keep = np.zeros(len(a["blasNodes"]), bool)
+        for info in a["infos"]:
+            first = int(info["clusterReferences"] - sb.blasClusterAddresses) // 8
+            keep[first:first + max(0, int(info["clusterReferencesCount"]) - 1)] = True
+        return (a["blasNodes"][keep].view(np.uint8), a["blasNodes4"][keep].view(np.uint8),
+                a["tlasNodes"][:n - 1].view(np.uint8), a["tlasNodes4"][:n - 1].view(np.uint8),
+                a["blasLeaves"] - np.uint64(sb.genClusterData))
+
+    r0, a0, h0, d0 = out[0]
+    for r, a, h, d in out[1:]:
+        harness.compare(r, r0, max_ulp=0)
+        for x, y in zip(written(a), written(a0)):
+            assert np.array_equal(x, y)
+        assert np.array_equal(h.view(np.uint8), h0.view(np.uint8))
+        assert np.array_equal(d.view(np.uint8), d0.view(np.uint8))
+
+
 def test_device_fbm_matches_numpy():
     """The device height-map generator reproduces the numpy generator texel
     for texel (R32 and R16)."""

@@ -55,6 +55,27 @@ def check_blas(arr, boxes, inst):
 
     visit(0)
     assert len(seen_nodes) == n - 1 and sorted(seen_leaves) == list(range(n))
+    check_wide(nodes, arr["blasNodes4"][first:first + n - 1])
+
+
+def check_wide(nodes, wide):
+    """Every four-wide record lists the node's leaf children and the
+    children of its internal children, with the binary tree's boxes."""
+    for i, nd in enumerate(nodes):
+        want = []
+        for c in (0, 1):
+            ch, lo, hi = int(nd["child%d" % c]), nd["lo%d" % c], nd["hi%d" % c]
+            if ch & abi.BVH_LEAF:
+                want.append((ch, tuple(lo), tuple(hi)))
+            else:
+                g = nodes[ch]
+                want += [(int(g["child0"]), tuple(g["lo0"]), tuple(g["hi0"])),
+                         (int(g["child1"]), tuple(g["lo1"]), tuple(g["hi1"]))]
+        w = wide[i]
+        got = [(int(w["child"][k]), tuple(w["lo"][:, k]), tuple(w["hi"][:, k])) for k in range(4)
+               if int(w["child"][k]) != abi.BVH4_EMPTY]
+        assert got == want, i
+        assert all(int(w["child"][k]) == abi.BVH4_EMPTY for k in range(len(want), 4))
 
 
 def test_accel_structure():
@@ -81,6 +102,7 @@ def test_accel_structure():
 
         tvisit(arr["tlasRoot"])
         assert sorted(seen) == list(range(n))
+        check_wide(arr["tlasNodes"][:n - 1], arr["tlasNodes4"][:n - 1])
     finally:
         g.close()
 
@@ -110,8 +132,12 @@ def test_trace_parity_primary_and_shadow():
     try:
         g.run(host.frame(), stages="accel")
         gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+        # the same traversal with the whole stack in private memory
+        ph, pd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW | abi.TRACE_PRIVATE_STACK)
     finally:
         g.close()
+    assert np.array_equal(ph.view(np.uint8), gh.view(np.uint8))
+    assert np.array_equal(pd.view(np.uint8), gd.view(np.uint8))
     assert (oh["t"] > 0).mean() > 0.3, "camera must see geometry"
     agree = compare_hits(gh, oh, gd, od)
     print("hit agreement", agree, "hits", (oh["t"] > 0).sum(), "shadowed", od["shadowed"].sum())

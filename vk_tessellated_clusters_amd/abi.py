@@ -16,6 +16,7 @@ LIB_PATH = PKG / "libtc_hip.so"
 u8, u16, u32, i32, u64, f32 = C.c_uint8, C.c_uint16, C.c_uint32, C.c_int32, C.c_uint64, C.c_float
 
 TC_OK = 0
+TC_ERROR_INVALID_ARGUMENT = 1
 TC_STATUS = {0: "TC_OK", 1: "TC_ERROR_INVALID_ARGUMENT", 2: "TC_ERROR_LAUNCH", 3: "TC_ERROR_OUT_OF_MEMORY",
              4: "TC_ERROR_NO_DEVICE", 5: "TC_ERROR_UNSUPPORTED", 6: "TC_ERROR_CAPACITY"}
 
@@ -174,7 +175,7 @@ class HitDecoded(C.Structure):
 class Accel(C.Structure):
     _fields_ = [("blasNodes", C.c_void_p), ("blasLeaves", C.c_void_p), ("tlasInstances", C.c_void_p),
                 ("tlasNodes", C.c_void_p), ("tlasRoot", C.c_void_p), ("numInstances", u32), ("capacity", u32),
-                ("memory", C.c_void_p), ("memoryBytes", u64)]
+                ("memory", C.c_void_p), ("memoryBytes", u64), ("blasNodes4", C.c_void_p), ("tlasNodes4", C.c_void_p)]
 
 
 class TraceParams(C.Structure):
@@ -184,6 +185,7 @@ class TraceParams(C.Structure):
 
 TRACE_CULL_BACK = 1
 TRACE_SHADOW = 2
+TRACE_PRIVATE_STACK = 4
 
 RAY_DT = np.dtype([("origin", "<f4", (3,)), ("tMin", "<f4"), ("direction", "<f4", (3,)), ("tMax", "<f4")])
 HIT_DT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instanceID", "<u4"), ("clusterID", "<u4"),
@@ -193,6 +195,8 @@ HIT_DECODED_DT = np.dtype([("instanceID", "<u4"), ("clusterID", "<u4"), ("triang
                            ("wPos", "<f4", (3,)), ("shadowed", "<u4")])
 BVH_NODE_DT = np.dtype([("lo0", "<f4", (3,)), ("child0", "<u4"), ("hi0", "<f4", (3,)), ("child1", "<u4"),
                         ("lo1", "<f4", (3,)), ("pad0", "<u4"), ("hi1", "<f4", (3,)), ("pad1", "<u4")])
+BVH4_NODE_DT = np.dtype([("lo", "<f4", (3, 4)), ("hi", "<f4", (3, 4)), ("child", "<u4", (4,)), ("pad", "<u4", (4,))])
+BVH4_EMPTY = 0xFFFFFFFF
 TLAS_INSTANCE_DT = np.dtype([("objectToWorld", "<f4", (12,)), ("worldToObject", "<f4", (12,)), ("blasNodes", "<u8"),
                              ("blasLeaves", "<u8"), ("blasRoot", "<u4"), ("numClusters", "<u4"), ("flags", "<u4"),
                              ("pad", "<u4")])
@@ -212,6 +216,15 @@ class Bindings(C.Structure):
                 ("build", C.c_void_p), ("displacementTextures", C.c_void_p), ("numRenderInstances", u32),
                 ("numDisplacementTextures", u32), ("workspace", C.c_void_p), ("workspaceBytes", u64),
                 ("vertexTape", C.c_void_p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("classifyBlocksPerCU", u32), ("splitBlocksPerCU", u32), ("instantiateBlocksPerCU", u32),
+                ("clasWavesPerCU", u32), ("traceBlocksPerCU", u32), ("traceThreads", u32), ("bvhBlocksPerCU", u32),
+                ("_reserved", u32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if not k.startswith("_")}
 
 
 class AdjointParams(C.Structure):
@@ -257,7 +270,7 @@ EXPORTS = [
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
     "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
-    "tc_renderer_rebind_instances", "tc_depth_adjoint",
+    "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning",
 ]
 
 
@@ -312,6 +325,8 @@ def lib():
         "tc_renderer_graph_launch": (i32, [P, C.POINTER(FrameConstants), P]),
         "tc_fbm_height": (i32, [P, u32, u32, u32, u32, u32, P]),
         "tc_depth_adjoint": (i32, [C.POINTER(AdjointParams), P]),
+        "tc_get_tuning": (i32, [C.POINTER(Tuning)]),
+        "tc_set_tuning": (i32, [C.POINTER(Tuning)]),
         "tc_renderer_rebind_instances": (i32, [P, P, P, u32]),
     }
     for name, (res, args) in sig.items():

@@ -117,6 +117,35 @@ static SortBuffers sort_buffers(const tc_bindings* b, const WsLayout& L) {
 tc_status make_args_public(const tc_bindings* b, const tc_config& cfg, KArgs& a) { return make_args(b, cfg, a); }
 int num_cu_public() { return num_cu(); }
 
+static constexpr tc_tuning kDefaultTuning = {5, 8, 3, 32, 16, 128, 8, 0};
+static std::mutex g_tuningMutex;
+static tc_tuning g_tuning = kDefaultTuning;
+
+tc_tuning tuning_snapshot() {
+  std::lock_guard<std::mutex> lock(g_tuningMutex);
+  return g_tuning;
+}
+
+extern "C" tc_status tc_get_tuning(tc_tuning* out) {
+  if (!out) return TC_ERROR_INVALID_ARGUMENT;
+  *out = tuning_snapshot();
+  return TC_OK;
+}
+
+extern "C" tc_status tc_set_tuning(const tc_tuning* t) {
+  if (t) {
+    const uint32_t v[7] = {t->classifyBlocksPerCU, t->splitBlocksPerCU, t->instantiateBlocksPerCU, t->clasWavesPerCU,
+                           t->traceBlocksPerCU, t->traceThreads, t->bvhBlocksPerCU};
+    for (uint32_t x : v)
+      if (x == 0 || x > 4096) return TC_ERROR_INVALID_ARGUMENT;
+    if (t->traceThreads != 64 && t->traceThreads != 128 && t->traceThreads != 256) return TC_ERROR_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_tuningMutex);
+  g_tuning = t ? *t : kDefaultTuning;
+  g_tuning._reserved = 0;
+  return TC_OK;
+}
+
 static tc_status launched() { return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH; }
 
 // ------------------------------------------------------ table templates
@@ -416,7 +445,7 @@ tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream 
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)0);
   uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
-  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 5);
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().classifyBlocksPerCU);
   hipLaunchKernelGGL(k_cluster_classify, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
   return launched();
 }
@@ -428,7 +457,7 @@ tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream st
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)1);
   uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
-  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 8);
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU);
   hipLaunchKernelGGL(k_triangle_split, dim3(grid), dim3(64), 0, st, a);
   return launched();
 }
@@ -440,7 +469,7 @@ tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config 
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)2);
   uint32_t tiles = (config.maxPartTriangles + kInstTile - 1) / kInstTile;
-  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * 3);
+  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().instantiateBlocksPerCU);
   if (config.flags & TC_FLAG_PN)
     hipLaunchKernelGGL(k_instantiate<true>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
   else

@@ -19,7 +19,9 @@ struct BvhArgs {
   uint32_t* child;     // 2 per internal node, relative to the instance start
   uint32_t* parent;    // [0, capacity) internal nodes, [capacity, 2 capacity) leaves
   uint32_t* counter;   // refit arrival counters
+  uint32_t* rangeEnd;  // per internal node: the far end of its key range (global position)
   tc_bvh_node* nodes;  // blasNodes
+  tc_bvh4_node* nodes4;  // blasNodes4
   tc_aabb* nodeBox;    // union box of every internal node
   // TLAS
   tc_tlas_instance* tlasInst;
@@ -32,12 +34,14 @@ struct BvhArgs {
   uint32_t* tlasParent;
   uint32_t* tlasCounter;
   tc_bvh_node* tlasNodes;
+  tc_bvh4_node* tlasNodes4;
   tc_aabb* tlasNodeBox;
   uint32_t* tlasRoot;  // [0] root word, [1] instance count
 };
 
 struct AccelLayout {
-  uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter;
+  uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter, rangeEnd;
+  uint64_t blasNodes4, tlasNodes4;
   uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasRoot;
   uint64_t total;
 };

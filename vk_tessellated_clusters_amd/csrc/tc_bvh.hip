@@ -7,7 +7,9 @@
 //   sort    the LDS radix sort of tc_sort.hip (stable, CLAS order breaks ties);
 //   build   Karras' binary radix tree over each instance's key segment
 //           (internal nodes of instance i live at [s_i, s_i + n_i - 1));
-//   refit   bottom-up with one arrival counter per internal node;
+//   refit   bottom-up with one arrival counter per internal node, in LDS
+//           for nodes whose key range lies inside one workgroup's chunk of
+//           sorted positions, in global memory for the few that straddle;
 //   TLAS    the same builder over the instances' world-space boxes.
 #include <algorithm>
 
@@ -86,7 +88,7 @@ __device__ __forceinline__ int delta(const uint64_t* keys, int64_t i, int64_t j,
 // Karras (2012) radix tree for the segment [first, last] of sorted keys;
 // thread handles internal node `g` (global index first + local j).
 __device__ __forceinline__ void karras_node(const uint64_t* keys, int64_t first, int64_t last, int64_t i, uint32_t* child,
-                                            uint32_t* parent, uint32_t leafParentBase) {
+                                            uint32_t* parent, uint32_t leafParentBase, uint32_t* rangeEnd = nullptr) {
   const int d = (delta(keys, i, i + 1, first, last) - delta(keys, i, i - 1, first, last)) >= 0 ? 1 : -1;
   const int dmin = delta(keys, i, i - d, first, last);
   int64_t lmax = 2;
@@ -104,6 +106,7 @@ __device__ __forceinline__ void karras_node(const uint64_t* keys, int64_t first,
   } while (t > 1);
   const int64_t gamma = i + s * d + min(d, 0);
   const int64_t lo = min(i, j), hi = max(i, j);
+  if (rangeEnd) rangeEnd[i] = (uint32_t)j;
   // children: leaf flag + position relative to `first`, or internal node
   const uint32_t c0 = (lo == gamma) ? (TC_BVH_LEAF | (uint32_t)(gamma - first)) : (uint32_t)(gamma - first);
   const uint32_t c1 = (hi == gamma + 1) ? (TC_BVH_LEAF | (uint32_t)(gamma + 1 - first)) : (uint32_t)(gamma + 1 - first);
@@ -140,7 +143,7 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     blas_segment(a, inst, n, first, last);
     a.leafAddr[k] = refs[a.sortedVals[k]];
     a.counter[k] = 0;
-    if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity);
+    if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity, a.rangeEnd);
   }
 }
 
@@ -149,44 +152,111 @@ __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[
   for (int c = 0; c < 3; c++) lo[c] = h.lo[c], hi[c] = h.hi[c];
 }
 
-// Bottom-up refit: each leaf climbs; the second arrival at a node writes it.
-__global__ void k_blas_refit(BvhArgs a) {
+// Bottom-up refit: every leaf climbs, the second arrival at a node writes
+// it.  A workgroup owns a chunk of kRefitChunk sorted positions; nodes whose
+// key range lies inside the chunk keep their arrival counter and box in
+// LDS (workgroup-scope ordering), the ancestors that straddle chunks use
+// global counters with device-scope fences.  Ranges nest, so a climb
+// switches from the local to the global form at most once.
+constexpr uint32_t kRefitChunk = 1024;
+constexpr uint32_t kRefitThreads = 256;
+
+__global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
+  __shared__ uint32_t s_cnt[kRefitChunk];
+  __shared__ tc_aabb s_box[kRefitChunk];
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  for (uint32_t c0 = blockIdx.x * kRefitChunk; c0 < n; c0 += gridDim.x * kRefitChunk) {
+    const uint32_t c1 = min(c0 + kRefitChunk, n);
+    for (uint32_t i = threadIdx.x; i < kRefitChunk; i += kRefitThreads) s_cnt[i] = 0;
+    __syncthreads();
+    for (uint32_t k = c0 + threadIdx.x; k < c1; k += kRefitThreads) {
+      const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
+      int64_t first, last;
+      blas_segment(a, inst, n, first, last);
+      if (first == last) continue;  // single CLAS: the root is the leaf
+      uint32_t node = a.parent[a.capacity + k];
+      while (true) {
+        const uint32_t far = a.rangeEnd[node];
+        const bool local = node >= c0 && node < c1 && far >= c0 && far < c1;
+        if (local) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          if (atomicAdd(&s_cnt[node - c0], 1u) == 0) break;  // first arrival stops
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else {
+          __threadfence();
+          if (atomicAdd(&a.counter[node], 1u) == 0) break;
+          __threadfence();
+        }
+        tc_bvh_node out;
+        float lo[2][3], hi[2][3];
+        for (int c = 0; c < 2; c++) {
+          const uint32_t ch = a.child[2 * node + c];
+          if (ch & TC_BVH_LEAF) {
+            leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
+          } else {
+            // a local node's children are local to the same chunk
+            const uint32_t g = (uint32_t)first + ch;
+            const tc_aabb bx = local ? s_box[g - c0] : a.nodeBox[g];
+            for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+          }
+        }
+        for (int q = 0; q < 3; q++) {
+          out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
+        }
+        out.child0 = a.child[2 * node + 0];
+        out.child1 = a.child[2 * node + 1];
+        out._pad0 = out._pad1 = 0;
+        a.nodes[node] = out;
+        tc_aabb self;
+        for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
+        a.nodeBox[node] = self;  // read by straddling parents and the TLAS
+        if (local) s_box[node - c0] = self;
+        if ((int64_t)node == first) break;  // instance root
+        node = a.parent[node];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Four-wide record of internal node i of a tree whose node array starts at
+// `nodes` (child indices relative to it).
+__device__ __forceinline__ void collapse4(const tc_bvh_node* nodes, uint32_t i, tc_bvh4_node* out) {
+  const tc_bvh_node n = nodes[i];
+  tc_bvh4_node o;
+  int s = 0;
+  auto put = [&](const float* lo, const float* hi, uint32_t c) {
+    for (int q = 0; q < 3; q++) o.lo[q][s] = lo[q], o.hi[q][s] = hi[q];
+    o.child[s] = c;
+    s++;
+  };
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = c ? n.child1 : n.child0;
+    if (ch & TC_BVH_LEAF) {
+      put(c ? n.lo1 : n.lo0, c ? n.hi1 : n.hi0, ch);
+    } else {
+      const tc_bvh_node g = nodes[ch];
+      put(g.lo0, g.hi0, g.child0);
+      put(g.lo1, g.hi1, g.child1);
+    }
+  }
+  for (; s < 4; s++) {
+    for (int q = 0; q < 3; q++) o.lo[q][s] = INFINITY, o.hi[q][s] = -INFINITY;
+    o.child[s] = TC_BVH4_EMPTY;
+  }
+  for (int q = 0; q < 4; q++) o._pad[q] = 0;
+  out[i] = o;
+}
+
+// One thread per internal BLAS node (after the refit).
+__global__ void k_blas_collapse4(BvhArgs a) {
+  const uint32_t n = min(a.build->blasClusterCounter, a.capacity);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
     int64_t first, last;
     blas_segment(a, inst, n, first, last);
-    if (first == last) continue;  // single CLAS: the root is the leaf
-    uint32_t node = a.parent[a.capacity + k];
-    while (true) {
-      __threadfence();
-      if (atomicAdd(&a.counter[node], 1u) == 0) break;  // first arrival stops
-      __threadfence();
-      tc_bvh_node out;
-      float lo[2][3], hi[2][3];
-      for (int c = 0; c < 2; c++) {
-        const uint32_t ch = a.child[2 * node + c];
-        if (ch & TC_BVH_LEAF) {
-          leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
-        } else {
-          const tc_aabb bx = a.nodeBox[(uint32_t)first + ch];
-          for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
-        }
-      }
-      for (int q = 0; q < 3; q++) {
-        out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
-      }
-      out.child0 = a.child[2 * node + 0];
-      out.child1 = a.child[2 * node + 1];
-      out._pad0 = out._pad1 = 0;
-      a.nodes[node] = out;
-      tc_aabb self;
-      for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
-      a.nodeBox[node] = self;
-      if ((int64_t)node == first) break;  // instance root
-      node = a.parent[node];
-    }
+    if ((int64_t)k < last) collapse4(a.nodes + first, (uint32_t)(k - first), a.nodes4 + first);
   }
 }
 
@@ -307,6 +377,11 @@ __global__ void k_tlas_refit(BvhArgs a) {
   }
 }
 
+__global__ void k_tlas_collapse4(BvhArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 < a.numInstances) collapse4(a.tlasNodes, i, a.tlasNodes4);
+}
+
 // TLAS root word (internal node 0, or the single instance) and the TLAS
 // element count read by the sort
 __global__ void k_tlas_root(BvhArgs a) {
@@ -333,13 +408,16 @@ AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances) {
   };
   const uint64_t c = capacity, n = std::max(numInstances, 1u);
   put(L.blasNodes, c * sizeof(tc_bvh_node));
+  put(L.blasNodes4, c * sizeof(tc_bvh4_node));
   put(L.blasLeaves, c * 8);
   put(L.nodeBox, c * sizeof(tc_aabb));
   put(L.child, 2 * c * 4);
   put(L.parent, 2 * c * 4);
   put(L.counter, c * 4);
+  put(L.rangeEnd, c * 4);
   put(L.tlasInstances, n * sizeof(tc_tlas_instance));
   put(L.tlasNodes, n * sizeof(tc_bvh_node));
+  put(L.tlasNodes4, n * sizeof(tc_bvh4_node));
   put(L.tlasNodeBox, n * sizeof(tc_aabb));
   put(L.tlasLeafBox, n * sizeof(tc_aabb));
   put(L.tlasChild, 2 * n * 4);
@@ -368,6 +446,8 @@ tc_status tc_accel_bind(void* memory, uint64_t bytes, uint32_t capacity, uint32_
   out->tlasInstances = reinterpret_cast<tc_tlas_instance*>(m + L.tlasInstances);
   out->tlasNodes = reinterpret_cast<tc_bvh_node*>(m + L.tlasNodes);
   out->tlasRoot = reinterpret_cast<uint32_t*>(m + L.tlasRoot);
+  out->blasNodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.blasNodes4);
+  out->tlasNodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.tlasNodes4);
   out->numInstances = numInstances;
   out->capacity = capacity;
   out->memory = memory;
@@ -411,10 +491,13 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   a.child = reinterpret_cast<uint32_t*>(m + L.child);
   a.parent = reinterpret_cast<uint32_t*>(m + L.parent);
   a.counter = reinterpret_cast<uint32_t*>(m + L.counter);
+  a.rangeEnd = reinterpret_cast<uint32_t*>(m + L.rangeEnd);
   a.nodes = reinterpret_cast<tc_bvh_node*>(m + L.blasNodes);
+  a.nodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.blasNodes4);
   a.nodeBox = reinterpret_cast<tc_aabb*>(m + L.nodeBox);
   a.tlasInst = reinterpret_cast<tc_tlas_instance*>(m + L.tlasInstances);
   a.tlasNodes = reinterpret_cast<tc_bvh_node*>(m + L.tlasNodes);
+  a.tlasNodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.tlasNodes4);
   a.tlasNodeBox = reinterpret_cast<tc_aabb*>(m + L.tlasNodeBox);
   a.tlasLeafBox = reinterpret_cast<tc_aabb*>(m + L.tlasLeafBox);
   a.tlasChild = reinterpret_cast<uint32_t*>(m + L.tlasChild);
@@ -422,7 +505,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   a.tlasCounter = reinterpret_cast<uint32_t*>(m + L.tlasCounter);
   a.tlasRoot = reinterpret_cast<uint32_t*>(m + L.tlasRoot);
 
-  const uint32_t grid = std::min<uint32_t>((capacity + 255) / 256, (uint32_t)num_cu_public() * 8);
+  const uint32_t grid = std::min<uint32_t>((capacity + 255) / 256, (uint32_t)num_cu_public() * tuning_snapshot().bvhBlocksPerCU);
   // BLAS: keys, sort by (instance, Morton), hierarchy, refit
   hipLaunchKernelGGL(k_blas_keys, dim3(grid), dim3(256), 0, st, a);
   uint32_t instBits = 1;
@@ -432,7 +515,9 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   a.sortedKeys = sb.keys[res];
   a.sortedVals = sb.vals[res];
   hipLaunchKernelGGL(k_blas_hierarchy, dim3(grid), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_blas_refit, dim3(grid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_blas_refit, dim3(std::min<uint32_t>((capacity + kRefitChunk - 1) / kRefitChunk, (uint32_t)num_cu_public() * 4)),
+                     dim3(kRefitThreads), 0, st, a);
+  hipLaunchKernelGGL(k_blas_collapse4, dim3(grid), dim3(256), 0, st, a);
   // TLAS over instances; its sort reuses the other ping-pong slot
   a.tlasKeys = sb.keys[res ^ 1];
   a.tlasVals = sb.vals[res ^ 1];
@@ -456,6 +541,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   const uint32_t tgrid = (a.numInstances + 255) / 256;
   hipLaunchKernelGGL(k_tlas_hierarchy, dim3(tgrid), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_tlas_refit, dim3(tgrid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_tlas_collapse4, dim3(tgrid), dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 

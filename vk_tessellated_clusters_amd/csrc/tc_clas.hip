@@ -112,7 +112,7 @@ extern "C" tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config,
   KArgs a;
   tc_status s = make_args_public(b, config, a);
   if (s != TC_OK) return s;
-  uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * 32);
+  uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
   hipLaunchKernelGGL(k_clas_instantiate, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }

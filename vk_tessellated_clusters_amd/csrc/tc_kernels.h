@@ -49,6 +49,7 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t numInstances);
 
 tc_status make_args_public(const tc_bindings* b, const tc_config& cfg, KArgs& a);
 int num_cu_public();
+tc_tuning tuning_snapshot();
 
 __global__ void k_prologue(KArgs a, uint32_t stage);
 __global__ void k_build_setup(KArgs a, int mode);

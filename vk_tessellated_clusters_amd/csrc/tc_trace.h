@@ -11,9 +11,13 @@ struct TraceArgs {
   const tc_scene_building* build;
   const tc_tlas_instance* tlasInstances;
   const tc_bvh_node* tlasNodes;
+  const tc_bvh4_node* tlasNodes4;
+  const tc_bvh4_node* blasNodes4;
+  uint64_t blasNodesBase;  // tc_tlas_instance.blasNodes - this = first node of the instance (x 64 B)
   const uint32_t* tlasRoot;
   const tc_ray* rays;
   uint32_t width, height, numRays, flags;
+  uint32_t numItems;  // camera rays: whole 8 x 8 tiles
   tc_hit* hits;
   tc_hit_decoded* decoded;
   uint32_t* hitVertices;

@@ -3,9 +3,10 @@
 // Stands for vkCmdTraceRaysKHR with render_raytrace.rgen.glsl:52-86 (rays),
 // the hardware traversal of the cluster BVH, render_raytrace_clusters.rchit.
 // glsl:128-236, 338-339 (decode, depth) and render_shading.glsl:263-274
-// (shadow rays).  One ray per lane; a per-lane LDS stack holds TLAS and BLAS
-// entries (bit 30 marks BLAS entries; BLAS entries above a TLAS entry belong
-// to the instance entered last).  Triangles use the watertight test of Woop,
+// (shadow rays).  One ray per lane over the four-wide views of the TLAS and
+// the BLAS; the lane's stack (LDS, overflowing into private memory) holds
+// TLAS and BLAS entries (bit 30 marks BLAS entries; BLAS entries above a
+// TLAS entry belong to the instance entered last).  Triangles use the watertight test of Woop,
 // Benthin and Wald (JCGT 2013) with an FP64 recomputation of zero edge
 // functions; closest-hit ties are broken by (instance, cluster, primitive)
 // so the result does not depend on traversal order.
@@ -15,8 +16,6 @@
 
 namespace tc {
 
-constexpr uint32_t kTraceThreads = 128;
-constexpr uint32_t kStackDepth = 64;
 constexpr uint32_t kBlasEntry = 1u << 30;
 
 struct WoopRay {
@@ -150,10 +149,63 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
   return false;
 }
 
-// Full two-level traversal of one ray.  `stack` is this lane's LDS stack
-// (stride kTraceThreads).  Returns true if anything was hit.
+// Traversal stack of one lane: the first kLdsStack entries in LDS (stride
+// T, the workgroup size), the rest in a private array.  Bound: every visited node
+// pushes at most 3 of its 4 slots, and the four-wide view visits every
+// other level, so a ray holds at most 3 * (ceil(dT / 2) + ceil(dB / 2))
+// entries for binary depths dT (TLAS) and dB (BLAS).  Karras trees over
+// 30-bit Morton codes with index tie-breaks have depth <= 30 + log2(size):
+// dT <= 46 (2^16 instances) and dB <= 52 (2^22 references), i.e. at most
+// 3 * (23 + 26) = 147 < kStackDepth entries.
+constexpr uint32_t kLdsStack = 24;
+constexpr uint32_t kStackDepth = 160;
+
+template <uint32_t T>
+struct RayStack {
+  uint32_t* lds;
+  uint32_t ldsDepth;  // kLdsStack, or 0 with TC_TRACE_PRIVATE_STACK
+  uint32_t sp;
+  uint32_t spill[kStackDepth];
+  __device__ __forceinline__ void push(uint32_t e) {
+    if (sp < ldsDepth)
+      lds[sp * T] = e;
+    else if (sp < kStackDepth)
+      spill[sp - ldsDepth] = e;
+    sp++;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    --sp;
+    return sp < ldsDepth ? lds[sp * T] : spill[sp - ldsDepth];
+  }
+};
+
+__device__ __forceinline__ void load_node4(const tc_bvh4_node* p, float lo[3][4], float hi[3][4], uint32_t c[4]) {
+  const float4* q = reinterpret_cast<const float4*>(p);
+  float4 v[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) v[i] = q[i];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    lo[a][0] = v[a].x, lo[a][1] = v[a].y, lo[a][2] = v[a].z, lo[a][3] = v[a].w;
+    hi[a][0] = v[3 + a].x, hi[a][1] = v[3 + a].y, hi[a][2] = v[3 + a].z, hi[a][3] = v[3 + a].w;
+  }
+  c[0] = __float_as_uint(v[6].x), c[1] = __float_as_uint(v[6].y), c[2] = __float_as_uint(v[6].z), c[3] = __float_as_uint(v[6].w);
+}
+
+__device__ __forceinline__ void cswap(float t[4], uint32_t c[4], int i, int j) {
+  if (t[j] < t[i]) {
+    const float tt = t[i];
+    t[i] = t[j], t[j] = tt;
+    const uint32_t cc = c[i];
+    c[i] = c[j], c[j] = cc;
+  }
+}
+
+// Full two-level traversal of one ray over the four-wide views.  Returns
+// true if anything was hit.
+template <uint32_t T>
 __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3], float tmin, float tmax, bool cullBack,
-                          bool anyHit, uint32_t* stack, HitState& best) {
+                          bool anyHit, RayStack<T>& stack, HitState& best) {
   best.h.t = tmax;
   best.h.u = best.h.v = 0.0f;
   best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
@@ -162,13 +214,14 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
   WoopRay world, ray;
   setup_ray(world, o, d);
   ray = world;
-  stack[0] = *a.tlasRoot;  // TLAS entry
-  uint32_t sp = 1;
+  stack.sp = 0;
+  uint32_t e = *a.tlasRoot;
   uint32_t inst = 0xFFFFFFFFu;
-  const tc_bvh_node* blasNodes = nullptr;
+  const tc_bvh4_node* blas4 = nullptr;
   const uint64_t* blasLeaves = nullptr;
-  while (sp > 0) {
-    const uint32_t e = stack[--sp * kTraceThreads];
+  while (true) {
+    const tc_bvh4_node* node;
+    uint32_t tag;
     if (!(e & kBlasEntry)) {
       if (inst != 0xFFFFFFFFu) {
         ray = world;
@@ -176,52 +229,60 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
       }
       if (e & TC_BVH_LEAF) {
         const uint32_t i = e & 0x3FFFFFFFu;
-        const tc_tlas_instance ti = a.tlasInstances[i];
-        if (ti.numClusters == 0) continue;
-        float ol[3], dl[3];
-        xform34(ti.worldToObject, o, 1.0f, ol);
-        xform34(ti.worldToObject, d, 0.0f, dl);
-        setup_ray(ray, ol, dl);
-        inst = i;
-        blasNodes = reinterpret_cast<const tc_bvh_node*>(ti.blasNodes);
-        blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
-        if (sp < kStackDepth) stack[sp++ * kTraceThreads] = kBlasEntry | ti.blasRoot;
+        const tc_tlas_instance& ti = a.tlasInstances[i];
+        const uint32_t nc = ti.numClusters;
+        if (nc != 0) {
+          float ol[3], dl[3];
+          xform34(ti.worldToObject, o, 1.0f, ol);
+          xform34(ti.worldToObject, d, 0.0f, dl);
+          setup_ray(ray, ol, dl);
+          inst = i;
+          blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
+          blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+          e = kBlasEntry | ti.blasRoot;
+          continue;
+        }
+        if (stack.sp == 0) break;
+        e = stack.pop();
         continue;
       }
-      const tc_bvh_node n = a.tlasNodes[e];
-      const float t0 = box_hit(ray, n.lo0, n.hi0, tmin, best.h.t);
-      const float t1 = box_hit(ray, n.lo1, n.hi1, tmin, best.h.t);
-      const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY;
-      if (h0 && h1) {
-        const bool near0 = t0 <= t1;
-        if (sp + 1 < kStackDepth) {
-          stack[sp++ * kTraceThreads] = near0 ? n.child1 : n.child0;
-          stack[sp++ * kTraceThreads] = near0 ? n.child0 : n.child1;
-        }
-      } else if (h0 || h1) {
-        if (sp < kStackDepth) stack[sp++ * kTraceThreads] = h0 ? n.child0 : n.child1;
+      node = a.tlasNodes4 + e;
+      tag = 0;
+    } else {
+      const uint32_t be = e & ~kBlasEntry;
+      if (be & TC_BVH_LEAF) {
+        const char* clas = reinterpret_cast<const char*>(blasLeaves[be & ~TC_BVH_LEAF]);
+        if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
+        if (stack.sp == 0) break;
+        e = stack.pop();
+        continue;
       }
+      node = blas4 + be;
+      tag = kBlasEntry;
+    }
+    float lo[3][4], hi[3][4], t[4];
+    uint32_t c[4];
+    load_node4(node, lo, hi, c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
+      t[k] = c[k] == TC_BVH4_EMPTY ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
+      c[k] |= tag;
+    }
+    cswap(t, c, 0, 1);
+    cswap(t, c, 2, 3);
+    cswap(t, c, 0, 2);
+    cswap(t, c, 1, 3);
+    cswap(t, c, 1, 2);
+    if (t[0] == INFINITY) {
+      if (stack.sp == 0) break;
+      e = stack.pop();
       continue;
     }
-    const uint32_t be = e & ~kBlasEntry;
-    if (be & TC_BVH_LEAF) {
-      const char* clas = reinterpret_cast<const char*>(blasLeaves[be & ~TC_BVH_LEAF]);
-      if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
-      continue;
-    }
-    const tc_bvh_node n = blasNodes[be];
-    const float t0 = box_hit(ray, n.lo0, n.hi0, tmin, best.h.t);
-    const float t1 = box_hit(ray, n.lo1, n.hi1, tmin, best.h.t);
-    const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY;
-    if (h0 && h1) {
-      const bool near0 = t0 <= t1;
-      if (sp + 1 < kStackDepth) {
-        stack[sp++ * kTraceThreads] = kBlasEntry | (near0 ? n.child1 : n.child0);
-        stack[sp++ * kTraceThreads] = kBlasEntry | (near0 ? n.child0 : n.child1);
-      }
-    } else if (h0 || h1) {
-      if (sp < kStackDepth) stack[sp++ * kTraceThreads] = kBlasEntry | (h0 ? n.child0 : n.child1);
-    }
+    if (t[3] != INFINITY) stack.push(c[3]);
+    if (t[2] != INFINITY) stack.push(c[2]);
+    if (t[1] != INFINITY) stack.push(c[1]);
+    e = c[0];
   }
   return best.h.instanceID != 0xFFFFFFFFu;
 }
@@ -309,20 +370,30 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   out.depth = pp[2] / pp[3];
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
-  __shared__ uint32_t s_stack[kStackDepth * kTraceThreads];
-  uint32_t* stack = s_stack + threadIdx.x;
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_trace(TraceArgs a) {
+  __shared__ uint32_t s_stack[kLdsStack * T];
+  RayStack<T> stack;
+  stack.lds = s_stack + threadIdx.x;
+  stack.ldsDepth = (a.flags & TC_TRACE_PRIVATE_STACK) ? 0u : kLdsStack;
   const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
   const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
-  for (uint32_t r = blockIdx.x * kTraceThreads + threadIdx.x; r < a.numRays; r += gridDim.x * kTraceThreads) {
+  // camera rays walk the image in 8 x 8 pixel tiles (one tile per wave)
+  const uint32_t tilesX = (a.width + 7) / 8;
+  for (uint32_t item = blockIdx.x * T + threadIdx.x; item < a.numItems; item += gridDim.x * T) {
     float o[3], d[3], tmin, tmax;
+    uint32_t r = item;
     if (a.rays) {
       const tc_ray ray = a.rays[r];
       for (int k = 0; k < 3; k++) o[k] = ray.origin[k], d[k] = ray.direction[k];
       tmin = ray.tMin;
       tmax = ray.tMax;
     } else {
-      primary_ray(a.view, r % a.width, r / a.width, a.width, a.height, o, d);
+      const uint32_t tile = item >> 6, in = item & 63;
+      const uint32_t x = (tile % tilesX) * 8 + (in & 7), y = (tile / tilesX) * 8 + (in >> 3);
+      if (x >= a.width || y >= a.height) continue;
+      r = y * a.width + x;
+      primary_ray(a.view, x, y, a.width, a.height, o, d);
       tmin = a.view->nearPlane;
       tmax = a.view->farPlane;
     }
@@ -374,13 +445,24 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   a.width = p->width;
   a.height = p->height;
   a.numRays = p->rays ? p->numRays : p->width * p->height;
+  a.numItems = p->rays ? p->numRays : ((p->width + 7) / 8) * ((p->height + 7) / 8) * 64;
+  a.tlasNodes4 = accel->tlasNodes4;
+  a.blasNodes4 = accel->blasNodes4;
+  a.blasNodesBase = reinterpret_cast<uint64_t>(accel->blasNodes);
   a.hits = p->hits;
   a.decoded = p->decoded;
   a.hitVertices = p->hitVertices;
   a.flags = p->flags;
   if (!p->rays && (!p->width || !p->height)) return TC_ERROR_INVALID_ARGUMENT;
   if (a.numRays == 0) return TC_OK;
-  const uint32_t grid = std::min<uint32_t>((a.numRays + kTraceThreads - 1) / kTraceThreads, (uint32_t)num_cu_public() * 16);
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kTraceThreads), 0, (hipStream_t)stream, a);
+  const tc_tuning tn = tuning_snapshot();
+  const uint32_t T = tn.traceThreads;
+  const uint32_t grid = std::min<uint32_t>((a.numItems + T - 1) / T, (uint32_t)num_cu_public() * tn.traceBlocksPerCU);
+  if (T == 64)
+    hipLaunchKernelGGL(k_trace<64>, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
+  else if (T == 256)
+    hipLaunchKernelGGL(k_trace<256>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(k_trace<128>, dim3(grid), dim3(128), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
