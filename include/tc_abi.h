@@ -162,6 +162,26 @@ typedef struct tc_template_instantiate_info {
   uint64_t vertexBufferStride;
 } tc_template_instantiate_info;
 
+/* shaderio_building.h:95-126 (ClasBuildInfo): one transient CLAS build.
+ * packed = triangleCount[8:0] | vertexCount[17:9] |
+ * positionTruncateBitCount[23:18] | indexType[27:24] (1: u8). */
+typedef struct tc_clas_build_info {
+  uint32_t clusterID;
+  uint32_t clusterFlags;
+  uint32_t packed;
+  uint32_t baseGeometryIndexAndFlags;
+  uint16_t indexBufferStride;
+  uint16_t vertexBufferStride;
+  uint16_t geometryIndexAndFlagsBufferStride;
+  uint16_t opacityMicromapIndexBufferStride;
+  uint64_t indexBuffer;
+  uint64_t vertexBuffer;
+  uint64_t geometryIndexAndFlagsBuffer;
+  uint64_t opacityMicromapArray;
+  uint64_t opacityMicromapIndexBuffer;
+} tc_clas_build_info;
+#define TC_CLAS_GEOMETRY_OPAQUE (4u << 29) /* ClasGeometryFlag_OPAQUE_BIT_NV */
+
 /* shaderio_building.h:141-149 */
 typedef struct tc_blas_build_info {
   uint32_t clusterReferencesCount;
@@ -627,6 +647,12 @@ tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t n
  * tempClusterSizes[i]. */
 tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config, tc_stream stream);
 
+/* Our CLAS build from transient build records (replaces the NV transient
+ * CLAS build, renderer_raytrace_clusters_tess.cpp:625-655; with
+ * TC_FLAG_1X_TRANSIENT, else a no-op): one block per transBuilds entry at
+ * transClusterAddresses[i]; writes transClusterSizes[i]. */
+tc_status tc_clas_build_transient(const tc_bindings* b, tc_config config, tc_stream stream);
+
 /* ---- cluster acceleration structures (replace the NV BLAS build and the
  *      TLAS build, renderer_raytrace_clusters_tess.cpp:695-745) ---- */
 
@@ -793,6 +819,7 @@ static_assert(offsetof(tc_readback, numGenDatas) == 48, "Readback layout");
 static_assert(sizeof(tc_clas_header) == 48, "clas header");
 static_assert(sizeof(tc_bvh_node) == 64, "bvh node");
 static_assert(sizeof(tc_bvh4_node) == 128, "bvh4 node");
+static_assert(sizeof(tc_clas_build_info) == 64, "ClasBuildInfo");
 static_assert(sizeof(tc_clas_template) == 32, "clas template");
 static_assert(sizeof(tc_vertex_tape) == 48, "vertex tape");
 static_assert(sizeof(tc_bindings) == 80, "bindings");

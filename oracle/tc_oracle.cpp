@@ -299,14 +299,35 @@ inline uint64_t gen_vertices_value(const oracle_ctx* c) {
 /* --------------------------------------------------- cluster_classify */
 
 /* Allocation counters the stage advances; the serial run uses the live
- * SceneBuilding values, the threaded run starts each unit at its prefix. */
+ * SceneBuilding values, the threaded run starts each unit at its prefix.
+ * partY is the upper half of the dual part counter (build.glsl:25-83):
+ * part-list slots taken from the top of the list by 1X transient
+ * clusters for their triangle mappings. */
 struct ClassifyCounters {
   uint32_t split, part, genCluster, genVertex;
   uint64_t genData;
+  uint32_t partY;
 };
 
-/* cluster_classify.comp.glsl:150-666 for one visible cluster (template-only
- * build: 1X/2X transient builds off, no animation).  WRITE=false only
+/* build.glsl:54-83 per item, in the canonical order: an allocation of n
+ * part slots is accepted iff x + y + n + 1 <= MAX_PART_TRIANGLES with x and
+ * y the counters before it (the reference decides per subgroup batch). */
+inline bool dual_part_ok(uint64_t x, uint64_t y, uint64_t n, uint32_t maxParts) { return x + y + n + 1 <= maxParts; }
+
+/* 1X transient cluster data: the exact size of our CLAS block for the
+ * cluster's simple triangles and all its vertices (the reference reserves
+ * basicClusterSizes[simpleCount], a worst case for NV's opaque format). */
+inline uint32_t clas_size(uint32_t nt, uint32_t nv) {
+  auto r16 = [](uint32_t v) { return (v + 15u) & ~15u; };
+  uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+  return (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+}
+
+/* cluster_classify.comp.glsl:150-666 for one visible cluster (2X transient
+ * builds off, no animation).  With TC_FLAG_1X_TRANSIENT a cluster with more
+ * than one but not all triangles simple (:261-276) gets a transient CLAS
+ * over its simple triangles (:314-447, :489-535) and only its other
+ * triangles are split or instantiated (:576-584).  WRITE=false only
  * advances the counters. */
 template <bool WRITE>
 void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, ClassifyCounters& k) {
@@ -317,6 +338,7 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
   uint32_t state = P<const uint32_t>(b->instanceStates)[ci.instanceID];
   uint32_t nv = cl.numVertices, nt = cl.numTriangles;
   const uint8_t* lt = P<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+  const bool flag1x = (cfg.flags & TC_FLAG_1X_TRANSIENT) != 0;
 
   /* :177-195 */
   v3 sv[256];
@@ -328,6 +350,7 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
   bool invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
   if (invisible) {
     simple = nt;
+    for (uint32_t t = 0; t < nt; t++) fac[t][0] = fac[t][1] = fac[t][2] = 1;
   } else {
     v3 eye = mk(b->viewPos[0], b->viewPos[1], b->viewPos[2]);
     for (uint32_t t = 0; t < nt; t++) {
@@ -339,56 +362,105 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
       if (amax3(fac[t][0], fac[t][1], fac[t][2]) == 1) simple++;
     }
   }
+  const bool full = simple == nt;
+  const bool trans = flag1x && !full && simple > 1; /* transientSimpleThreshold = 1 */
 
-  if (simple == nt) {
-    /* :300-445 full cluster -> template instantiation */
+  if (full || trans) {
+    /* :300-447 one allocation per cluster */
     uint32_t genOffset = k.genCluster++;
-    uint64_t dataSize = P<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID];
+    uint64_t dataSize = full ? P<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID] : clas_size(simple, nv);
+    uint32_t vertexSize = full ? nv : nv + (simple * 3 + 11) / 12;
+    uint32_t partSize = full ? 0 : (8 + simple + 24 - 1) / 24;
+    uint32_t partOffset = 0;
+    if (trans) {
+      partOffset = dual_part_ok(k.part, k.partY, partSize, cfg.maxPartTriangles) ? cfg.maxPartTriangles - k.partY - partSize
+                                                                                 : cfg.maxPartTriangles;
+      k.partY += partSize;
+    }
     uint64_t dataOffset = k.genData;
     k.genData += dataSize;
     uint32_t vertexOffset = k.genVertex;
-    k.genVertex += nv;
-    if (!WRITE) return;
-    fadd(&c->readback->numFullClusters, 1u);
-    bool ok = !(uint64_t(vertexOffset) + nv > cfg.maxGeneratedVertices || uint64_t(genOffset) + 1 > cfg.maxGeneratedClusters ||
-                dataOffset + dataSize > max_data_bytes(cfg));
-    if (ok) {
-      uint32_t tempOffset = fadd(&b->tempInstantiateCounter, 1u);
-      tc_template_instantiate_info ti;
-      ti.clusterIdOffset = 0;
-      ti.geometryIndexOffset = 0;
-      ti.clusterTemplateAddress = P<const uint64_t>(inst.clusterTemplateAdresses)[ci.clusterID];
-      ti.vertexBufferAddress = gen_vertices_value(c) + uint64_t(vertexOffset) * 12ull;
-      ti.vertexBufferStride = 12;
-      P<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
-      P<uint32_t>(b->tempInstanceIDs)[tempOffset] = ci.instanceID;
-      P<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
-      fadd(&P<tc_blas_build_info>(b->blasBuildInfos)[ci.instanceID].clusterReferencesCount, 1u);
-      fadd(&c->readback->numTotalTriangles, simple);
-      /* :459-488 displaced cluster vertices */
-      bool disp = has_disp(c, cfg, inst);
-      for (uint32_t v = 0; v < nv; v++) {
-        v3 p = sv[v];
-        if (disp) {
-          uint32_t vi = cl.firstLocalVertex + v;
-          p = displace(c, inst, p, ld3(inst.normals, vi), ld2(inst.texcoords, vi));
+    k.genVertex += vertexSize;
+    if (WRITE) {
+      if (full) fadd(&c->readback->numFullClusters, 1u);
+      bool ok = !(uint64_t(vertexOffset) + vertexSize > cfg.maxGeneratedVertices ||
+                  uint64_t(genOffset) + 1 > cfg.maxGeneratedClusters || dataOffset + dataSize > max_data_bytes(cfg) ||
+                  (trans && uint64_t(partOffset) + partSize > cfg.maxPartTriangles));
+      if (ok) {
+        const uint64_t vbuf = gen_vertices_value(c) + uint64_t(vertexOffset) * 12ull;
+        if (full) {
+          uint32_t tempOffset = fadd(&b->tempInstantiateCounter, 1u);
+          tc_template_instantiate_info ti;
+          ti.clusterIdOffset = 0;
+          ti.geometryIndexOffset = 0;
+          ti.clusterTemplateAddress = P<const uint64_t>(inst.clusterTemplateAdresses)[ci.clusterID];
+          ti.vertexBufferAddress = vbuf;
+          ti.vertexBufferStride = 12;
+          P<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
+          P<uint32_t>(b->tempInstanceIDs)[tempOffset] = ci.instanceID;
+          P<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
+        } else {
+          /* :395-436 transient build record */
+          uint32_t transOffset = fadd(&b->transBuildCounter, 1u);
+          tc_clas_build_info bi;
+          memset(&bi, 0, sizeof(bi));
+          bi.clusterID = (TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER << 30) | partOffset;
+          bi.packed = simple | (nv << 9) | ((b->positionTruncateBitCount & 63u) << 18) | (1u << 24);
+          bi.baseGeometryIndexAndFlags = TC_CLAS_GEOMETRY_OPAQUE;
+          bi.indexBufferStride = 1;
+          bi.vertexBufferStride = 12;
+          bi.vertexBuffer = vbuf;
+          bi.indexBuffer = vbuf + uint64_t(nv) * 12ull;
+          P<tc_clas_build_info>(b->transBuilds)[transOffset] = bi;
+          P<uint32_t>(b->transInstanceIDs)[transOffset] = ci.instanceID;
+          P<uint64_t>(b->transClusterAddresses)[transOffset] = b->genClusterData + dataOffset;
+          P<tc_tess_triangle_info>(b->partTriangles)[partOffset].cluster = ci;
         }
-        float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + v);
-        o[0] = p.x, o[1] = p.y, o[2] = p.z;
+        fadd(&P<tc_blas_build_info>(b->blasBuildInfos)[ci.instanceID].clusterReferencesCount, 1u);
+        fadd(&c->readback->numTotalTriangles, simple);
+        /* :459-488 displaced cluster vertices */
+        bool disp = has_disp(c, cfg, inst);
+        for (uint32_t v = 0; v < nv; v++) {
+          v3 p = sv[v];
+          if (disp) {
+            uint32_t vi = cl.firstLocalVertex + v;
+            p = displace(c, inst, p, ld3(inst.normals, vi), ld2(inst.texcoords, vi));
+          }
+          float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + v);
+          o[0] = p.x, o[1] = p.y, o[2] = p.z;
+        }
+        if (trans) {
+          /* :497-534 triangle mapping bytes after the ClusterInfo of the
+           * part slot, u8 indices after the vertices */
+          uint8_t* map = P<uint8_t>(b->partTriangles) + uint64_t(partOffset) * 24ull + 8ull;
+          uint8_t* idx = P<uint8_t>(b->genVertices) + (uint64_t(vertexOffset) + nv) * 12ull;
+          uint32_t o = 0;
+          for (uint32_t t = 0; t < nt; t++) {
+            if (amax3(fac[t][0], fac[t][1], fac[t][2]) != 1) continue;
+            map[o] = (uint8_t)t;
+            for (int j = 0; j < 3; j++) idx[o * 3 + j] = lt[t * 3 + j];
+            o++;
+          }
+          /* zero the rest of the index slots so the vertex stream is
+           * deterministic */
+          for (uint32_t j = simple * 3; j < (vertexSize - nv) * 12; j++) idx[j] = 0;
+        }
       }
     }
-    return;
+    if (full) return;
   }
 
   /* :543-666 per-triangle split / part emission */
   for (uint32_t t = 0; t < nt; t++) {
     uint32_t f[3] = {fac[t][0], fac[t][1], fac[t][2]};
     uint32_t m = amax3(f[0], f[1], f[2]);
+    if (trans && m == 1) continue; /* :576-584 in the transient CLAS */
     bool needSplit = m > TC_TESSTABLE_SIZE;
     bool needPart = m <= TC_TESSTABLE_SIZE;
     uint32_t offSplit = needSplit ? k.split++ : 0;
     uint32_t offPart = needPart ? k.part++ : 0;
     if (!WRITE) continue;
+    bool partOk = flag1x ? dual_part_ok(offPart, k.partY, 1, cfg.maxPartTriangles) : offPart < cfg.maxPartTriangles;
     tc_tess_triangle_info ti;
     ti.cluster = ci;
     ti.subTriangle.vtxEncoded[0] = 0;
@@ -400,7 +472,7 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
       uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
       ti.subTriangle.triangleID_config |= cf << 16;
       P<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-    } else if (needPart && offPart < cfg.maxPartTriangles) {
+    } else if (needPart && partOk) {
       uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
       ti.subTriangle.triangleID_config |= cf << 16;
       P<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
@@ -461,9 +533,14 @@ void split_item(const oracle_ctx* c, tc_config cfg, uint32_t itemIndex, SplitCou
       for (int i = 0; i < 3; i++) f[i] = split_factor(f[i], cfg.maxSplitFactor);
     uint32_t cf = get_config(f, ti.subTriangle.vtxEncoded);
     ti.subTriangle.triangleID_config = (ti.subTriangle.triangleID_config & 0xFFFFu) | (cf << 16);
+    /* with transient builds the part list is shared with their slots at
+     * its top (build.glsl:68-83) */
+    bool partOk = (cfg.flags & TC_FLAG_1X_TRANSIENT)
+                      ? dual_part_ok(offPart, uint32_t(b->dualPartTriangleCounter >> 32), 1, cfg.maxPartTriangles)
+                      : offPart < cfg.maxPartTriangles;
     if (needSplit && offSplit < cfg.maxSplitTriangles)
       P<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-    else if (needPart && offPart < cfg.maxPartTriangles)
+    else if (needPart && partOk)
       P<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
   }
 }
@@ -549,7 +626,7 @@ void classify_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
   tc_scene_building* b = c->build;
   uint32_t n = b->visibleClusterCounter; /* clamped by build_setup CLASSIFY */
   ClassifyCounters base{(uint32_t)b->splitTriangleCounter, b->partTriangleCounter, b->genClusterCounter, b->genVertexCounter,
-                        b->genClusterDataCounter};
+                        b->genClusterDataCounter, uint32_t(b->dualPartTriangleCounter >> 32)};
   if (nthreads <= 1) {
     ClassifyCounters k = base;
     for (uint32_t i = 0; i < n; i++) classify_cluster<true>(c, cfg, i, k);
@@ -557,7 +634,7 @@ void classify_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
   } else {
     std::vector<ClassifyCounters> cnt(n);
     parallel_for(n, nthreads, [&](uint32_t i) {
-      ClassifyCounters k{0, 0, 0, 0, 0};
+      ClassifyCounters k{0, 0, 0, 0, 0, 0};
       classify_cluster<false>(c, cfg, i, k);
       cnt[i] = k;
     });
@@ -566,11 +643,11 @@ void classify_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
       ClassifyCounters d = cnt[i];
       cnt[i] = run;
       run.split += d.split, run.part += d.part, run.genCluster += d.genCluster, run.genVertex += d.genVertex,
-          run.genData += d.genData;
+          run.genData += d.genData, run.partY += d.partY;
     }
-    /* the full-cluster template records are appended with a counter whose
-     * order must follow cluster order: emit them serially, the rest in
-     * parallel.  Full clusters only touch tempInstantiateCounter. */
+    /* the template / transient records of clusters with an allocation are
+     * appended with counters whose order must follow cluster order: emit
+     * those clusters serially, the rest in parallel. */
     std::vector<uint8_t> isFull(n);
     for (uint32_t i = 0; i < n; i++) isFull[i] = (i + 1 < n ? cnt[i + 1].genCluster : run.genCluster) != cnt[i].genCluster;
     parallel_for(n, nthreads, [&](uint32_t i) {
@@ -590,6 +667,7 @@ void classify_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
   b->genClusterCounter = base.genCluster;
   b->genVertexCounter = base.genVertex;
   b->genClusterDataCounter = base.genData;
+  b->dualPartTriangleCounter = (uint64_t(base.partY) << 32) | base.part;
 }
 
 void split_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
@@ -622,6 +700,7 @@ void split_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
   }
   b->splitWriteCounter = base.split;
   b->partTriangleCounter = base.part;
+  b->dualPartTriangleCounter = (b->dualPartTriangleCounter & ~0xFFFFFFFFull) | base.part;
 }
 
 void instantiate_all(const oracle_ctx* c, tc_config cfg, uint32_t nthreads) {
@@ -837,8 +916,11 @@ void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode) {
     uint32_t count = end - start;
     b->dispatchTriangleSplit = tc_dispatch_indirect{(count + 63) / 64, 1, 1};
   } else if (mode == TC_BUILD_SETUP_BUILD_BLAS) {
+    const bool trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) != 0;
     uint32_t temp = b->tempInstantiateCounter;
-    rb->numBlasClusters = temp;
+    uint32_t tr = trans ? b->transBuildCounter : 0u;
+    rb->numBlasClusters = temp + tr;
+    if (trans) rb->numTransBuilds = tr;
     rb->numTempInstantiations = temp;
     rb->numGenDatas = b->genClusterDataCounter;
     rb->numGenVertices = b->genVertexCounter;
@@ -847,11 +929,27 @@ void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode) {
     b->tempInstantiateCounter = temp;
     rb->numActualTempInstantiations = temp;
     b->dispatchBlasTempInsert = tc_dispatch_indirect{(temp + 63) / 64, 1, 1};
+    if (trans) {
+      tr = std::min(cfg.maxGeneratedClusters, temp + tr) - temp;
+      b->transBuildCounter = tr;
+      rb->numActualTransBuilds = tr;
+      b->dispatchBlasTransInsert = tc_dispatch_indirect{(tr + 63) / 64, 1, 1};
+    }
   } else if (mode == TC_BUILD_SETUP_INSTANTIATE_TESS) {
     uint32_t part = b->partTriangleCounter;
-    rb->numPartTriangles = part;
     rb->numSplitTriangles = b->splitWriteCounter;
-    part = std::min(part, cfg.maxPartTriangles);
+    if (cfg.flags & TC_FLAG_1X_TRANSIENT) {
+      /* :240-253; the instantiated parts are the accepted prefix
+       * x + y + 2 <= MAX_PART_TRIANGLES of the lower counter */
+      uint32_t y = uint32_t(b->dualPartTriangleCounter >> 32);
+      rb->numPartTriangles = part + y;
+      rb->numTransPartTriangles = y;
+      uint32_t cap = cfg.maxPartTriangles > y + 1 ? cfg.maxPartTriangles - y - 1 : 0u;
+      part = std::min(part, cap);
+    } else {
+      rb->numPartTriangles = part;
+      part = std::min(part, cfg.maxPartTriangles);
+    }
     b->partTriangleCounter = part;
     b->dispatchTriangleInstantiate = tc_dispatch_indirect{(part + 31) / 32, 1, 1};
   }
@@ -878,14 +976,17 @@ void oracle_blas_setup_insertion(const oracle_ctx* c, tc_config) {
 /* blas_clusters_insert.comp.glsl:97-135 in CLAS order.  clusterReferences
  * holds the address of the per-instance list in the build's address space;
  * blasClusterAddresses is the host pointer of the same array. */
-void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config, int transient) {
-  if (transient) return;
+void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config cfg, int transient) {
+  if (transient && !(cfg.flags & TC_FLAG_1X_TRANSIENT)) return;
   tc_scene_building* b = c->build;
-  uint32_t n = b->tempInstantiateCounter;
+  uint32_t n = transient ? b->transBuildCounter : b->tempInstantiateCounter;
+  const uint32_t* ids = P<const uint32_t>(transient ? b->transInstanceIDs : b->tempInstanceIDs);
+  const uint64_t* addrs = P<const uint64_t>(transient ? b->transClusterAddresses : b->tempClusterAddresses);
+  const uint32_t* sizes = P<const uint32_t>(transient ? b->transClusterSizes : b->tempClusterSizes);
   for (uint32_t i = 0; i < n; i++) {
-    uint32_t inst = P<const uint32_t>(b->tempInstanceIDs)[i];
-    uint64_t addr = P<const uint64_t>(b->tempClusterAddresses)[i];
-    uint32_t size = P<const uint32_t>(b->tempClusterSizes)[i];
+    uint32_t inst = ids[i];
+    uint64_t addr = addrs[i];
+    uint32_t size = sizes[i];
     tc_blas_build_info& bi = P<tc_blas_build_info>(b->blasBuildInfos)[inst];
     uint32_t idx = bi.clusterReferencesCount++;
     P<uint64_t>(bi.clusterReferences)[idx] = addr;
@@ -936,6 +1037,58 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
     h._pad[0] = h._pad[1] = h._pad[2] = 0;
     memcpy(dst, &h, sizeof(h));
     P<uint32_t>(b->tempClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+  }
+}
+
+/* Our CLAS build from transient build records (stands for the NV
+ * transient CLAS build, renderer_raytrace_clusters_tess.cpp:625-655):
+ * triangles in record order (primitive = transient triangle index), group
+ * boxes of TC_CLAS_GROUP_TRIANGLES triangles, all the record's vertices. */
+void oracle_clas_build_transient(const oracle_ctx* c, tc_config cfg) {
+  tc_scene_building* b = c->build;
+  if (!(cfg.flags & TC_FLAG_1X_TRANSIENT)) return;
+  uint32_t n = std::min(b->transBuildCounter, cfg.maxGeneratedClusters);
+  auto r16 = [](uint32_t v) { return (v + 15u) & ~15u; };
+  for (uint32_t i = 0; i < n; i++) {
+    const tc_clas_build_info& info = P<const tc_clas_build_info>(b->transBuilds)[i];
+    uint32_t nt = info.packed & 511u, nv = (info.packed >> 9) & 511u;
+    uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+    char* dst = P<char>(P<const uint64_t>(b->transClusterAddresses)[i]);
+    tc_aabb* groups = reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header));
+    uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
+    float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
+    const char* vsrc = P<const char>(info.vertexBuffer);
+    const uint8_t* isrc = P<const uint8_t>(info.indexBuffer);
+    for (uint32_t v = 0; v < nv; v++)
+      for (int k = 0; k < 3; k++) verts[v * 3 + k] = reinterpret_cast<const float*>(vsrc + uint64_t(v) * info.vertexBufferStride)[k];
+    for (uint32_t t = 0; t < nt; t++) {
+      const uint8_t* ix = isrc + uint64_t(t) * 3u * info.indexBufferStride;
+      tris[t] = uint32_t(ix[0]) | (uint32_t(ix[info.indexBufferStride]) << 8) | (uint32_t(ix[2u * info.indexBufferStride]) << 16) |
+                (t << 24);
+    }
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t g = 0; g < ng; g++) {
+      tc_aabb box;
+      for (int k = 0; k < 3; k++) box.lo[k] = INFINITY, box.hi[k] = -INFINITY;
+      for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < std::min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES); t++)
+        for (int k = 0; k < 3; k++) {
+          uint32_t vi = (tris[t] >> (8 * k)) & 0xFF;
+          for (int a = 0; a < 3; a++) {
+            box.lo[a] = std::min(box.lo[a], verts[vi * 3 + a]);
+            box.hi[a] = std::max(box.hi[a], verts[vi * 3 + a]);
+          }
+        }
+      groups[g] = box;
+      for (int a = 0; a < 3; a++) lo[a] = std::min(lo[a], box.lo[a]), hi[a] = std::max(hi[a], box.hi[a]);
+    }
+    tc_clas_header h;
+    for (int a = 0; a < 3; a++) h.lo[a] = lo[a], h.hi[a] = hi[a];
+    h.clusterID = info.clusterID;
+    h.counts = nt | (nv << 8) | (ng << 16);
+    h.vertexBase = info.vertexBuffer >= b->genVertices ? (uint32_t)((info.vertexBuffer - b->genVertices) / 12u) : 0xFFFFFFFFu;
+    h._pad[0] = h._pad[1] = h._pad[2] = 0;
+    memcpy(dst, &h, sizeof(h));
+    P<uint32_t>(b->transClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
   }
 }
 
@@ -1107,7 +1260,15 @@ void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded&
   }
   uint32_t mode = h.clusterID >> 30, clusterID = h.clusterID & 0x3FFFFFFFu, triangleID = h.primitiveID;
   v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v), base = bw;
-  if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
+  if (mode == TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER) {
+    /* rchit:173-178: the part slot holds the ClusterInfo, then one byte per
+     * transient triangle naming the cluster triangle */
+    const uint8_t* slot = P<const uint8_t>(c->build->partTriangles) + uint64_t(clusterID) * 24ull;
+    tc_cluster_info ci;
+    memcpy(&ci, slot, sizeof(ci));
+    triangleID = slot[8 + h.primitiveID];
+    clusterID = ci.clusterID;
+  } else if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
     const tc_tess_triangle_info& ti = P<const tc_tess_triangle_info>(c->build->partTriangles)[clusterID];
     triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
     uint32_t cfg = ti.subTriangle.triangleID_config >> 16;

@@ -97,6 +97,9 @@ void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_ray* rays, 
  * separate calls. */
 void oracle_frame_tessellate(const oracle_ctx* c, tc_config cfg, uint32_t numSplitPasses);
 
+/* Our CLAS blocks from the transient build records (TC_FLAG_1X_TRANSIENT). */
+void oracle_clas_build_transient(const oracle_ctx* c, tc_config cfg);
+
 /* Same chain on nthreads threads for the CPU baseline (parallel over
  * clusters / items with a deterministic merge); identical output. */
 void oracle_frame_tessellate_mt(const oracle_ctx* c, tc_config cfg, uint32_t numSplitPasses, uint32_t nthreads);

@@ -36,7 +36,8 @@ def oracle():
         L = C.CDLL(str(ORACLE_LIB))
         ctx = C.POINTER(OracleCtx)
         for n in ("oracle_instances_classify", "oracle_clusters_cull", "oracle_cluster_classify", "oracle_triangle_split",
-                  "oracle_triangle_tess_template_instantiate", "oracle_blas_setup_insertion", "oracle_clas_instantiate"):
+                  "oracle_triangle_tess_template_instantiate", "oracle_blas_setup_insertion", "oracle_clas_instantiate",
+                  "oracle_clas_build_transient"):
             getattr(L, n).argtypes = [ctx, abi.Config]
             getattr(L, n).restype = None
         L.oracle_build_setup.argtypes = [ctx, abi.Config, C.c_int]
@@ -157,6 +158,23 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
     nref = b.blasClusterCounter
     refs = fetch(base["blasClusterAddresses"], 8 * nref).view(np.uint64)
     R.arrays["blasClusterOffsets"] = refs - np.uint64(base["genClusterData"])
+    if cfg.flags & abi.FLAG_1X_TRANSIENT:
+        ntr = min(b.transBuildCounter, cfg.maxGeneratedClusters)
+        tb = fetch(base["transBuilds"], 64 * ntr).view(abi.CLAS_BUILD_INFO_DT).copy()
+        for f in ("vertexBuffer", "indexBuffer"):
+            tb[f] = (tb[f].astype(np.int64) - np.int64(base["genVertices"])).astype(np.uint64)
+        R.arrays["transBuilds"] = tb
+        R.arrays["transInstanceIDs"] = fetch(base["transInstanceIDs"], 4 * ntr).view(np.uint32)
+        ta = fetch(base["transClusterAddresses"], 8 * ntr).view(np.uint64)
+        R.arrays["transClusterOffsets"] = ta - np.uint64(base["genClusterData"])
+        tsz = fetch(base["transClusterSizes"], 4 * ntr).view(np.uint32)
+        R.arrays["transClusterSizes"] = tsz
+        blocks = [fetch(int(base["genClusterData"] + int(o)), int(sz)) for o, sz in zip(R.arrays["transClusterOffsets"], tsz)]
+        R.arrays["transClasBlocks"] = np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
+        # written bytes of every transient part slot: ClusterInfo + mappings
+        slots = [fetch(int(base["partTriangles"] + 24 * (int(r["clusterID"]) & 0x3FFFFFFF)), 8 + (int(r["packed"]) & 511))
+                 for r in tb]
+        R.arrays["transSlots"] = np.concatenate(slots) if slots else np.zeros(0, np.uint8)
     return R
 
 
@@ -244,6 +262,13 @@ class OracleFrame:
         sb.blasBuildInfos = alloc("blasBuildInfos", 16 * n)
         sb.blasBuildSizes = alloc("blasBuildSizes", 4 * n)
         sb.blasClusterAddresses = alloc("blasClusterAddresses", 8 * cfg.maxGeneratedClusters)
+        if cfg.flags & abi.FLAG_1X_TRANSIENT:
+            sb.transBuilds = alloc("transBuilds", 64 * cfg.maxGeneratedClusters)
+            sb.transInstanceIDs = alloc("transInstanceIDs", 4 * cfg.maxGeneratedClusters)
+            sb.transClusterAddresses = alloc("transClusterAddresses", 8 * cfg.maxGeneratedClusters)
+            sb.transClusterSizes = alloc("transClusterSizes", 4 * cfg.maxGeneratedClusters)
+            sb.transTriIndices = sb.genVertices
+            sb.transTriMappings = sb.partTriangles
         self.clas_bytes = min(cfg.maxGeneratedClusterMegs << 20, 1 << 30)
         sb.genClusterData = alloc("genClusterData", self.clas_bytes)
         self.sb = sb
@@ -279,8 +304,10 @@ class OracleFrame:
         if stages == "tess":
             return
         O.oracle_clas_instantiate(ctx, self.cfg)
+        O.oracle_clas_build_transient(ctx, self.cfg)
         O.oracle_blas_setup_insertion(ctx, self.cfg)
         O.oracle_blas_clusters_insert(ctx, self.cfg, 0)
+        O.oracle_blas_clusters_insert(ctx, self.cfg, 1)
 
     def trace(self, width, height, flags, nthreads=8, rays=None):
         xf = np.zeros(24 * len(self.instances), np.float32)
@@ -381,8 +408,11 @@ def compare(gpu: Results, ref: Results, max_ulp: int = 2):
     """Raises AssertionError naming the first differing output."""
     assert gpu.counters == ref.counters, _diff_dict("SceneBuilding", gpu.counters, ref.counters)
     assert gpu.readback == ref.readback, _diff_dict("Readback", gpu.readback, ref.readback)
-    for k in ("instanceStates", "visibleClusters", "splitTriangles", "partTriangles", "tempInstantiations",
-              "tempInstanceIDs", "tempClusterOffsets", "tempClusterSizes", "blasBuildInfos", "blasClusterOffsets"):
+    keys = ["instanceStates", "visibleClusters", "splitTriangles", "partTriangles", "tempInstantiations",
+            "tempInstanceIDs", "tempClusterOffsets", "tempClusterSizes", "blasBuildInfos", "blasClusterOffsets"]
+    keys += [k for k in ("transBuilds", "transInstanceIDs", "transClusterOffsets", "transClusterSizes", "transClasBlocks",
+                         "transSlots") if k in ref.arrays]
+    for k in keys:
         a, b = gpu.arrays[k], ref.arrays[k]
         assert a.shape == b.shape, "%s: shape %s vs %s" % (k, a.shape, b.shape)
         if not np.array_equal(a, b):

@@ -64,6 +64,25 @@ def test_capacity_overflow_parity():
     harness.compare(res, ref)
 
 
+def test_transient_1x_parity():
+    """1X transient clusters: records, part slots at the top of the part
+    list, index and mapping bytes, CLAS blocks and both insert passes."""
+    cfg = harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_1X_TRANSIENT)
+    res, ref = run_both(small_grid(tess_rate_pixels=16.0), cfg)
+    assert len(ref.arrays["transBuilds"]) > 10
+    harness.compare(res, ref)
+
+
+def test_transient_1x_capacity_overflow_parity():
+    # the part list is shared by parts (from the bottom) and transient slots
+    # (from the top); acceptance stays a prefix of the canonical order
+    cfg = harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_1X_TRANSIENT,
+                               maxPartTriangles=6000, maxGeneratedVertices=200000)
+    res, ref = run_both(small_grid(tess_rate_pixels=16.0), cfg)
+    assert ref.readback["numPartTriangles"] > 6000
+    harness.compare(res, ref)
+
+
 def test_many_instances_parity():
     host = scene.scene_grid_instances(num_instances=16, quads=32, patch_size=4.0, tex_size=128, cam_height=1.5)
     res, ref = run_both(host, harness.small_config())

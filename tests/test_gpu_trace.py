@@ -160,3 +160,22 @@ def test_trace_parity_icosphere():
     # about 17% of the pixels
     assert 0.12 < (oh["t"] > 0).mean() < 0.25
     compare_hits(gh, oh, gd, od)
+
+
+def test_trace_parity_transient_1x():
+    """Hits on 1X transient CLAS decode to the base cluster and triangle."""
+    host = scene.scene_grid_instances(num_instances=4, quads=64, patch_size=8.0, tex_size=256, tess_rate_pixels=16.0)
+    cfg = harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_1X_TRANSIENT)
+    of = harness.OracleFrame(host, cfg)
+    of.run(host.frame())
+    W, H = 160, 90
+    oh, od = of.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(host.frame(), stages="accel")
+        gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    finally:
+        g.close()
+    assert ((oh["clusterID"] >> 30) == abi.RT_CLUSTER_MODE_1X_SUBSET_CLUSTER).sum() > 20
+    agree = compare_hits(gh, oh, gd, od)
+    print("hit agreement", agree)

@@ -34,6 +34,7 @@ BUILD_SETUP_BUILD_BLAS = 5
 
 RT_CLUSTER_MODE_FULL_CLUSTER = 0
 RT_CLUSTER_MODE_SINGLE_TESSELLATED = 1
+RT_CLUSTER_MODE_1X_SUBSET_CLUSTER = 2
 
 TEXTURE_R16_UNORM = 0
 TEXTURE_R32_FLOAT = 1
@@ -195,6 +196,12 @@ HIT_DECODED_DT = np.dtype([("instanceID", "<u4"), ("clusterID", "<u4"), ("triang
                            ("wPos", "<f4", (3,)), ("shadowed", "<u4")])
 BVH_NODE_DT = np.dtype([("lo0", "<f4", (3,)), ("child0", "<u4"), ("hi0", "<f4", (3,)), ("child1", "<u4"),
                         ("lo1", "<f4", (3,)), ("pad0", "<u4"), ("hi1", "<f4", (3,)), ("pad1", "<u4")])
+CLAS_BUILD_INFO_DT = np.dtype([("clusterID", "<u4"), ("clusterFlags", "<u4"), ("packed", "<u4"),
+                               ("baseGeometryIndexAndFlags", "<u4"), ("indexBufferStride", "<u2"),
+                               ("vertexBufferStride", "<u2"), ("geometryIndexAndFlagsBufferStride", "<u2"),
+                               ("opacityMicromapIndexBufferStride", "<u2"), ("indexBuffer", "<u8"), ("vertexBuffer", "<u8"),
+                               ("geometryIndexAndFlagsBuffer", "<u8"), ("opacityMicromapArray", "<u8"),
+                               ("opacityMicromapIndexBuffer", "<u8")])
 BVH4_NODE_DT = np.dtype([("lo", "<f4", (3, 4)), ("hi", "<f4", (3, 4)), ("child", "<u4", (4,)), ("pad", "<u4", (4,))])
 BVH4_EMPTY = 0xFFFFFFFF
 TLAS_INSTANCE_DT = np.dtype([("objectToWorld", "<f4", (12,)), ("worldToObject", "<f4", (12,)), ("blasNodes", "<u8"),
@@ -270,7 +277,7 @@ EXPORTS = [
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
     "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
-    "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning",
+    "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning", "tc_clas_build_transient",
 ]
 
 
@@ -325,6 +332,7 @@ def lib():
         "tc_renderer_graph_launch": (i32, [P, C.POINTER(FrameConstants), P]),
         "tc_fbm_height": (i32, [P, u32, u32, u32, u32, u32, P]),
         "tc_depth_adjoint": (i32, [C.POINTER(AdjointParams), P]),
+        "tc_clas_build_transient": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_get_tuning": (i32, [C.POINTER(Tuning)]),
         "tc_set_tuning": (i32, [C.POINTER(Tuning)]),
         "tc_renderer_rebind_instances": (i32, [P, P, P, u32]),

@@ -71,7 +71,8 @@ static bool config_ok(const tc_config& c) {
   return true;
 }
 
-static constexpr uint32_t kSupportedFlags = TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE;
+static constexpr uint32_t kSupportedFlags =
+    TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -490,19 +491,21 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
   WsLayout L;
   tc_status s = make_args(b, config, a, &L);
   if (s != TC_OK) return s;
-  if (transient) return TC_OK;  // no transient builds in this configuration
+  if (transient && !(config.flags & TC_FLAG_1X_TRANSIENT)) return TC_OK;  // no transient builds configured
+  const int tr = transient ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   SortBuffers sb = sort_buffers(b, L);
   uint32_t* segStart = sb.digitTotals + kSortRadix;
   uint32_t grid = std::min<uint32_t>((sb.capacity + 255) / 256, (uint32_t)num_cu() * 4);
-  hipLaunchKernelGGL(k_insert_keys, dim3(grid), dim3(256), 0, st, a, sb.keys[0], sb.vals[0], sb.capacity);
+  hipLaunchKernelGGL(k_insert_keys, dim3(grid), dim3(256), 0, st, a, sb.keys[0], sb.vals[0], sb.capacity, tr);
   uint32_t bits = 1;
   while ((1u << bits) < a.numInstances) bits++;
   uint32_t res = 0;
-  radix_sort_pairs(sb, &b->build->tempInstantiateCounter, 0, bits, st, &res);
-  hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart, sb.capacity);
+  radix_sort_pairs(sb, tr ? &b->build->transBuildCounter : &b->build->tempInstantiateCounter, 0, bits, st, &res);
+  hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart, sb.capacity,
+                     tr);
   hipLaunchKernelGGL(k_insert_scatter, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res],
-                     (const uint32_t*)sb.vals[res], (const uint32_t*)segStart, sb.capacity);
+                     (const uint32_t*)sb.vals[res], (const uint32_t*)segStart, sb.capacity, tr);
   return launched();
 }
 

@@ -6,7 +6,8 @@
 // a base cluster) and a vertex range; the result at tempClusterAddresses[i]
 // is a self-contained block
 //   tc_clas_header | group boxes | triangle words | vertices
-// and tempClusterSizes[i] receives its size.
+// and tempClusterSizes[i] receives its size.  Transient build records
+// (ClasBuildInfo, 1X clusters) produce the same block format.
 #include <algorithm>
 
 #include "tc_kernels.h"
@@ -26,6 +27,62 @@ __device__ __forceinline__ float wave_max(float v) {
 
 __host__ __device__ __forceinline__ uint32_t r16(uint32_t v) { return (v + 15u) & ~15u; }
 
+// Group boxes and header of a block whose triangle words (s_t) and
+// vertices (s_v) are in LDS and already copied to the block; returns the
+// block size.  One wave.
+__device__ __forceinline__ uint32_t finish_block(char* dst, const float* s_v, const uint32_t* s_t, uint32_t nt, uint32_t nv,
+                                                 uint32_t clusterID, uint32_t vertexBase) {
+  const uint32_t lane = lane_id();
+  const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+  tc_aabb* groups = reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header));
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t g = lane; g < ng; g += kWave) {
+    float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
+    for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
+      const uint32_t w = s_t[t];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const uint32_t vi = (w >> (8 * k)) & 0xFF;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          glo[c] = fminf(glo[c], s_v[vi * 3 + c]);
+          ghi[c] = fmaxf(ghi[c], s_v[vi * 3 + c]);
+        }
+      }
+    }
+    tc_aabb box;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      box.lo[c] = glo[c];
+      box.hi[c] = ghi[c];
+      lo[c] = fminf(lo[c], glo[c]);
+      hi[c] = fmaxf(hi[c], ghi[c]);
+    }
+    groups[g] = box;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    lo[c] = wave_min(lo[c]);
+    hi[c] = wave_max(hi[c]);
+  }
+  if (lane == 0) {
+    tc_clas_header h;
+    h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
+    h.hi[0] = hi[0], h.hi[1] = hi[1], h.hi[2] = hi[2];
+    h.clusterID = clusterID;
+    h.counts = nt | (nv << 8) | (ng << 16);
+    h.vertexBase = vertexBase;
+    h._pad[0] = h._pad[1] = h._pad[2] = 0;
+    *reinterpret_cast<tc_clas_header*>(dst) = h;
+  }
+  return (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+}
+
+__device__ __forceinline__ uint32_t vertex_base(const tc_scene_building* b, uint64_t vbuf) {
+  return vbuf >= b->genVertices ? (uint32_t)((vbuf - b->genVertices) / 12u) : 0xFFFFFFFFu;
+}
+
 // One wave per CLAS, grid-stride over tempInstantiateCounter.
 __global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
   __shared__ float s_v[kMaxClusterVerts * 3];
@@ -41,7 +98,6 @@ __global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
     const tc_clas_template T = *ptr<const tc_clas_template>(info.clusterTemplateAddress);
     const uint32_t nt = T.numTriangles, nv = T.numVertices, ng = T.numGroups;
     char* dst = ptr<char>(dsts[i]);
-    tc_aabb* groups = reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header));
     uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
     float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
     const char* src = ptr<const char>(info.vertexBufferAddress);
@@ -57,49 +113,51 @@ __global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
       tris[t] = w;
     }
     __syncthreads();
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t g = lane; g < ng; g += kWave) {
-      float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
-      for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
-        const uint32_t w = s_t[t];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const uint32_t vi = (w >> (8 * k)) & 0xFF;
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            glo[c] = fminf(glo[c], s_v[vi * 3 + c]);
-            ghi[c] = fmaxf(ghi[c], s_v[vi * 3 + c]);
-          }
-        }
-      }
-      tc_aabb box;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        box.lo[c] = glo[c];
-        box.hi[c] = ghi[c];
-        lo[c] = fminf(lo[c], glo[c]);
-        hi[c] = fmaxf(hi[c], ghi[c]);
-      }
-      groups[g] = box;
+    const uint32_t size = finish_block(dst, s_v, s_t, nt, nv, T.clusterID + info.clusterIdOffset,
+                                       vertex_base(b, info.vertexBufferAddress));
+    if (lane == 0) sizes[i] = size;
+    __syncthreads();
+  }
+}
+
+// Transient builds (stands for the NV transient CLAS build,
+// renderer_raytrace_clusters_tess.cpp:625-655): triangles in record order
+// with the transient triangle index as primitive, all the record's
+// vertices.  One wave per CLAS, grid-stride over transBuildCounter.
+__global__ __launch_bounds__(64) void k_clas_build_transient(KArgs a) {
+  __shared__ float s_v[kMaxClusterVerts * 3];
+  __shared__ uint32_t s_t[kMaxClusterTris];
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->transBuildCounter, a.cfg.maxGeneratedClusters);
+  const uint32_t lane = lane_id();
+  const tc_clas_build_info* infos = ptr<const tc_clas_build_info>(b->transBuilds);
+  const uint64_t* dsts = ptr<const uint64_t>(b->transClusterAddresses);
+  uint32_t* sizes = ptr<uint32_t>(b->transClusterSizes);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const tc_clas_build_info info = infos[i];
+    const uint32_t nt = min(info.packed & 511u, kMaxClusterTris), nv = min((info.packed >> 9) & 511u, kMaxClusterVerts);
+    const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+    char* dst = ptr<char>(dsts[i]);
+    uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
+    float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
+    const char* src = ptr<const char>(info.vertexBuffer);
+    const uint8_t* ix = ptr<const uint8_t>(info.indexBuffer);
+    const uint32_t is = info.indexBufferStride;
+    for (uint32_t k = lane; k < nv * 3; k += kWave) {
+      const uint32_t v = k / 3, c = k - v * 3;
+      const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
+      s_v[k] = f;
+      verts[k] = f;
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      lo[c] = wave_min(lo[c]);
-      hi[c] = wave_max(hi[c]);
+    for (uint32_t t = lane; t < nt; t += kWave) {
+      const uint8_t* q = ix + (uint64_t)t * 3u * is;
+      const uint32_t w = (uint32_t)q[0] | ((uint32_t)q[is] << 8) | ((uint32_t)q[2u * is] << 16) | (t << 24);
+      s_t[t] = w;
+      tris[t] = w;
     }
-    if (lane == 0) {
-      tc_clas_header h;
-      h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
-      h.hi[0] = hi[0], h.hi[1] = hi[1], h.hi[2] = hi[2];
-      h.clusterID = T.clusterID + info.clusterIdOffset;
-      h.counts = nt | (nv << 8) | (ng << 16);
-      h.vertexBase = info.vertexBufferAddress >= b->genVertices ? (uint32_t)((info.vertexBufferAddress - b->genVertices) / 12u)
-                                                                 : 0xFFFFFFFFu;
-      h._pad[0] = h._pad[1] = h._pad[2] = 0;
-      *reinterpret_cast<tc_clas_header*>(dst) = h;
-      sizes[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
-    }
+    __syncthreads();
+    const uint32_t size = finish_block(dst, s_v, s_t, nt, nv, info.clusterID, vertex_base(b, info.vertexBuffer));
+    if (lane == 0) sizes[i] = size;
     __syncthreads();
   }
 }
@@ -114,5 +172,15 @@ extern "C" tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config,
   if (s != TC_OK) return s;
   uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
   hipLaunchKernelGGL(k_clas_instantiate, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+extern "C" tc_status tc_clas_build_transient(const tc_bindings* b, tc_config config, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args_public(b, config, a);
+  if (s != TC_OK) return s;
+  if (!(config.flags & TC_FLAG_1X_TRANSIENT)) return TC_OK;
+  uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
+  hipLaunchKernelGGL(k_clas_build_transient, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }

@@ -7,7 +7,8 @@
 //      for triangles going to the split list);
 //   2. wave 0 scans the 64 clusters' counts, takes the tile's offsets with
 //      one decoupled look-back and makes the allocations of the full
-//      clusters (template instantiations);
+//      clusters (template instantiations) and, with TC_FLAG_1X_TRANSIENT,
+//      of the clusters whose simple triangles form a transient CLAS;
 //   3. the waves emit split / part records and displaced full-cluster
 //      vertices at those offsets.
 // Offsets are those of a serial run in (visible cluster, triangle) order.
@@ -19,14 +20,19 @@ struct ClusterSlot {
   tc_cluster_info ci;
   uint32_t nv, nt;
   uint32_t firstLocalVertex;
-  uint32_t full;
+  uint32_t full, trans;  // template instantiation / 1X transient build
+  uint32_t simple;
   uint32_t ns, np;
-  uint32_t dataSize;
+  uint32_t dataSize, vertexSize, partSize;
   uint32_t offSplit, offPart;
   uint32_t genOffset, vertexOffset;
+  uint32_t partOffset;  // transient part slots (top of the part list)
+  uint32_t yIncl;       // upper part counter after this cluster
   uint32_t ok;
   uint64_t dataOffset;
 };
+
+constexpr uint32_t kSimpleCode = 1u | (1u << 4) | (1u << 8);  // factors (1,1,1), no split
 
 __device__ __forceinline__ uint32_t pack_code(bool split, const uint32_t f[3]) {
   return (split ? 0x8000u : 0u) | f[0] | (f[1] << 4) | (f[2] << 8);
@@ -42,6 +48,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
   const uint32_t epoch = h->epoch, numTiles = h->numTiles, numItems = h->itemEnd;
   const uint64_t B0 = h->base[0], B1 = h->base[1], B2 = h->base[2], B3 = h->base[3], B4 = h->base[4], B5 = h->base[5];
+  const uint64_t B6 = h->base[6], B7 = h->base[7];
   const ViewParams vp = view_params(a);
   tc_scene_building* b = a.build;
   const tc_config cfg = a.cfg;
@@ -107,16 +114,24 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         }
         __builtin_amdgcn_wave_barrier();
       }
+      if (invisible)
+        for (uint32_t t = lane; t < nt; t += kWave) s_code[k][t] = (uint16_t)kSimpleCode;
       if (lane == 0) {
         ClusterSlot s;
         s.ci = ci;
         s.nv = nv;
         s.nt = nt;
         s.firstLocalVertex = cl.firstLocalVertex;
+        s.simple = simple;
         s.full = simple == nt;
+        // :261-276 transientSimpleThreshold = 1
+        s.trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) && !s.full && simple > 1;
         s.ns = s.full ? 0 : ns;
-        s.np = s.full ? 0 : np;
-        s.dataSize = s.full ? ptr<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID] : 0u;
+        s.np = s.full ? 0 : (s.trans ? np - simple : np);
+        s.dataSize = s.full ? ptr<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID]
+                            : (s.trans ? clas_block_size(simple, nv) : 0u);
+        s.vertexSize = s.full ? nv : (s.trans ? nv + (simple * 3 + 11) / 12 : 0u);
+        s.partSize = s.trans ? (8 + simple + 24 - 1) / 24 : 0u;
         s_cl[k] = s;
       }
     }
@@ -128,15 +143,19 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       const bool valid = k < cnt;
       ClusterSlot s;
       if (valid) s = s_cl[k];
+      const bool alloc = valid && (s.full || s.trans);
       const uint32_t cs = valid ? s.ns : 0u, cp = valid ? s.np : 0u;
-      const uint32_t cg = valid && s.full ? 1u : 0u, cv = valid && s.full ? s.nv : 0u;
-      const uint64_t cd = valid && s.full ? (uint64_t)s.dataSize : 0ull;
+      const uint32_t cg = alloc ? 1u : 0u, cv = alloc ? s.vertexSize : 0u;
+      const uint64_t cd = alloc ? (uint64_t)s.dataSize : 0ull;
+      const uint32_t cy = valid ? s.partSize : 0u;
+      const uint32_t cf = valid && s.full ? 1u : 0u, ct = valid && s.trans ? 1u : 0u;
       const uint32_t is = wave_incl_scan(cs), ip = wave_incl_scan(cp), ig = wave_incl_scan(cg), iv = wave_incl_scan(cv);
       const uint64_t id = wave_incl_scan64(cd);
-      uint64_t agg[5] = {shfl(is, kWave - 1), shfl(ip, kWave - 1), shfl(ig, kWave - 1), shfl(iv, kWave - 1),
-                         shfl64(id, kWave - 1)};
-      uint64_t excl[5];
-      lookback<5>(a.status, a.maxTiles, epoch, tile, agg, excl);
+      const uint32_t iy = wave_incl_scan(cy), iff = wave_incl_scan(cf), it = wave_incl_scan(ct);
+      uint64_t agg[8] = {shfl(is, kWave - 1), shfl(ip, kWave - 1), shfl(ig, kWave - 1), shfl(iv, kWave - 1),
+                         shfl64(id, kWave - 1), shfl(iy, kWave - 1), shfl(iff, kWave - 1), shfl(it, kWave - 1)};
+      uint64_t excl[8];
+      lookback<8>(a.status, a.maxTiles, epoch, tile, agg, excl);
       bool ok = false;
       if (valid) {
         s.offSplit = (uint32_t)(B0 + excl[0] + (is - cs));
@@ -144,42 +163,76 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         s.genOffset = (uint32_t)(B2 + excl[2] + (ig - cg));
         s.vertexOffset = (uint32_t)(B3 + excl[3] + (iv - cv));
         s.dataOffset = B4 + excl[4] + (id - cd);
-        ok = s.full && !((uint64_t)s.vertexOffset + s.nv > cfg.maxGeneratedVertices ||
-                         (uint64_t)s.genOffset + 1 > cfg.maxGeneratedClusters ||
-                         s.dataOffset + s.dataSize > max_data_bytes(cfg));
+        const uint64_t yBefore = B6 + excl[5] + (iy - cy);
+        s.yIncl = (uint32_t)(yBefore + cy);
+        // build.glsl:54-66 transient part slots from the top of the list
+        s.partOffset = s.trans && (uint64_t)s.offPart + yBefore + s.partSize + 1 <= cfg.maxPartTriangles
+                           ? (uint32_t)(cfg.maxPartTriangles - yBefore - s.partSize)
+                           : cfg.maxPartTriangles;
+        ok = alloc && !((uint64_t)s.vertexOffset + s.vertexSize > cfg.maxGeneratedVertices ||
+                        (uint64_t)s.genOffset + 1 > cfg.maxGeneratedClusters ||
+                        s.dataOffset + s.dataSize > max_data_bytes(cfg) ||
+                        (s.trans && (uint64_t)s.partOffset + s.partSize > cfg.maxPartTriangles));
         s.ok = ok;
         s_cl[k] = s;
         if (ok) {
-          // :372-392; accepted clusters are a prefix of the canonical order
-          const uint32_t tempOffset = (uint32_t)(B5 + (s.genOffset - B2));
+          // accepted template / transient clusters are prefixes of their
+          // kind in the canonical order, so their slots are ranks
           const tc_render_instance& inst = a.instances[s.ci.instanceID];
-          tc_template_instantiate_info ti;
-          ti.clusterIdOffset = 0;
-          ti.geometryIndexOffset = 0;
-          ti.clusterTemplateAddress = ptr<const uint64_t>(inst.clusterTemplateAdresses)[s.ci.clusterID];
-          ti.vertexBufferAddress = b->genVertices + (uint64_t)s.vertexOffset * 12ull;
-          ti.vertexBufferStride = 12;
-          ptr<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
-          ptr<uint32_t>(b->tempInstanceIDs)[tempOffset] = s.ci.instanceID;
-          ptr<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + s.dataOffset;
+          const uint64_t vbuf = b->genVertices + (uint64_t)s.vertexOffset * 12ull;
+          if (s.full) {
+            // :372-392
+            const uint32_t tempOffset = (uint32_t)(B5 + excl[6] + (iff - cf));
+            tc_template_instantiate_info ti;
+            ti.clusterIdOffset = 0;
+            ti.geometryIndexOffset = 0;
+            ti.clusterTemplateAddress = ptr<const uint64_t>(inst.clusterTemplateAdresses)[s.ci.clusterID];
+            ti.vertexBufferAddress = vbuf;
+            ti.vertexBufferStride = 12;
+            ptr<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = ti;
+            ptr<uint32_t>(b->tempInstanceIDs)[tempOffset] = s.ci.instanceID;
+            ptr<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + s.dataOffset;
+          } else {
+            // :395-436 transient build record
+            const uint32_t transOffset = (uint32_t)(B7 + excl[7] + (it - ct));
+            tc_clas_build_info bi;
+            bi.clusterID = (TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER << 30) | s.partOffset;
+            bi.clusterFlags = 0;
+            bi.packed = s.simple | (s.nv << 9) | ((b->positionTruncateBitCount & 63u) << 18) | (1u << 24);
+            bi.baseGeometryIndexAndFlags = TC_CLAS_GEOMETRY_OPAQUE;
+            bi.indexBufferStride = 1;
+            bi.vertexBufferStride = 12;
+            bi.geometryIndexAndFlagsBufferStride = 0;
+            bi.opacityMicromapIndexBufferStride = 0;
+            bi.indexBuffer = vbuf + (uint64_t)s.nv * 12ull;
+            bi.vertexBuffer = vbuf;
+            bi.geometryIndexAndFlagsBuffer = 0;
+            bi.opacityMicromapArray = 0;
+            bi.opacityMicromapIndexBuffer = 0;
+            ptr<tc_clas_build_info>(b->transBuilds)[transOffset] = bi;
+            ptr<uint32_t>(b->transInstanceIDs)[transOffset] = s.ci.instanceID;
+            ptr<uint64_t>(b->transClusterAddresses)[transOffset] = b->genClusterData + s.dataOffset;
+            ptr<tc_tess_triangle_info>(b->partTriangles)[s.partOffset].cluster = s.ci;
+          }
         }
       }
       // :302, :442-444 statistics and BLAS reference counts, aggregated
       add_blas_reference_counts(ptr<tc_blas_build_info>(b->blasBuildInfos), valid ? s.ci.instanceID : 0u, ok ? 1u : 0u, ok);
-      const uint32_t nFull = __popcll(ballot(cg != 0)), nOk = __popcll(ballot(ok));
-      const uint32_t okTris = wave_sum(ok ? s.nt : 0u);
+      const uint32_t nFull = __popcll(ballot(cf != 0));
+      const uint32_t nOkFull = __popcll(ballot(ok && cf)), nOkTrans = __popcll(ballot(ok && ct));
+      const uint32_t okTris = wave_sum(ok ? s.simple : 0u);
       if (lane == 0) {
         if (nFull) atomicAdd(&a.readback->numFullClusters, nFull);
-        if (nOk) {
-          atomicAdd(&b->tempInstantiateCounter, nOk);
-          atomicAdd(&a.readback->numTotalTriangles, okTris);
-        }
+        if (nOkFull) atomicAdd(&b->tempInstantiateCounter, nOkFull);
+        if (nOkTrans) atomicAdd(&b->transBuildCounter, nOkTrans);
+        if (okTris) atomicAdd(&a.readback->numTotalTriangles, okTris);
         if (tile == numTiles - 1) {
           b->splitTriangleCounter = (int32_t)(uint32_t)(B0 + excl[0] + agg[0]);
           b->partTriangleCounter = (uint32_t)(B1 + excl[1] + agg[1]);
           b->genClusterCounter = (uint32_t)(B2 + excl[2] + agg[2]);
           b->genVertexCounter = (uint32_t)(B3 + excl[3] + agg[3]);
           b->genClusterDataCounter = B4 + excl[4] + agg[4];
+          b->dualPartTriangleCounter = ((B6 + excl[5] + agg[5]) << 32) | (uint32_t)(B1 + excl[1] + agg[1]);
         }
       }
     }
@@ -188,8 +241,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
     // ---- 3. emission
     for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
       const ClusterSlot s = s_cl[k];
-      if (s.full) {
-        if (!s.ok) continue;
+      if (s.ok) {
         // :459-488 displaced cluster vertices
         const tc_render_instance& inst = a.instances[s.ci.instanceID];
         const bool disp = has_disp(a, inst.displacementIndex);
@@ -207,12 +259,38 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
           }
           st3(b->genVertices, (uint64_t)s.vertexOffset + v, p);
         }
-      } else {
-        // :600-666 split / part records in triangle order
+        if (s.trans) {
+          // :497-534 triangle mapping bytes after the ClusterInfo of the part
+          // slot, u8 indices after the vertices, in triangle order
+          const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[s.ci.clusterID];
+          const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+          uint8_t* map = ptr<uint8_t>(b->partTriangles) + (uint64_t)s.partOffset * 24ull + 8ull;
+          uint8_t* idx = ptr<uint8_t>(b->genVertices) + ((uint64_t)s.vertexOffset + s.nv) * 12ull;
+          uint32_t run = 0;
+          for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
+            const uint32_t t = t0 + lane;
+            const bool simple = t < s.nt && s_code[k][t] == kSimpleCode;
+            const uint64_t vm = ballot(simple);
+            const uint32_t o = run + mask_prefix(vm);
+            run += __popcll(vm);
+            if (simple) {
+              map[o] = (uint8_t)t;
+              idx[o * 3 + 0] = lt[t * 3 + 0];
+              idx[o * 3 + 1] = lt[t * 3 + 1];
+              idx[o * 3 + 2] = lt[t * 3 + 2];
+            }
+          }
+          // zero the rest of the index slots so the vertex stream is deterministic
+          for (uint32_t j = s.simple * 3 + lane; j < (s.vertexSize - s.nv) * 12; j += kWave) idx[j] = 0;
+        }
+      }
+      if (!s.full) {
+        // :543-666 split / part records in triangle order (the simple
+        // triangles of a transient cluster are in its CLAS)
         uint32_t runSplit = s.offSplit, runPart = s.offPart;
         for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
           const uint32_t t = t0 + lane;
-          const bool valid = t < s.nt;
+          const bool valid = t < s.nt && !(s.trans && s_code[k][t] == kSimpleCode);
           const uint32_t code = valid ? (uint32_t)s_code[k][t] : 0u;
           const bool split = valid && (code & 0x8000u);
           const bool part = valid && !split;
@@ -222,7 +300,10 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
           runSplit += __popcll(vs);
           runPart += __popcll(vpm);
           const bool writeSplit = split && offSplit < cfg.maxSplitTriangles;
-          const bool writePart = part && offPart < cfg.maxPartTriangles;
+          // build.glsl:68-83 with transient builds: x + y + 2 <= MAX_PART_TRIANGLES
+          const bool writePart = part && ((cfg.flags & TC_FLAG_1X_TRANSIENT)
+                                              ? (uint64_t)offPart + s.yIncl + 2 <= cfg.maxPartTriangles
+                                              : offPart < cfg.maxPartTriangles);
           if (writeSplit || writePart) {
             uint32_t f[3] = {code & 15u, (code >> 4) & 15u, (code >> 8) & 15u};
             uint32_t vtx[3] = {0u, 32768u, 32768u << 16};

@@ -379,7 +379,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 //   [63:44] epoch  [43:42] flag (1 aggregate, 2 inclusive)  [41:0] value
 constexpr uint32_t kEpochBits = 20;
 constexpr uint64_t kValueMask = (1ull << 42) - 1;
-constexpr uint32_t kMaxCounters = 5;
+constexpr uint32_t kMaxCounters = 8;
 
 __device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint64_t value) {
   return ((uint64_t)epoch << 44) | ((uint64_t)flag << 42) | (value & kValueMask);

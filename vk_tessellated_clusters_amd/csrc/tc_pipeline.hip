@@ -47,6 +47,8 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
     h->base[3] = b->genVertexCounter;
     h->base[4] = b->genClusterDataCounter;
     h->base[5] = b->tempInstantiateCounter;
+    h->base[6] = (uint32_t)(b->dualPartTriangleCounter >> 32);
+    h->base[7] = b->transBuildCounter;
   } else if (stage == kStageSplit) {
     uint32_t s = min(b->splitPassStart, a.cfg.maxSplitTriangles), en = min(b->splitPassEnd, a.cfg.maxSplitTriangles);
     uint32_t n = en > s ? en - s : 0;
@@ -55,6 +57,7 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
     h->itemEnd = s + n;
     h->base[0] = b->splitWriteCounter;
     h->base[1] = b->partTriangleCounter;
+    h->base[2] = (uint32_t)(b->dualPartTriangleCounter >> 32);
   } else {
     uint32_t n = min(b->partTriangleCounter, a.cfg.maxPartTriangles);
     h->numTiles = min((n + kInstTile - 1) / kInstTile, a.maxTiles);
@@ -69,7 +72,8 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
 
 // ---------------------------------------------------------- build_setup
 
-// build_setup.comp.glsl:102-264 (no transient builds)
+// build_setup.comp.glsl:102-264 (1X transient builds with
+// TC_FLAG_1X_TRANSIENT)
 __global__ void k_build_setup(KArgs a, int mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   tc_scene_building* b = a.build;
@@ -106,8 +110,11 @@ __global__ void k_build_setup(KArgs a, int mode) {
     b->splitPassEnd = end;
     b->dispatchTriangleSplit = tc_dispatch_indirect{(end - start + 63) / 64, 1, 1};
   } else if (mode == TC_BUILD_SETUP_BUILD_BLAS) {
+    const bool trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) != 0;
     uint32_t temp = b->tempInstantiateCounter;
-    rb->numBlasClusters = temp;
+    uint32_t tr = trans ? b->transBuildCounter : 0u;
+    rb->numBlasClusters = temp + tr;
+    if (trans) rb->numTransBuilds = tr;
     rb->numTempInstantiations = temp;
     rb->numGenDatas = b->genClusterDataCounter;
     rb->numGenVertices = b->genVertexCounter;
@@ -116,11 +123,27 @@ __global__ void k_build_setup(KArgs a, int mode) {
     b->tempInstantiateCounter = temp;
     rb->numActualTempInstantiations = temp;
     b->dispatchBlasTempInsert = tc_dispatch_indirect{(temp + 63) / 64, 1, 1};
+    if (trans) {
+      tr = min(cfg.maxGeneratedClusters, temp + tr) - temp;
+      b->transBuildCounter = tr;
+      rb->numActualTransBuilds = tr;
+      b->dispatchBlasTransInsert = tc_dispatch_indirect{(tr + 63) / 64, 1, 1};
+    }
   } else if (mode == TC_BUILD_SETUP_INSTANTIATE_TESS) {
     uint32_t part = b->partTriangleCounter;
-    rb->numPartTriangles = part;
     rb->numSplitTriangles = b->splitWriteCounter;
-    part = min(part, cfg.maxPartTriangles);
+    if (cfg.flags & TC_FLAG_1X_TRANSIENT) {
+      // :240-253: the instantiated parts are the accepted prefix
+      // x + y + 2 <= MAX_PART_TRIANGLES of the lower counter
+      const uint32_t y = (uint32_t)(b->dualPartTriangleCounter >> 32);
+      rb->numPartTriangles = part + y;
+      rb->numTransPartTriangles = y;
+      const uint32_t cap = cfg.maxPartTriangles > y + 1 ? cfg.maxPartTriangles - y - 1 : 0u;
+      part = min(part, cap);
+    } else {
+      rb->numPartTriangles = part;
+      part = min(part, cfg.maxPartTriangles);
+    }
     b->partTriangleCounter = part;
     b->dispatchTriangleInstantiate = tc_dispatch_indirect{(part + 31) / 32, 1, 1};
   }
@@ -167,7 +190,7 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
   WsHeader* h = a.ws;
   const uint32_t lane = lane_id();
   const uint32_t epoch = h->epoch, numTiles = h->numTiles, itemBegin = h->itemBegin, itemEnd = h->itemEnd;
-  const uint64_t B0 = h->base[0], B1 = h->base[1];
+  const uint64_t B0 = h->base[0], B1 = h->base[1], transY = h->base[2];
   const ViewParams vp = view_params(a);
   const TableView T = load_table(a.tessTable);
   tc_scene_building* b = a.build;
@@ -267,9 +290,14 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
         ti.subTriangle.vtxEncoded[1] = vtx[1];
         ti.subTriangle.vtxEncoded[2] = vtx[2];
         ti.subTriangle.triangleID_config = (itp->triangleID_config & 0xFFFFu) | (cf << 16);
+        // with transient builds the part list shares its top with their
+        // slots (build.glsl:68-83): x + y + 2 <= MAX_PART_TRIANGLES
+        const bool partOk = (cfg.flags & TC_FLAG_1X_TRANSIENT)
+                                ? (uint64_t)offPart + transY + 2 <= cfg.maxPartTriangles
+                                : offPart < cfg.maxPartTriangles;
         if (needSplit && offSplit < cfg.maxSplitTriangles)
           ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-        else if (needPart && offPart < cfg.maxPartTriangles)
+        else if (needPart && partOk)
           ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
       }
     }
@@ -277,6 +305,7 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
     if (tile == numTiles - 1 && lane == 0) {
       b->splitWriteCounter = (uint32_t)(B0 + excl[0] + agg[0]);
       b->partTriangleCounter = (uint32_t)(B1 + excl[1] + agg[1]);
+      b->dualPartTriangleCounter = (transY << 32) | (uint32_t)(B1 + excl[1] + agg[1]);
     }
     __syncthreads();
   }

@@ -156,6 +156,11 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&B.blasBuildInfos, sizeof(tc_blas_build_info) * (uint64_t)numInstances},
       {&B.blasBuildSizes, 4ull * numInstances},
       {&B.blasClusterAddresses, 8ull * c.maxGeneratedClusters},
+      // renderer_raytrace_clusters_tess.cpp:224-229 (1X transient builds)
+      {&B.transBuilds, (c.flags & TC_FLAG_1X_TRANSIENT) ? sizeof(tc_clas_build_info) * (uint64_t)c.maxGeneratedClusters : 0},
+      {&B.transInstanceIDs, (c.flags & TC_FLAG_1X_TRANSIENT) ? 4ull * c.maxGeneratedClusters : 0},
+      {&B.transClusterAddresses, (c.flags & TC_FLAG_1X_TRANSIENT) ? 8ull * c.maxGeneratedClusters : 0},
+      {&B.transClusterSizes, (c.flags & TC_FLAG_1X_TRANSIENT) ? 4ull * c.maxGeneratedClusters : 0},
       {&B.genClusterData, (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull},
       {&oWs, wsBytes},
       {&oXf, 24ull * 4 * numInstances},
@@ -192,8 +197,13 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_instance_xforms_host(instancesHost, numInstances, xf.data());
   for (uint64_t* f : {&B.instanceStates, &B.visibleClusters, &B.splitTriangles, &B.partTriangles, &B.genVertices,
                       &B.tempInstanceIDs, &B.tempInstantiations, &B.tempClusterAddresses, &B.tempClusterSizes,
-                      &B.blasBuildInfos, &B.blasBuildSizes, &B.blasClusterAddresses, &B.genClusterData})
+                      &B.blasBuildInfos, &B.blasBuildSizes, &B.blasClusterAddresses, &B.genClusterData, &B.transBuilds,
+                      &B.transInstanceIDs, &B.transClusterAddresses, &B.transClusterSizes})
     *f += base;
+  if (!(c.flags & TC_FLAG_1X_TRANSIENT)) B.transBuilds = B.transInstanceIDs = B.transClusterAddresses = B.transClusterSizes = 0;
+  // :251, :293 the transient index and mapping bytes alias these lists
+  B.transTriIndices = B.genVertices;
+  B.transTriMappings = B.partTriangles;
   r->splitBytes = sizeof(tc_tess_triangle_info) * (uint64_t)c.maxSplitTriangles;
 
   tc_status s = TC_OK;
@@ -267,8 +277,10 @@ tc_status tc_renderer_build_accel(tc_renderer* rr, tc_stream stream) {
   tc_renderer_get_bindings(rr, &b);
   tc_status s;
   if ((s = tc_clas_instantiate(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_clas_build_transient(&b, r->cfg, stream)) != TC_OK) return s;
   if ((s = tc_blas_setup_insertion(&b, r->cfg, stream)) != TC_OK) return s;
   if ((s = tc_blas_clusters_insert(&b, r->cfg, 0, stream)) != TC_OK) return s;
+  if ((s = tc_blas_clusters_insert(&b, r->cfg, 1, stream)) != TC_OK) return s;
   return tc_accel_build(&b, r->cfg, &r->accel, r->xforms, stream);
 }
 

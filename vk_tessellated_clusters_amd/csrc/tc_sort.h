@@ -19,9 +19,9 @@ struct SortBuffers {
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
                       uint32_t* outIndex);
 
-__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity);
-__global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity);
+__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity, int transient);
+__global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity, int transient);
 __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
-                                 uint32_t capacity);
+                                 uint32_t capacity, int transient);
 
 }  // namespace tc

@@ -137,41 +137,63 @@ void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit
 
 // ------------------------------------------------- blas_clusters_insert
 
-__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity) {
-  const tc_scene_building* b = a.build;
-  const uint32_t n = min(b->tempInstantiateCounter, capacity);
-  const uint32_t* inst = ptr<const uint32_t>(b->tempInstanceIDs);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    keys[i] = inst[i];
+// The list an insert pass reads: template instantiations, or (transient)
+// the 1X transient builds, which follow the templates in every instance's
+// reference list (blas_clusters_insert runs twice, :99-128).
+struct InsertList {
+  uint32_t n;
+  const uint32_t* ids;
+  const uint64_t* addrs;
+  const uint32_t* sizes;
+};
+
+__device__ __forceinline__ InsertList insert_list(const tc_scene_building* b, int transient, uint32_t capacity) {
+  InsertList l;
+  l.n = min(transient ? b->transBuildCounter : b->tempInstantiateCounter, capacity);
+  l.ids = ptr<const uint32_t>(transient ? b->transInstanceIDs : b->tempInstanceIDs);
+  l.addrs = ptr<const uint64_t>(transient ? b->transClusterAddresses : b->tempClusterAddresses);
+  l.sizes = ptr<const uint32_t>(transient ? b->transClusterSizes : b->tempClusterSizes);
+  return l;
+}
+
+__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity, int transient) {
+  const InsertList l = insert_list(a.build, transient, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
+    keys[i] = l.ids[i];
     vals[i] = i;
   }
 }
 
-__global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity) {
-  const uint32_t n = min(a.build->tempInstantiateCounter, capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    if (k == 0 || keys[k - 1] != keys[k]) segStart[(uint32_t)keys[k]] = k;
+// segStart[instance] = first sorted position of the instance minus the
+// references it already holds (templates, for the transient pass), so that
+// k - segStart is the slot in its list (modulo 2^32).
+__global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity, int transient) {
+  const InsertList l = insert_list(a.build, transient, capacity);
+  const tc_blas_build_info* infos = ptr<const tc_blas_build_info>(a.build->blasBuildInfos);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < l.n; k += gridDim.x * blockDim.x)
+    if (k == 0 || keys[k - 1] != keys[k]) {
+      const uint32_t inst = (uint32_t)keys[k];
+      segStart[inst] = k - (transient ? infos[inst].clusterReferencesCount : 0u);
+    }
 }
 
 // blas_clusters_insert.comp.glsl:97-135: every CLAS lands at its rank among
 // the CLAS of its instance (CLAS order), in that instance's reference list.
 __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
-                                 uint32_t capacity) {
+                                 uint32_t capacity, int transient) {
   tc_scene_building* b = a.build;
-  const uint32_t n = min(b->tempInstantiateCounter, capacity);
+  const InsertList l = insert_list(b, transient, capacity);
   tc_blas_build_info* infos = ptr<tc_blas_build_info>(b->blasBuildInfos);
-  const uint64_t* addrs = ptr<const uint64_t>(b->tempClusterAddresses);
-  const uint32_t* sizes = ptr<const uint32_t>(b->tempClusterSizes);
   uint64_t sizeSum = 0;
-  for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {
+  for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < l.n; k0 += gridDim.x * blockDim.x) {
     const uint32_t k = k0 + threadIdx.x;
-    if (k < n) {
+    if (k < l.n) {
       const uint32_t inst = (uint32_t)keys[k];
       const uint32_t idx = vals[k];
       const uint32_t rank = k - segStart[inst];
-      ptr<uint64_t>(infos[inst].clusterReferences)[rank] = addrs[idx];
-      if (k + 1 == n || keys[k + 1] != keys[k]) infos[inst].clusterReferencesCount = rank + 1;
-      sizeSum += sizes[idx];
+      ptr<uint64_t>(infos[inst].clusterReferences)[rank] = l.addrs[idx];
+      if (k + 1 == l.n || keys[k + 1] != keys[k]) infos[inst].clusterReferencesCount = rank + 1;
+      sizeSum += l.sizes[idx];
     }
   }
   sizeSum = wave_sum64(sizeSum);

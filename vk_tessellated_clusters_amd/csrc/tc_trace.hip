@@ -335,7 +335,14 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   uint32_t triangleID = h.primitiveID;
   const v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v);
   v3 base = bw;
-  if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
+  if (mode == TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER) {
+    // :173-178 the part slot holds the ClusterInfo, then one byte per
+    // transient triangle naming the cluster triangle
+    const uint8_t* slot = ptr<const uint8_t>(a.build->partTriangles) + (uint64_t)clusterID * 24ull;
+    const tc_cluster_info ci = *reinterpret_cast<const tc_cluster_info*>(slot);
+    triangleID = slot[8 + h.primitiveID];
+    clusterID = ci.clusterID;
+  } else if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
     const tc_tess_triangle_info ti = ptr<const tc_tess_triangle_info>(a.build->partTriangles)[clusterID];
     triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
     const uint32_t cfg = ti.subTriangle.triangleID_config >> 16;

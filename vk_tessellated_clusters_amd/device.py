@@ -184,8 +184,10 @@ class Renderer:
         s = self.stream_ptr(stream)
         b = C.byref(self.bindings)
         abi.check(self.L.tc_clas_instantiate(b, self.config, s), "tc_clas_instantiate")
+        abi.check(self.L.tc_clas_build_transient(b, self.config, s), "tc_clas_build_transient")
         abi.check(self.L.tc_blas_setup_insertion(b, self.config, s), "tc_blas_setup_insertion")
         abi.check(self.L.tc_blas_clusters_insert(b, self.config, 0, s), "tc_blas_clusters_insert")
+        abi.check(self.L.tc_blas_clusters_insert(b, self.config, 1, s), "tc_blas_clusters_insert")
 
     # ------------------------------------------------------------ readback
 
