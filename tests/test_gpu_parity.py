@@ -150,6 +150,39 @@ def test_launch_shapes_do_not_change_results():
         assert np.array_equal(d.view(np.uint8), d0.view(np.uint8))
 
 
+def test_graph_replays_match_eager_frames():
+    """Graph replays with new frame constants (queued back to back, more
+    than the pinned ring holds) end in the same state as an eager frame
+    with the last constants."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    host = small_grid()
+    cfg = harness.small_config()
+    frames = []
+    for k in range(20):
+        host.camera.eye = (8.0 + 0.3 * k, 2.5, -2.4 + 0.1 * k)
+        frames.append(host.frame())
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(frames[-1], stages="accel")
+        eager = g.results()
+        eh, ed = g.trace(64, 36, abi.TRACE_CULL_BACK)
+        stream = torch.cuda.Stream()
+        s = C.c_void_p(stream.cuda_stream)
+        L = g.r.L
+        abi.check(L.tc_renderer_graph_capture(g.r.handle, C.byref(frames[0]), None, s))
+        for fc in frames:
+            abi.check(L.tc_renderer_graph_launch(g.r.handle, C.byref(fc), s))
+        stream.synchronize()
+        replay = g.results()
+        rh, rd = g.trace(64, 36, abi.TRACE_CULL_BACK)
+    finally:
+        g.close()
+    harness.compare(replay, eager, max_ulp=0)
+    assert np.array_equal(rh.view(np.uint8), eh.view(np.uint8))
+
+
 def test_device_fbm_matches_numpy():
     """The device height-map generator reproduces the numpy generator texel
     for texel (R32 and R16)."""

@@ -27,7 +27,18 @@ struct Renderer {
   uint64_t workspaceBytes = 0;
   tc_vertex_tape* tape = nullptr;  // with TC_FLAG_VERTEX_TAPE
   tc_scene_building hostBuild{};
-  tc_frame_constants* pinnedView = nullptr;
+  // Frame constants travel through a ring of pinned slots: the host fills
+  // slot hostFrame % kViewSlots, k_fetch_view (in the stream or the graph)
+  // reads the slot named by a device counter that advances once per
+  // frame, and an event per slot keeps the host from refilling a slot
+  // that a queued frame has not read yet.
+  tc_frame_constants* pinnedRing = nullptr;
+  uint32_t* frameCounter = nullptr;  // device
+  uint64_t hostFrame = 0;
+  hipEvent_t slotEvent[8] = {};
+  bool slotPending[8] = {};
+  bool capturing = false;
+  tc_frame_constants lastFrame{};  // constants of the latest frame queued or captured
   uint64_t splitBytes = 0;
   float* xforms = nullptr;  // per instance objectToWorld | worldToObject (3x4 row-major)
   tc_accel accel{};
@@ -39,6 +50,18 @@ struct Renderer {
 };
 
 static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
+constexpr uint32_t kViewSlots = 8;
+
+// Copies the pinned slot the device frame counter names into the frame
+// constants and advances the counter (one workgroup).
+__global__ __launch_bounds__(256) void k_fetch_view(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view) {
+  const uint32_t slot = counter[0] % kViewSlots;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(ring + slot);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(view);
+  for (uint32_t i = threadIdx.x; i < sizeof(tc_frame_constants) / 4; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) counter[0] = counter[0] + 1;
+}
 
 // inverse of a 4x4 row-major matrix in double precision (cofactors)
 static bool invert4(const double m[16], double out[16]) {
@@ -79,6 +102,26 @@ __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building*
   }
   for (uint64_t i = gid; i < sizeof(tc_readback) / 4; i += stride) reinterpret_cast<uint32_t*>(rb)[i] = 0;
   for (uint64_t i = gid; i < splitWords; i += stride) split[i] = ~0u;
+}
+
+// Writes the frame constants into a ring slot, first waiting until the
+// frame that used the slot last has read it.
+static bool fill_slot(Renderer* r, uint32_t slot, const tc_frame_constants* fc) {
+  if (r->slotPending[slot]) {
+    if (hipEventSynchronize(r->slotEvent[slot]) != hipSuccess) return false;
+    r->slotPending[slot] = false;
+  }
+  memcpy(&r->pinnedRing[slot], fc, sizeof(tc_frame_constants));
+  if (fc != &r->lastFrame) r->lastFrame = *fc;
+  return true;
+}
+
+// Marks the slot as read by the work just queued on `st`.
+static bool queued_slot(Renderer* r, uint32_t slot, hipStream_t st) {
+  if (hipEventRecord(r->slotEvent[slot], st) != hipSuccess) return false;
+  r->slotPending[slot] = true;
+  r->hostFrame++;
+  return true;
 }
 
 }  // namespace tc
@@ -136,7 +179,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape;
+  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
       {&oRb, sizeof(tc_readback)},
@@ -166,6 +209,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&oXf, 24ull * 4 * numInstances},
       {&oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances)},
       {&oTape, (c.flags & TC_FLAG_VERTEX_TAPE) ? sizeof(tc_vertex_tape) * (uint64_t)c.maxGeneratedVertices : 0},
+      {&oCounter, 256},
   };
   uint64_t o = 0;
   for (auto& s : slots) {
@@ -189,6 +233,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   r->workspaceBytes = wsBytes;
   r->xforms = reinterpret_cast<float*>(mem + oXf);
   if (c.flags & TC_FLAG_VERTEX_TAPE) r->tape = reinterpret_cast<tc_vertex_tape*>(mem + oTape);
+  r->frameCounter = reinterpret_cast<uint32_t*>(mem + oCounter);
   r->ownInstances = r->instances;
   r->ownXforms = r->xforms;
   tc_accel_bind(mem + oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances), c.maxGeneratedClusters, numInstances,
@@ -212,12 +257,15 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
        hipMemcpy(r->textures, texturesHost, sizeof(tc_displacement_texture) * numTextures, hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(r->buildTemplate, &B, sizeof(B), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(r->xforms, xf.data(), xf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(r->arena, 0, oInst) != hipSuccess ||
+      hipMemset(r->arena, 0, oInst) != hipSuccess || hipMemset(r->frameCounter, 0, 256) != hipSuccess ||
       hipMemset(reinterpret_cast<void*>(B.blasBuildSizes), 0, 4ull * numInstances) != hipSuccess)
     s = TC_ERROR_LAUNCH;
   if (s == TC_OK) s = tc_table_upload(&r->table);
-  if (s == TC_OK && hipHostMalloc((void**)&r->pinnedView, sizeof(tc_frame_constants), hipHostMallocDefault) != hipSuccess)
+  if (s == TC_OK &&
+      hipHostMalloc((void**)&r->pinnedRing, sizeof(tc_frame_constants) * kViewSlots, hipHostMallocDefault) != hipSuccess)
     s = TC_ERROR_OUT_OF_MEMORY;
+  for (uint32_t k = 0; s == TC_OK && k < kViewSlots; k++)
+    if (hipEventCreateWithFlags(&r->slotEvent[k], hipEventDisableTiming) != hipSuccess) s = TC_ERROR_LAUNCH;
   if (s == TC_OK) {
     tc_bindings b;
     tc_renderer_get_bindings(reinterpret_cast<tc_renderer*>(r), &b);
@@ -238,7 +286,12 @@ tc_status tc_renderer_destroy(tc_renderer* rr) {
   if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec);
   if (r->graph) (void)hipGraphDestroy(r->graph);
   if (r->table) tc_table_free(r->table);
-  if (r->pinnedView) (void)hipHostFree(r->pinnedView);
+  for (uint32_t k = 0; k < kViewSlots; k++)
+    if (r->slotEvent[k]) (void)hipEventDestroy(r->slotEvent[k]);
+  if (r->pinnedRing) {
+    (void)hipDeviceSynchronize();
+    (void)hipHostFree(r->pinnedRing);
+  }
   if (r->arena) (void)hipFree(r->arena);
   delete r;
   return TC_OK;
@@ -330,7 +383,10 @@ tc_status tc_renderer_graph_capture(tc_renderer* rr, const tc_frame_constants* f
   if (r->graph) (void)hipGraphDestroy(r->graph), r->graph = nullptr;
   if (params) r->graphParams = *params;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return TC_ERROR_LAUNCH;
+  r->capturing = true;
+  r->lastFrame = *frameHost;
   tc_status s = tc_renderer_frame(rr, frameHost, params ? &r->graphParams : nullptr, stream);
+  r->capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(st, &g);
   if (s != TC_OK || e != hipSuccess) {
@@ -349,8 +405,10 @@ tc_status tc_renderer_graph_capture(tc_renderer* rr, const tc_frame_constants* f
 tc_status tc_renderer_graph_launch(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
   if (!rr || !R(rr)->graphExec) return TC_ERROR_INVALID_ARGUMENT;
   Renderer* r = R(rr);
-  if (frameHost) memcpy(r->pinnedView, frameHost, sizeof(tc_frame_constants));
-  return hipGraphLaunch(r->graphExec, (hipStream_t)stream) == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+  const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
+  if (!fill_slot(r, slot, frameHost ? frameHost : &r->lastFrame)) return TC_ERROR_LAUNCH;
+  if (hipGraphLaunch(r->graphExec, (hipStream_t)stream) != hipSuccess) return TC_ERROR_LAUNCH;
+  return queued_slot(r, slot, (hipStream_t)stream) ? TC_OK : TC_ERROR_LAUNCH;
 }
 
 tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {
@@ -365,9 +423,12 @@ tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* fra
   if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
   Renderer* r = R(rr);
   hipStream_t st = (hipStream_t)stream;
-  memcpy(r->pinnedView, frameHost, sizeof(tc_frame_constants));
-  if (hipMemcpyAsync(r->view, r->pinnedView, sizeof(tc_frame_constants), hipMemcpyHostToDevice, st) != hipSuccess)
-    return TC_ERROR_LAUNCH;
+  // during graph capture only the fetch is recorded; each replay fills its
+  // slot in tc_renderer_graph_launch
+  const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
+  if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
+  hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter, r->view);
+  if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
   hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
                      (const tc_frame_constants*)r->view, r->readback, reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles),
                      r->splitBytes / 4);

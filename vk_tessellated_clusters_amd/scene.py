@@ -420,3 +420,57 @@ def scene_batched(num_meshes=500, seed=2, tex_size=1024, width=1920, height=1080
     cam = Camera(eye=(0.0, 8.0, -12.0), center=(0.0, 1.0, field * 0.4), width=width, height=height, near=0.05,
                  far=10 * field)
     return HostScene(geos, insts, texs, cam, tess_rate_pixels, "batched")
+
+
+# ------------------------------------------------------- large 4K frame
+
+
+def scene_large(patches=8, quads_x=512, quads_z=256, quad_size=1.0 / 16.0, tex_size=4096, seed=5, octaves=6,
+                width=3840, height=2160, tess_rate_pixels=2.0, disp_scale=1.5):
+    """patches x patches instances of a quads_x x quads_z-quad grid patch
+    (512 x 256 quads: 262,144 triangles each, 16.8M base triangles for the
+    8 x 8 field) on a square lattice, one tex_size^2 R16 fBm height map,
+    3840 x 2160 at 2 px per segment.  The camera follows camera_spline()."""
+    sx, sz = quads_x * quad_size, quads_z * quad_size
+    geo = grid_patch(quads_x, quads_z, sx, sz)
+    insts = []
+    for j in range(patches):
+        for i in range(patches):
+            M = rigid_transform(0.0, [i * sx, 0.0, j * sz])
+            insts.append(Instance(0, M, displacement_index=0, displacement_scale=disp_scale))
+    tex = FbmTexture(tex_size, seed, octaves, abi.TEXTURE_R16_UNORM)
+    span_x, span_z = patches * sx, patches * sz
+    eye, center = camera_spline(0, 100, span_x, span_z, seed)
+    cam = Camera(eye=eye, center=center, width=width, height=height, near=0.05, far=4 * max(span_x, span_z))
+    host = HostScene([geo], insts, [tex], cam, tess_rate_pixels, "large")
+    host.spline = (span_x, span_z, seed)
+    return host
+
+
+def camera_spline(frame: int, frames: int, span_x: float, span_z: float, seed: int = 5):
+    """Eye and look-at point of frame `frame` of `frames` on a closed
+    Catmull-Rom spline through 6 seeded control points above the field
+    (heights 3..8, looking 12 units ahead along the path, slightly down)."""
+    rng = np.random.default_rng(seed)
+    ctrl = np.stack([rng.uniform(0.15, 0.85, 6) * span_x, rng.uniform(3.0, 8.0, 6), rng.uniform(0.15, 0.85, 6) * span_z],
+                    1)
+
+    def at(u):
+        u = u % 1.0
+        k = u * len(ctrl)
+        i = int(math.floor(k))
+        t = k - i
+        p0, p1, p2, p3 = (ctrl[(i + d) % len(ctrl)] for d in (-1, 0, 1, 2))
+        return 0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t +
+                      (-p0 + 3 * p1 - 3 * p2 + p3) * t * t * t)
+
+    u = frame / float(frames)
+    eye = at(u)
+    ahead = at(u + 0.02)
+    d = ahead - eye
+    d[1] = 0.0
+    n = np.linalg.norm(d)
+    d = d / n if n > 0 else np.array([0.0, 0.0, 1.0])
+    center = eye + 12.0 * d
+    center[1] = 0.0
+    return tuple(float(x) for x in eye), tuple(float(x) for x in center)
