@@ -994,6 +994,16 @@ void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config cfg, int transie
   }
 }
 
+/* Zeroes the 16-byte padding after the group boxes, triangle words and
+ * vertices of a CLAS block. */
+static void zero_block_padding(char* dst, uint32_t nt, uint32_t nv, uint32_t ng) {
+  auto r16 = [](uint32_t v) { return (v + 15u) & ~15u; };
+  uint32_t g1 = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u), t1 = g1 + r16(nt * 4u);
+  memset(dst + sizeof(tc_clas_header) + ng * 24u, 0, g1 - (sizeof(tc_clas_header) + ng * 24u));
+  memset(dst + g1 + nt * 4u, 0, t1 - (g1 + nt * 4u));
+  memset(dst + t1 + nv * 12u, 0, r16(nv * 12u) - nv * 12u);
+}
+
 /* Our CLAS build (stands for the NV template instantiation,
  * renderer_raytrace_clusters_tess.cpp:595-623): header, group boxes of
  * TC_CLAS_GROUP_TRIANGLES triangles, triangle words and vertex copy. */
@@ -1036,6 +1046,7 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
                                                                : 0xFFFFFFFFu;
     h._pad[0] = h._pad[1] = h._pad[2] = 0;
     memcpy(dst, &h, sizeof(h));
+    zero_block_padding(dst, nt, nv, ng);
     P<uint32_t>(b->tempClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
   }
 }
@@ -1088,6 +1099,7 @@ void oracle_clas_build_transient(const oracle_ctx* c, tc_config cfg) {
     h.vertexBase = info.vertexBuffer >= b->genVertices ? (uint32_t)((info.vertexBuffer - b->genVertices) / 12u) : 0xFFFFFFFFu;
     h._pad[0] = h._pad[1] = h._pad[2] = 0;
     memcpy(dst, &h, sizeof(h));
+    zero_block_padding(dst, nt, nv, ng);
     P<uint32_t>(b->transClusterSizes)[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
   }
 }

@@ -179,6 +179,18 @@ def test_graph_replays_match_eager_frames():
         rh, rd = g.trace(64, 36, abi.TRACE_CULL_BACK)
     finally:
         g.close()
+    ce, cr = eager.arrays["clasBlocks"], replay.arrays["clasBlocks"]
+    if ce.shape == cr.shape and not np.array_equal(ce, cr):
+        # name the first differing block and the part of it that differs
+        off = 0
+        for i, sz in enumerate(eager.arrays["tempClusterSizes"]):
+            a, b = ce[off:off + int(sz)], cr[off:off + int(sz)]
+            if not np.array_equal(a, b):
+                d = np.nonzero(a != b)[0]
+                print("block", i, "of", len(eager.arrays["tempClusterSizes"]), "size", int(sz), "differs at bytes",
+                      d[:8], "count", d.size, "headers", a[:48].view(abi.CLAS_HEADER_DT), b[:48].view(abi.CLAS_HEADER_DT))
+                break
+            off += int(sz)
     harness.compare(replay, eager, max_ulp=0)
     assert np.array_equal(rh.view(np.uint8), eh.view(np.uint8))
 

@@ -66,6 +66,19 @@ __device__ __forceinline__ uint32_t finish_block(char* dst, const float* s_v, co
     lo[c] = wave_min(lo[c]);
     hi[c] = wave_max(hi[c]);
   }
+  // zero the 16-byte padding after each part so blocks are deterministic
+  {
+    const uint32_t g0 = (uint32_t)sizeof(tc_clas_header) + ng * 24u, g1 = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u);
+    const uint32_t t0 = g1 + nt * 4u, t1 = g1 + r16(nt * 4u);
+    const uint32_t v0 = t1 + nv * 12u, v1 = t1 + r16(nv * 12u);
+    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
+    const uint32_t npad = (g1 - g0) / 4 + (t1 - t0) / 4 + (v1 - v0) / 4;
+    for (uint32_t k = lane; k < npad; k += kWave) {
+      const uint32_t a = (g1 - g0) / 4, b = a + (t1 - t0) / 4;
+      const uint32_t byte = k < a ? g0 + 4 * k : (k < b ? t0 + 4 * (k - a) : v0 + 4 * (k - b));
+      w[byte / 4] = 0u;
+    }
+  }
   if (lane == 0) {
     tc_clas_header h;
     h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
