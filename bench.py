@@ -31,6 +31,23 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
+# stages that are one memory-bound kernel, and that kernel
+ROOFLINE_KERNELS = {"cluster_classify": "k_cluster_classify", "triangle_tess_template_instantiate": "k_instantiate",
+                    "clas_instantiate": "k_clas_instantiate"}
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary profiles/<kernel>_<workload>_traffic.txt (FETCH_SIZE and
+    WRITE_SIZE passes scaled by a copy kernel of known size), or None."""
+    path = ROOT / "profiles" / ("%s_%s_traffic.txt" % (kernel, workload))
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch="):
+            return int(line.split("=", 1)[1])
+    return None
+
 
 def bench_config(abi):
     return abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
@@ -132,9 +149,11 @@ def staged_frame(r, fc, stream, timer, trace_params=None):
     timer.mark("triangle_tess_template_instantiate")
     abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_BUILD_BLAS, s))
     abi.check(L.tc_clas_instantiate(b, cfg, s))
+    abi.check(L.tc_clas_build_transient(b, cfg, s))
     timer.mark("clas_instantiate")
     abi.check(L.tc_blas_setup_insertion(b, cfg, s))
     abi.check(L.tc_blas_clusters_insert(b, cfg, 0, s))
+    abi.check(L.tc_blas_clusters_insert(b, cfg, 1, s))
     timer.mark("blas_insert")
     accel, xf = r.accel()
     abi.check(L.tc_accel_build(b, cfg, C.byref(accel), C.c_void_p(xf), s))
@@ -176,9 +195,10 @@ def algorithmic_bytes(r, host, cfg):
     instantiate = int((24 + 96 + 3 + 16 + 44) * nparts + 12 * nv_parts.sum() + min(tex_bytes, 8 * int(nv_parts.sum())))
     clas = int((32 + 32 + 4 + 8) * ntemp + sizes.sum() + 12 * sb.genVertexCounter + 4 * (nt_parts.sum()))
     insert = int(ntemp * (4 + 8 + 4 + 8) + 2 * 12 * ntemp * 2)
-    # keys from CLAS headers, 5 radix passes of (8 B key + 4 B value) read and
-    # written, hierarchy + refit writing (N - 1) nodes of 64 B + 24 B boxes
-    accel = int(ntemp * (32 + 12) + 5 * 2 * 12 * ntemp + ntemp * (64 + 24 + 16 + 8))
+    # keys from CLAS headers (48 B), 5 radix passes of (8 B key + 4 B value)
+    # read and written, hierarchy + refit writing (N - 1) binary nodes of
+    # 64 B + 24 B boxes and the 128 B four-wide records
+    accel = int(ntemp * (48 + 12) + 5 * 2 * 12 * ntemp + ntemp * (64 + 24 + 128 + 16 + 8))
     return {"cluster_classify": classify, "triangle_split": split, "triangle_tess_template_instantiate": instantiate,
             "clas_instantiate": clas, "blas_insert": insert, "blas_tlas_build": accel}, rb, sb
 
@@ -285,6 +305,8 @@ def main():
     ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
                     help="pick launch shapes by measurement first (default for batched500)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after timing, copy a buffer of this size once (scale reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
@@ -360,10 +382,13 @@ def main():
     n_shadow = int((dec_np["instanceID"] != 0xFFFFFFFF).sum())
     bvh_ms = stage_ms.get("clas_instantiate", 0) + stage_ms.get("blas_insert", 0) + stage_ms.get("blas_tlas_build", 0)
     trace_ms = stage_ms.get("trace_rays", 0.0)
-    dom = max(alg, key=lambda k: stage_ms.get(k, 0.0))
+    # roofline kernel: the slowest stage that is one HBM-bound kernel
+    # (traversal is latency-bound and reported as Mrays/s)
+    dom = max(ROOFLINE_KERNELS, key=lambda k: stage_ms.get(k, 0.0))
     dom_ms = stage_ms[dom]
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
     frame_bytes = sum(alg.values())
+    traffic = profiled_traffic(ROOFLINE_KERNELS[dom], args.workload)
 
     out = {
         "metric": "displaced tris/s, cluster-BVH build ms/frame, Mrays/s, HBM GB/s (1 GPU)",
@@ -388,8 +413,8 @@ def main():
         "mrays_per_s": (n_primary + n_shadow) / (trace_ms * 1e-3) / 1e6 if trace_ms else None,
         "rays_per_frame": {"primary": n_primary, "shadow": n_shadow},
         "frame_hbm_gbs": frame_bytes / (ms * 1e-3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": alg[dom],
+        "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": alg[dom],
                      "launch_ms": dom_ms},
     }
     if args.workload == "batched500":
@@ -408,6 +433,13 @@ def main():
         batched = b0.elapsed_time(b1) / args.steps
         pm = per_mesh_ms(torch, ds, cfg, fc, stream, args.per_mesh_reps)
         out["batched_vs_per_mesh"] = {"batched_graph_ms": batched, "per_mesh_ms": pm, "speedup": pm / batched}
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = src.numel() * 4
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_sample_instances, args.cpu_budget_s)
     print(json.dumps(out))
