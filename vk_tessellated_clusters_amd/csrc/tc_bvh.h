@@ -20,6 +20,8 @@ struct BvhArgs {
   uint32_t* parent;    // [0, capacity) internal nodes, [capacity, 2 capacity) leaves
   uint32_t* counter;   // refit arrival counters
   uint32_t* rangeEnd;  // per internal node: the far end of its key range (global position)
+  uint32_t* pendingCount;  // per instance: straddling nodes listed by the chunk refit
+  uint32_t* pendingList;   // their node indices, per instance from (first >> 10) + instance
   tc_bvh_node* nodes;  // blasNodes
   tc_bvh4_node* nodes4;  // blasNodes4
   tc_aabb* nodeBox;    // union box of every internal node
@@ -40,7 +42,7 @@ struct BvhArgs {
 };
 
 struct AccelLayout {
-  uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter, rangeEnd;
+  uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter, rangeEnd, pendingCount, pendingList;
   uint64_t blasNodes4, tlasNodes4;
   uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasRoot;
   uint64_t total;

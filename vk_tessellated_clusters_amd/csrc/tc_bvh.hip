@@ -145,6 +145,8 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     a.counter[k] = 0;
     if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity, a.rangeEnd);
   }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.numInstances; i += gridDim.x * blockDim.x)
+    a.pendingCount[i] = 0;
 }
 
 __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[3], float hi[3]) {
@@ -152,14 +154,51 @@ __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[
   for (int c = 0; c < 3; c++) lo[c] = h.lo[c], hi[c] = h.hi[c];
 }
 
-// Bottom-up refit: every leaf climbs, the second arrival at a node writes
-// it.  A workgroup owns a chunk of kRefitChunk sorted positions; nodes whose
+// Bottom-up refit in two launches, with no device-scope fence.
+//
+// k_blas_refit: a workgroup owns a chunk of kRefitChunk sorted positions.
+// Every leaf climbs; the second arrival at a node writes it.  Nodes whose
 // key range lies inside the chunk keep their arrival counter and box in
-// LDS (workgroup-scope ordering), the ancestors that straddle chunks use
-// global counters with device-scope fences.  Ranges nest, so a climb
-// switches from the local to the global form at most once.
-constexpr uint32_t kRefitChunk = 1024;
+// LDS (workgroup-scope ordering).  A climb stops at the first node whose
+// range straddles chunks; the second arrival there lists the node as
+// pending for its instance (both children are then final).
+//
+// k_blas_refit_top: one workgroup per instance climbs from its pending
+// nodes through the straddling nodes with global counters (continuing the
+// counts of the first launch) and workgroup-scope ordering, since every
+// climber of an instance is in that workgroup.  A pending node's range
+// holds exactly one chunk boundary (its split), so an instance has fewer
+// pending nodes than chunks it spans; its list starts at
+// (first >> kRefitChunkBits) + instance.
+constexpr uint32_t kRefitChunkBits = 10;
+constexpr uint32_t kRefitChunk = 1u << kRefitChunkBits;
 constexpr uint32_t kRefitThreads = 256;
+
+__device__ __forceinline__ void refit_node(const BvhArgs& a, uint32_t node, int64_t first, const tc_aabb* localBoxes,
+                                           uint32_t c0, tc_aabb& self) {
+  tc_bvh_node out;
+  float lo[2][3], hi[2][3];
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = a.child[2 * node + c];
+    if (ch & TC_BVH_LEAF) {
+      leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
+    } else {
+      // a local node's children are local to the same chunk
+      const uint32_t g = (uint32_t)first + ch;
+      const tc_aabb bx = localBoxes ? localBoxes[g - c0] : a.nodeBox[g];
+      for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+    }
+  }
+  for (int q = 0; q < 3; q++) {
+    out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
+  }
+  out.child0 = a.child[2 * node + 0];
+  out.child1 = a.child[2 * node + 1];
+  out._pad0 = out._pad1 = 0;
+  a.nodes[node] = out;
+  for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
+  a.nodeBox[node] = self;  // read by straddling parents and the TLAS
+}
 
 __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
   __shared__ uint32_t s_cnt[kRefitChunk];
@@ -178,45 +217,50 @@ __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
       uint32_t node = a.parent[a.capacity + k];
       while (true) {
         const uint32_t far = a.rangeEnd[node];
-        const bool local = node >= c0 && node < c1 && far >= c0 && far < c1;
-        if (local) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-          if (atomicAdd(&s_cnt[node - c0], 1u) == 0) break;  // first arrival stops
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        } else {
-          __threadfence();
-          if (atomicAdd(&a.counter[node], 1u) == 0) break;
-          __threadfence();
-        }
-        tc_bvh_node out;
-        float lo[2][3], hi[2][3];
-        for (int c = 0; c < 2; c++) {
-          const uint32_t ch = a.child[2 * node + c];
-          if (ch & TC_BVH_LEAF) {
-            leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
-          } else {
-            // a local node's children are local to the same chunk
-            const uint32_t g = (uint32_t)first + ch;
-            const tc_aabb bx = local ? s_box[g - c0] : a.nodeBox[g];
-            for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+        if (!(node >= c0 && node < c1 && far >= c0 && far < c1)) {
+          // straddling: the second arrival lists it for the second launch
+          if (atomicAdd(&a.counter[node], 1u) == 1u) {
+            const uint32_t slot = atomicAdd(&a.pendingCount[inst], 1u);
+            a.pendingList[((uint32_t)first >> kRefitChunkBits) + inst + slot] = node;
           }
+          break;
         }
-        for (int q = 0; q < 3; q++) {
-          out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
-        }
-        out.child0 = a.child[2 * node + 0];
-        out.child1 = a.child[2 * node + 1];
-        out._pad0 = out._pad1 = 0;
-        a.nodes[node] = out;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&s_cnt[node - c0], 1u) == 0) break;  // first arrival stops
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         tc_aabb self;
-        for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
-        a.nodeBox[node] = self;  // read by straddling parents and the TLAS
-        if (local) s_box[node - c0] = self;
+        refit_node(a, node, first, s_box, c0, self);
+        s_box[node - c0] = self;
         if ((int64_t)node == first) break;  // instance root
         node = a.parent[node];
       }
     }
     __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kRefitThreads) void k_blas_refit_top(BvhArgs a) {
+  const uint32_t inst = blockIdx.x;
+  if (inst >= a.numInstances) return;
+  const tc_scene_building* b = a.build;
+  const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  const tc_blas_build_info& bi = ptr<const tc_blas_build_info>(b->blasBuildInfos)[inst];
+  if (bi.clusterReferencesCount < 2) return;
+  const int64_t first = (int64_t)((bi.clusterReferences - b->blasClusterAddresses) / 8);
+  if (first >= (int64_t)n) return;
+  const uint32_t count = a.pendingCount[inst];
+  const uint32_t base = ((uint32_t)first >> kRefitChunkBits) + inst;
+  for (uint32_t t = threadIdx.x; t < count; t += kRefitThreads) {
+    uint32_t node = a.pendingList[base + t];
+    while (true) {
+      tc_aabb self;
+      refit_node(a, node, first, nullptr, 0, self);
+      if ((int64_t)node == first) break;
+      node = a.parent[node];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (atomicAdd(&a.counter[node], 1u) == 0) break;  // first arrival stops
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
   }
 }
 
@@ -415,6 +459,8 @@ AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances) {
   put(L.parent, 2 * c * 4);
   put(L.counter, c * 4);
   put(L.rangeEnd, c * 4);
+  put(L.pendingCount, n * 4);
+  put(L.pendingList, (c / kRefitChunk + n + 2) * 4);
   put(L.tlasInstances, n * sizeof(tc_tlas_instance));
   put(L.tlasNodes, n * sizeof(tc_bvh_node));
   put(L.tlasNodes4, n * sizeof(tc_bvh4_node));
@@ -492,6 +538,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   a.parent = reinterpret_cast<uint32_t*>(m + L.parent);
   a.counter = reinterpret_cast<uint32_t*>(m + L.counter);
   a.rangeEnd = reinterpret_cast<uint32_t*>(m + L.rangeEnd);
+  a.pendingCount = reinterpret_cast<uint32_t*>(m + L.pendingCount);
+  a.pendingList = reinterpret_cast<uint32_t*>(m + L.pendingList);
   a.nodes = reinterpret_cast<tc_bvh_node*>(m + L.blasNodes);
   a.nodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.blasNodes4);
   a.nodeBox = reinterpret_cast<tc_aabb*>(m + L.nodeBox);
@@ -517,6 +565,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   hipLaunchKernelGGL(k_blas_hierarchy, dim3(grid), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_blas_refit, dim3(std::min<uint32_t>((capacity + kRefitChunk - 1) / kRefitChunk, (uint32_t)num_cu_public() * 4)),
                      dim3(kRefitThreads), 0, st, a);
+  hipLaunchKernelGGL(k_blas_refit_top, dim3(a.numInstances), dim3(kRefitThreads), 0, st, a);
   hipLaunchKernelGGL(k_blas_collapse4, dim3(grid), dim3(256), 0, st, a);
   // TLAS over instances; its sort reuses the other ping-pong slot
   a.tlasKeys = sb.keys[res ^ 1];
