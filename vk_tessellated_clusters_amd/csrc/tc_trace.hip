@@ -377,8 +377,10 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   out.depth = pp[2] / pp[3];
 }
 
+// Five waves per SIMD (at most 96 VGPRs): the traversal is latency-bound
+// and gains more from occupancy than it loses to the two spilled registers.
 template <uint32_t T>
-__global__ __launch_bounds__(T) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace(TraceArgs a) {
   __shared__ uint32_t s_stack[kLdsStack * T];
   RayStack<T> stack;
   stack.lds = s_stack + threadIdx.x;
