@@ -119,9 +119,16 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
   const tc_aabb* groups = reinterpret_cast<const tc_aabb*>(clas + sizeof(tc_clas_header));
   const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
   const float* verts = clas_vertices(clas, nt, ng);
+  // group boxes first (ng <= 32), then the triangles of the groups hit, so
+  // the lanes of a wave run the triangle loop together
+  uint32_t mask = 0;
   for (uint32_t g = 0; g < ng; g++) {
     const tc_aabb gb = groups[g];
-    if (box_hit(r, gb.lo, gb.hi, tmin, best.h.t) == INFINITY) continue;
+    if (box_hit(r, gb.lo, gb.hi, tmin, best.h.t) != INFINITY) mask |= 1u << g;
+  }
+  while (mask) {
+    const uint32_t g = __builtin_ctz(mask);
+    mask &= mask - 1;
     const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
     for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
       const uint32_t w = tris[t];
@@ -219,70 +226,67 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
   uint32_t inst = 0xFFFFFFFFu;
   const tc_bvh4_node* blas4 = nullptr;
   const uint64_t* blasLeaves = nullptr;
-  while (true) {
-    const tc_bvh4_node* node;
-    uint32_t tag;
-    if (!(e & kBlasEntry)) {
-      if (inst != 0xFFFFFFFFu) {
-        ray = world;
-        inst = 0xFFFFFFFFu;
-      }
-      if (e & TC_BVH_LEAF) {
-        const uint32_t i = e & 0x3FFFFFFFu;
-        const tc_tlas_instance& ti = a.tlasInstances[i];
-        const uint32_t nc = ti.numClusters;
-        if (nc != 0) {
-          float ol[3], dl[3];
-          xform34(ti.worldToObject, o, 1.0f, ol);
-          xform34(ti.worldToObject, d, 0.0f, dl);
-          setup_ray(ray, ol, dl);
-          inst = i;
-          blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
-          blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
-          e = kBlasEntry | ti.blasRoot;
+  constexpr uint32_t kDone = 0xFFFFFFFFu;
+  // while-while: every lane walks nodes until it holds a CLAS leaf (or has
+  // finished), then the lanes with leaves intersect them together
+  while (e != kDone) {
+    while (true) {
+      if (!(e & kBlasEntry)) {
+        if (inst != 0xFFFFFFFFu) {
+          ray = world;
+          inst = 0xFFFFFFFFu;
+        }
+        if (e & TC_BVH_LEAF) {
+          const uint32_t i = e & 0x3FFFFFFFu;
+          const tc_tlas_instance& ti = a.tlasInstances[i];
+          if (ti.numClusters != 0) {
+            float ol[3], dl[3];
+            xform34(ti.worldToObject, o, 1.0f, ol);
+            xform34(ti.worldToObject, d, 0.0f, dl);
+            setup_ray(ray, ol, dl);
+            inst = i;
+            blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
+            blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+            e = kBlasEntry | ti.blasRoot;
+            continue;
+          }
+          e = stack.sp ? stack.pop() : kDone;
+          if (e == kDone) break;
           continue;
         }
-        if (stack.sp == 0) break;
-        e = stack.pop();
-        continue;
+      } else if (e & TC_BVH_LEAF) {
+        break;  // CLAS leaf
       }
-      node = a.tlasNodes4 + e;
-      tag = 0;
-    } else {
-      const uint32_t be = e & ~kBlasEntry;
-      if (be & TC_BVH_LEAF) {
-        const char* clas = reinterpret_cast<const char*>(blasLeaves[be & ~TC_BVH_LEAF]);
-        if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
-        if (stack.sp == 0) break;
-        e = stack.pop();
-        continue;
-      }
-      node = blas4 + be;
-      tag = kBlasEntry;
-    }
-    float lo[3][4], hi[3][4], t[4];
-    uint32_t c[4];
-    load_node4(node, lo, hi, c);
+      const tc_bvh4_node* node = (e & kBlasEntry) ? blas4 + (e & ~kBlasEntry) : a.tlasNodes4 + e;
+      const uint32_t tag = e & kBlasEntry;
+      float lo[3][4], hi[3][4], t[4];
+      uint32_t c[4];
+      load_node4(node, lo, hi, c);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
-      t[k] = c[k] == TC_BVH4_EMPTY ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
-      c[k] |= tag;
+      for (int k = 0; k < 4; k++) {
+        const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
+        t[k] = c[k] == TC_BVH4_EMPTY ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
+        c[k] |= tag;
+      }
+      cswap(t, c, 0, 1);
+      cswap(t, c, 2, 3);
+      cswap(t, c, 0, 2);
+      cswap(t, c, 1, 3);
+      cswap(t, c, 1, 2);
+      if (t[0] == INFINITY) {
+        e = stack.sp ? stack.pop() : kDone;
+        if (e == kDone) break;
+        continue;
+      }
+      if (t[3] != INFINITY) stack.push(c[3]);
+      if (t[2] != INFINITY) stack.push(c[2]);
+      if (t[1] != INFINITY) stack.push(c[1]);
+      e = c[0];
     }
-    cswap(t, c, 0, 1);
-    cswap(t, c, 2, 3);
-    cswap(t, c, 0, 2);
-    cswap(t, c, 1, 3);
-    cswap(t, c, 1, 2);
-    if (t[0] == INFINITY) {
-      if (stack.sp == 0) break;
-      e = stack.pop();
-      continue;
-    }
-    if (t[3] != INFINITY) stack.push(c[3]);
-    if (t[2] != INFINITY) stack.push(c[2]);
-    if (t[1] != INFINITY) stack.push(c[1]);
-    e = c[0];
+    if (e == kDone) break;
+    const char* clas = reinterpret_cast<const char*>(blasLeaves[e & ~(kBlasEntry | TC_BVH_LEAF)]);
+    if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
+    e = stack.sp ? stack.pop() : kDone;
   }
   return best.h.instanceID != 0xFFFFFFFFu;
 }
