@@ -1,16 +1,20 @@
 // tc_classify.hip -- cluster_classify.comp.glsl:150-666 on gfx950.
 //
-// A workgroup of four waves owns a tile of 64 visible clusters:
-//   1. each wave classifies clusters of the tile one at a time (vertices
-//      staged in LDS, one triangle per lane) and leaves a 16-bit code per
-//      triangle in LDS: split flag plus three 4-bit factors (split factors
-//      for triangles going to the split list);
+// A workgroup of four waves owns a tile of 64 visible clusters and works
+// across the whole tile at each step, so memory latency overlaps over many
+// clusters instead of being paid cluster by cluster:
+//   0. one lane per cluster loads the cluster and instance records into LDS;
+//   1. the 256 threads run flat over (cluster, triangle) pairs: factors from
+//      world-space vertices (read through L1/L2), a 16-bit code per triangle
+//      in LDS (split flag plus three 4-bit factors, split factors for
+//      triangles going to the split list) and per-cluster counts;
 //   2. wave 0 scans the 64 clusters' counts, takes the tile's offsets with
 //      one decoupled look-back and makes the allocations of the full
 //      clusters (template instantiations) and, with TC_FLAG_1X_TRANSIENT,
 //      of the clusters whose simple triangles form a transient CLAS;
-//   3. the waves emit split / part records and displaced full-cluster
-//      vertices at those offsets.
+//   3. the 256 threads run flat over the displaced vertices of the accepted
+//      clusters; each wave then emits the split / part records (and the
+//      transient index and mapping bytes) of its share of the clusters.
 // Offsets are those of a serial run in (visible cluster, triangle) order.
 #include "tc_stage.h"
 
@@ -38,10 +42,21 @@ __device__ __forceinline__ uint32_t pack_code(bool split, const uint32_t f[3]) {
   return (split ? 0x8000u : 0u) | f[0] | (f[1] << 4) | (f[2] << 8);
 }
 
+struct ClusterHead {
+  tc_cluster_info ci;
+  uint32_t nv, nt;
+  uint32_t firstLocalVertex, firstLocalTriangle;
+  uint32_t invisible;
+};
+
 __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
-  __shared__ float s_pos[kClassifyWaves][kMaxClusterVerts * 3];
   __shared__ uint16_t s_code[kClassifyTile][kMaxClusterTris];
   __shared__ ClusterSlot s_cl[kClassifyTile];
+  __shared__ ClusterHead s_h[kClassifyTile];
+  __shared__ float s_m[kClassifyTile][16];  // instance world matrices
+  __shared__ uint32_t s_simple[kClassifyTile], s_ns[kClassifyTile], s_np[kClassifyTile];
+  __shared__ uint32_t s_vstart[kClassifyTile], s_vcluster[kClassifyTile];
+  __shared__ uint32_t s_vtotal;
   __shared__ uint32_t s_tile;
 
   WsHeader* h = a.ws;
@@ -52,7 +67,6 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
   const ViewParams vp = view_params(a);
   tc_scene_building* b = a.build;
   const tc_config cfg = a.cfg;
-  float* pos = s_pos[wave];
 
   while (true) {
     if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -62,80 +76,92 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
     const uint32_t tileBase = tile * kClassifyTile;
     const uint32_t cnt = min(kClassifyTile, numItems - tileBase);
 
-    // ---- 1. factors and counts (:177-258)
-    for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
+    // ---- 0. cluster and instance records (:150-176)
+    if (threadIdx.x < cnt) {
+      const uint32_t k = threadIdx.x;
       tc_cluster_info ci = ptr<const tc_cluster_info>(b->visibleClusters)[tileBase + k];
       if (ci.instanceID >= a.numInstances) ci.instanceID = 0;
       const tc_render_instance& inst = a.instances[ci.instanceID];
       if (ci.clusterID >= inst.numClusters) ci.clusterID = 0;
       const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ci.clusterID];
       const uint32_t state = ptr<const uint32_t>(b->instanceStates)[ci.instanceID];
-      const uint32_t nv = min((uint32_t)cl.numVertices, kMaxClusterVerts);
-      const uint32_t nt = min((uint32_t)cl.numTriangles, kMaxClusterTris);
-      const bool invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
-      uint32_t simple = 0, ns = 0, np = 0;
-      if (invisible) {
-        simple = nt;  // :203-211
-      } else {
-        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
-        for (uint32_t v = lane; v < nv; v += kWave) {
-          v3 p = ld3(inst.positions, cl.firstLocalVertex + v);
-          pos[v * 3 + 0] = p.x;
-          pos[v * 3 + 1] = p.y;
-          pos[v * 3 + 2] = p.z;
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        float m[16];
+      ClusterHead hd;
+      hd.ci = ci;
+      hd.nv = min((uint32_t)cl.numVertices, kMaxClusterVerts);
+      hd.nt = min((uint32_t)cl.numTriangles, kMaxClusterTris);
+      hd.firstLocalVertex = cl.firstLocalVertex;
+      hd.firstLocalTriangle = cl.firstLocalTriangle;
+      hd.invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
+      s_h[k] = hd;
 #pragma unroll
-        for (int i = 0; i < 16; i++) m[i] = inst.worldMatrix[i];
-        for (uint32_t t0 = 0; t0 < nt; t0 += kWave) {
-          const uint32_t t = t0 + lane;
-          const bool valid = t < nt;
-          uint32_t f[3] = {1, 1, 1};
-          if (valid) {
-            const uint32_t i0 = lt[t * 3 + 0], i1 = lt[t * 3 + 1], i2 = lt[t * 3 + 2];
-            v3 w0 = xform_point(m, mk(pos[i0 * 3], pos[i0 * 3 + 1], pos[i0 * 3 + 2]));
-            v3 w1 = xform_point(m, mk(pos[i1 * 3], pos[i1 * 3 + 1], pos[i1 * 3 + 2]));
-            v3 w2 = xform_point(m, mk(pos[i2 * 3], pos[i2 * 3 + 1], pos[i2 * 3 + 2]));
-            tess_factors(w0, w1, w2, vp, f);
-          }
-          const uint32_t mx = umax3(f[0], f[1], f[2]);
-          const bool split = valid && mx > TC_TESSTABLE_SIZE;
-          if (split) {
-            f[0] = split_factor(f[0], cfg.maxSplitFactor);
-            f[1] = split_factor(f[1], cfg.maxSplitFactor);
-            f[2] = split_factor(f[2], cfg.maxSplitFactor);
-          }
-          if (valid) s_code[k][t] = (uint16_t)pack_code(split, f);
-          simple += __popcll(ballot(valid && mx == 1));
-          ns += __popcll(ballot(split));
-          np += __popcll(ballot(valid && !split));
-        }
-        __builtin_amdgcn_wave_barrier();
+      for (int i = 0; i < 16; i++) s_m[k][i] = inst.worldMatrix[i];
+      s_simple[k] = hd.invisible ? hd.nt : 0u;  // :203-211 invisible: all 1x1x1
+      s_ns[k] = 0;
+      s_np[k] = 0;
+    }
+    __syncthreads();
+
+    // ---- 1. factors and counts, flat over (cluster, triangle) (:177-258).
+    // A wave covers 64 triangles of one cluster.
+    for (uint32_t idx = threadIdx.x; idx < cnt * kMaxClusterTris; idx += kClassifyWaves * kWave) {
+      const uint32_t k = idx / kMaxClusterTris, t = idx % kMaxClusterTris;
+      const ClusterHead& hd = s_h[k];
+      const bool valid = t < hd.nt;
+      uint32_t f[3] = {1, 1, 1};
+      if (valid && !hd.invisible) {
+        const tc_render_instance& inst = a.instances[hd.ci.instanceID];
+        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + hd.firstLocalTriangle + 3 * t;
+        const uint32_t i0 = lt[0], i1 = lt[1], i2 = lt[2];
+        const v3 w0 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i0));
+        const v3 w1 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i1));
+        const v3 w2 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i2));
+        tess_factors(w0, w1, w2, vp, f);
       }
-      if (invisible)
-        for (uint32_t t = lane; t < nt; t += kWave) s_code[k][t] = (uint16_t)kSimpleCode;
-      if (lane == 0) {
-        ClusterSlot s;
-        s.ci = ci;
-        s.nv = nv;
-        s.nt = nt;
-        s.firstLocalVertex = cl.firstLocalVertex;
-        s.simple = simple;
-        s.full = simple == nt;
-        // :261-276 transientSimpleThreshold = 1
-        s.trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) && !s.full && simple > 1;
-        s.ns = s.full ? 0 : ns;
-        s.np = s.full ? 0 : (s.trans ? np - simple : np);
-        s.dataSize = s.full ? ptr<const uint32_t>(inst.clusterTemplateInstantiatonSizes)[ci.clusterID]
-                            : (s.trans ? clas_block_size(simple, nv) : 0u);
-        s.vertexSize = s.full ? nv : (s.trans ? nv + (simple * 3 + 11) / 12 : 0u);
-        s.partSize = s.trans ? (8 + simple + 24 - 1) / 24 : 0u;
-        s_cl[k] = s;
+      const uint32_t mx = umax3(f[0], f[1], f[2]);
+      const bool split = valid && mx > TC_TESSTABLE_SIZE;
+      if (split) {
+        f[0] = split_factor(f[0], cfg.maxSplitFactor);
+        f[1] = split_factor(f[1], cfg.maxSplitFactor);
+        f[2] = split_factor(f[2], cfg.maxSplitFactor);
+      }
+      if (valid) s_code[k][t] = (uint16_t)pack_code(split, f);
+      if (!hd.invisible) {
+        const uint32_t simple = __popcll(ballot(valid && mx == 1));
+        const uint32_t ns = __popcll(ballot(split));
+        const uint32_t np = __popcll(ballot(valid && !split));
+        if (lane == 0) {
+          if (simple) atomicAdd(&s_simple[k], simple);
+          if (ns) atomicAdd(&s_ns[k], ns);
+          if (np) atomicAdd(&s_np[k], np);
+        }
       }
     }
     __syncthreads();
+
+    // ---- 1b. per-cluster classification (wave 0, one lane per cluster)
+    if (wave == 0 && lane < cnt) {
+      const uint32_t k = lane;
+      const ClusterHead hd = s_h[k];
+      const uint32_t simple = s_simple[k], ns = s_ns[k], np = s_np[k];
+      ClusterSlot s;
+      s.ci = hd.ci;
+      s.nv = hd.nv;
+      s.nt = hd.nt;
+      s.firstLocalVertex = hd.firstLocalVertex;
+      s.simple = simple;
+      s.full = simple == hd.nt;
+      // :261-276 transientSimpleThreshold = 1
+      s.trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) && !s.full && simple > 1;
+      s.ns = s.full ? 0 : ns;
+      s.np = s.full ? 0 : (s.trans ? np - simple : np);
+      s.dataSize = s.full ? ptr<const uint32_t>(a.instances[hd.ci.instanceID].clusterTemplateInstantiatonSizes)[hd.ci.clusterID]
+                          : (s.trans ? clas_block_size(simple, hd.nv) : 0u);
+      s.vertexSize = s.full ? hd.nv : (s.trans ? hd.nv + (simple * 3 + 11) / 12 : 0u);
+      s.partSize = s.trans ? (8 + simple + 24 - 1) / 24 : 0u;
+      s_cl[k] = s;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 
     // ---- 2. offsets: scan over the tile's clusters + look-back (wave 0)
     if (wave == 0) {
@@ -216,6 +242,21 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
           }
         }
       }
+      // accepted clusters, compacted, with their first vertex in the tile's
+      // flat vertex range (phase 3)
+      {
+        const uint64_t vm = ballot(ok);
+        const uint32_t m = mask_prefix(vm);
+        const uint32_t nvOk = ok ? s.nv : 0u;
+        const uint32_t vincl = wave_incl_scan(nvOk);
+        if (ok) {
+          s_vstart[m] = vincl - nvOk;
+          s_vcluster[m] = k;
+        }
+        const uint32_t nOk = __popcll(vm);
+        if (lane >= nOk) s_vstart[lane] = 0xFFFFFFFFu;
+        if (lane == kWave - 1) s_vtotal = vincl;
+      }
       // :302, :442-444 statistics and BLAS reference counts, aggregated
       add_blas_reference_counts(ptr<tc_blas_build_info>(b->blasBuildInfos), valid ? s.ci.instanceID : 0u, ok ? 1u : 0u, ok);
       const uint32_t nFull = __popcll(ballot(cf != 0));
@@ -238,87 +279,90 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
     }
     __syncthreads();
 
-    // ---- 3. emission
-    for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
-      const ClusterSlot s = s_cl[k];
-      if (s.ok) {
-        // :459-488 displaced cluster vertices
+    // ---- 3a. displaced vertices of the accepted clusters, flat (:459-488)
+    {
+      const uint32_t total = s_vtotal;
+      for (uint32_t v = threadIdx.x; v < total; v += kClassifyWaves * kWave) {
+        const uint32_t m = find_owner(s_vstart, v);
+        const ClusterSlot& s = s_cl[s_vcluster[m]];
+        const uint32_t local = v - s_vstart[m];
         const tc_render_instance& inst = a.instances[s.ci.instanceID];
         const bool disp = has_disp(a, inst.displacementIndex);
-        tc_displacement_texture tex;
-        if (disp) tex = a.textures[inst.displacementIndex];
-        for (uint32_t v = lane; v < s.nv; v += kWave) {
-          const uint32_t vi = s.firstLocalVertex + v;
-          v3 p = ld3(inst.positions, vi);
-          if (a.tape) write_tape(a, (uint64_t)s.vertexOffset + v, p, disp ? normalize(ld3(inst.normals, vi)) : mk(0.f, 0.f, 0.f),
-                                 disp ? ld2(inst.texcoords, vi) : v2{0.f, 0.f}, inst.displacementScale,
-                                 inst.displacementOffset, disp ? (uint32_t)inst.displacementIndex : ~0u);
-          if (disp) {
-            const float hgt = displaced_height(a, tex, inst.displacementScale, inst.displacementOffset, ld2(inst.texcoords, vi));
-            p = add(p, mul(normalize(ld3(inst.normals, vi)), hgt));
-          }
-          st3(b->genVertices, (uint64_t)s.vertexOffset + v, p);
+        const uint32_t vi = s.firstLocalVertex + local;
+        v3 p = ld3(inst.positions, vi);
+        if (a.tape) write_tape(a, (uint64_t)s.vertexOffset + local, p, disp ? normalize(ld3(inst.normals, vi)) : mk(0.f, 0.f, 0.f),
+                               disp ? ld2(inst.texcoords, vi) : v2{0.f, 0.f}, inst.displacementScale,
+                               inst.displacementOffset, disp ? (uint32_t)inst.displacementIndex : ~0u);
+        if (disp) {
+          const tc_displacement_texture tex = a.textures[inst.displacementIndex];
+          const float hgt = displaced_height(a, tex, inst.displacementScale, inst.displacementOffset, ld2(inst.texcoords, vi));
+          p = add(p, mul(normalize(ld3(inst.normals, vi)), hgt));
         }
-        if (s.trans) {
-          // :497-534 triangle mapping bytes after the ClusterInfo of the part
-          // slot, u8 indices after the vertices, in triangle order
-          const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[s.ci.clusterID];
-          const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
-          uint8_t* map = ptr<uint8_t>(b->partTriangles) + (uint64_t)s.partOffset * 24ull + 8ull;
-          uint8_t* idx = ptr<uint8_t>(b->genVertices) + ((uint64_t)s.vertexOffset + s.nv) * 12ull;
-          uint32_t run = 0;
-          for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
-            const uint32_t t = t0 + lane;
-            const bool simple = t < s.nt && s_code[k][t] == kSimpleCode;
-            const uint64_t vm = ballot(simple);
-            const uint32_t o = run + mask_prefix(vm);
-            run += __popcll(vm);
-            if (simple) {
-              map[o] = (uint8_t)t;
-              idx[o * 3 + 0] = lt[t * 3 + 0];
-              idx[o * 3 + 1] = lt[t * 3 + 1];
-              idx[o * 3 + 2] = lt[t * 3 + 2];
-            }
-          }
-          // zero the rest of the index slots so the vertex stream is deterministic
-          for (uint32_t j = s.simple * 3 + lane; j < (s.vertexSize - s.nv) * 12; j += kWave) idx[j] = 0;
-        }
+        st3(b->genVertices, (uint64_t)s.vertexOffset + local, p);
       }
-      if (!s.full) {
-        // :543-666 split / part records in triangle order (the simple
-        // triangles of a transient cluster are in its CLAS)
-        uint32_t runSplit = s.offSplit, runPart = s.offPart;
+    }
+
+    // ---- 3b. per cluster: transient bytes, split / part records
+    for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
+      const ClusterSlot s = s_cl[k];
+      if (s.full) continue;
+      const tc_render_instance& inst = a.instances[s.ci.instanceID];
+      if (s.ok && s.trans) {
+        // :497-534 triangle mapping bytes after the ClusterInfo of the part
+        // slot, u8 indices after the vertices, in triangle order
+        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + s_h[k].firstLocalTriangle;
+        uint8_t* map = ptr<uint8_t>(b->partTriangles) + (uint64_t)s.partOffset * 24ull + 8ull;
+        uint8_t* idx = ptr<uint8_t>(b->genVertices) + ((uint64_t)s.vertexOffset + s.nv) * 12ull;
+        uint32_t run = 0;
         for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
           const uint32_t t = t0 + lane;
-          const bool valid = t < s.nt && !(s.trans && s_code[k][t] == kSimpleCode);
-          const uint32_t code = valid ? (uint32_t)s_code[k][t] : 0u;
-          const bool split = valid && (code & 0x8000u);
-          const bool part = valid && !split;
-          const uint64_t vs = ballot(split), vpm = ballot(part);
-          const uint32_t offSplit = runSplit + mask_prefix(vs);
-          const uint32_t offPart = runPart + mask_prefix(vpm);
-          runSplit += __popcll(vs);
-          runPart += __popcll(vpm);
-          const bool writeSplit = split && offSplit < cfg.maxSplitTriangles;
-          // build.glsl:68-83 with transient builds: x + y + 2 <= MAX_PART_TRIANGLES
-          const bool writePart = part && ((cfg.flags & TC_FLAG_1X_TRANSIENT)
-                                              ? (uint64_t)offPart + s.yIncl + 2 <= cfg.maxPartTriangles
-                                              : offPart < cfg.maxPartTriangles);
-          if (writeSplit || writePart) {
-            uint32_t f[3] = {code & 15u, (code >> 4) & 15u, (code >> 8) & 15u};
-            uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
-            const uint32_t cf = get_config(f, vtx);
-            tc_tess_triangle_info ti;
-            ti.cluster = s.ci;
-            ti.subTriangle.vtxEncoded[0] = vtx[0];
-            ti.subTriangle.vtxEncoded[1] = vtx[1];
-            ti.subTriangle.vtxEncoded[2] = vtx[2];
-            ti.subTriangle.triangleID_config = t | (cf << 16);
-            if (writeSplit)
-              ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-            else
-              ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
+          const bool simple = t < s.nt && s_code[k][t] == kSimpleCode;
+          const uint64_t vm = ballot(simple);
+          const uint32_t o = run + mask_prefix(vm);
+          run += __popcll(vm);
+          if (simple) {
+            map[o] = (uint8_t)t;
+            idx[o * 3 + 0] = lt[t * 3 + 0];
+            idx[o * 3 + 1] = lt[t * 3 + 1];
+            idx[o * 3 + 2] = lt[t * 3 + 2];
           }
+        }
+        // zero the rest of the index slots so the vertex stream is deterministic
+        for (uint32_t j = s.simple * 3 + lane; j < (s.vertexSize - s.nv) * 12; j += kWave) idx[j] = 0;
+      }
+      // :543-666 split / part records in triangle order (the simple
+      // triangles of a transient cluster are in its CLAS)
+      uint32_t runSplit = s.offSplit, runPart = s.offPart;
+      for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        const bool valid = t < s.nt && !(s.trans && s_code[k][t] == kSimpleCode);
+        const uint32_t code = valid ? (uint32_t)s_code[k][t] : 0u;
+        const bool split = valid && (code & 0x8000u);
+        const bool part = valid && !split;
+        const uint64_t vs = ballot(split), vpm = ballot(part);
+        const uint32_t offSplit = runSplit + mask_prefix(vs);
+        const uint32_t offPart = runPart + mask_prefix(vpm);
+        runSplit += __popcll(vs);
+        runPart += __popcll(vpm);
+        const bool writeSplit = split && offSplit < cfg.maxSplitTriangles;
+        // build.glsl:68-83 with transient builds: x + y + 2 <= MAX_PART_TRIANGLES
+        const bool writePart = part && ((cfg.flags & TC_FLAG_1X_TRANSIENT)
+                                            ? (uint64_t)offPart + s.yIncl + 2 <= cfg.maxPartTriangles
+                                            : offPart < cfg.maxPartTriangles);
+        if (writeSplit || writePart) {
+          uint32_t f[3] = {code & 15u, (code >> 4) & 15u, (code >> 8) & 15u};
+          uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
+          const uint32_t cf = get_config(f, vtx);
+          tc_tess_triangle_info ti;
+          ti.cluster = s.ci;
+          ti.subTriangle.vtxEncoded[0] = vtx[0];
+          ti.subTriangle.vtxEncoded[1] = vtx[1];
+          ti.subTriangle.vtxEncoded[2] = vtx[2];
+          ti.subTriangle.triangleID_config = t | (cf << 16);
+          if (writeSplit)
+            ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
+          else
+            ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
         }
       }
     }
