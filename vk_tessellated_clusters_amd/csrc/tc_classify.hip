@@ -47,6 +47,12 @@ struct ClusterHead {
   uint32_t nv, nt;
   uint32_t firstLocalVertex, firstLocalTriangle;
   uint32_t invisible;
+  uint32_t disp;  // displaced (texture index valid)
+  // instance fields the flat phases read, so their loads do not chase the
+  // instance record
+  uint64_t positions, normals, texcoords, localTriangles;
+  float dispScale, dispOffset;
+  tc_displacement_texture tex;
 };
 
 __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
@@ -92,6 +98,14 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       hd.firstLocalVertex = cl.firstLocalVertex;
       hd.firstLocalTriangle = cl.firstLocalTriangle;
       hd.invisible = (cfg.flags & TC_FLAG_CULLING) && !(state & TC_INSTANCE_VISIBLE_BIT);
+      hd.disp = has_disp(a, inst.displacementIndex);
+      hd.positions = inst.positions;
+      hd.normals = inst.normals;
+      hd.texcoords = inst.texcoords;
+      hd.localTriangles = inst.clusterLocalTriangles;
+      hd.dispScale = inst.displacementScale;
+      hd.dispOffset = inst.displacementOffset;
+      if (hd.disp) hd.tex = a.textures[inst.displacementIndex];
       s_h[k] = hd;
 #pragma unroll
       for (int i = 0; i < 16; i++) s_m[k][i] = inst.worldMatrix[i];
@@ -109,12 +123,11 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       const bool valid = t < hd.nt;
       uint32_t f[3] = {1, 1, 1};
       if (valid && !hd.invisible) {
-        const tc_render_instance& inst = a.instances[hd.ci.instanceID];
-        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + hd.firstLocalTriangle + 3 * t;
+        const uint8_t* lt = ptr<const uint8_t>(hd.localTriangles) + hd.firstLocalTriangle + 3 * t;
         const uint32_t i0 = lt[0], i1 = lt[1], i2 = lt[2];
-        const v3 w0 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i0));
-        const v3 w1 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i1));
-        const v3 w2 = xform_point(s_m[k], ld3(inst.positions, hd.firstLocalVertex + i2));
+        const v3 w0 = xform_point(s_m[k], ld3(hd.positions, hd.firstLocalVertex + i0));
+        const v3 w1 = xform_point(s_m[k], ld3(hd.positions, hd.firstLocalVertex + i1));
+        const v3 w2 = xform_point(s_m[k], ld3(hd.positions, hd.firstLocalVertex + i2));
         tess_factors(w0, w1, w2, vp, f);
       }
       const uint32_t mx = umax3(f[0], f[1], f[2]);
@@ -284,21 +297,22 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       const uint32_t total = s_vtotal;
       for (uint32_t v = threadIdx.x; v < total; v += kClassifyWaves * kWave) {
         const uint32_t m = find_owner(s_vstart, v);
-        const ClusterSlot& s = s_cl[s_vcluster[m]];
+        const uint32_t k = s_vcluster[m];
+        const ClusterHead& hd = s_h[k];
         const uint32_t local = v - s_vstart[m];
-        const tc_render_instance& inst = a.instances[s.ci.instanceID];
-        const bool disp = has_disp(a, inst.displacementIndex);
-        const uint32_t vi = s.firstLocalVertex + local;
-        v3 p = ld3(inst.positions, vi);
-        if (a.tape) write_tape(a, (uint64_t)s.vertexOffset + local, p, disp ? normalize(ld3(inst.normals, vi)) : mk(0.f, 0.f, 0.f),
-                               disp ? ld2(inst.texcoords, vi) : v2{0.f, 0.f}, inst.displacementScale,
-                               inst.displacementOffset, disp ? (uint32_t)inst.displacementIndex : ~0u);
-        if (disp) {
-          const tc_displacement_texture tex = a.textures[inst.displacementIndex];
-          const float hgt = displaced_height(a, tex, inst.displacementScale, inst.displacementOffset, ld2(inst.texcoords, vi));
-          p = add(p, mul(normalize(ld3(inst.normals, vi)), hgt));
+        const uint64_t out = (uint64_t)s_cl[k].vertexOffset + local;
+        const uint32_t vi = hd.firstLocalVertex + local;
+        v3 p = ld3(hd.positions, vi);
+        if (hd.disp) {
+          const v3 n = normalize(ld3(hd.normals, vi));
+          const v2 uv = ld2(hd.texcoords, vi);
+          if (a.tape) write_tape(a, out, p, n, uv, hd.dispScale, hd.dispOffset, (uint32_t)a.instances[hd.ci.instanceID].displacementIndex);
+          const float hgt = displaced_height(a, hd.tex, hd.dispScale, hd.dispOffset, uv);
+          p = add(p, mul(n, hgt));
+        } else if (a.tape) {
+          write_tape(a, out, p, mk(0.f, 0.f, 0.f), v2{0.f, 0.f}, hd.dispScale, hd.dispOffset, ~0u);
         }
-        st3(b->genVertices, (uint64_t)s.vertexOffset + local, p);
+        st3(b->genVertices, out, p);
       }
     }
 
@@ -306,11 +320,10 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
     for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
       const ClusterSlot s = s_cl[k];
       if (s.full) continue;
-      const tc_render_instance& inst = a.instances[s.ci.instanceID];
       if (s.ok && s.trans) {
         // :497-534 triangle mapping bytes after the ClusterInfo of the part
         // slot, u8 indices after the vertices, in triangle order
-        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + s_h[k].firstLocalTriangle;
+        const uint8_t* lt = ptr<const uint8_t>(s_h[k].localTriangles) + s_h[k].firstLocalTriangle;
         uint8_t* map = ptr<uint8_t>(b->partTriangles) + (uint64_t)s.partOffset * 24ull + 8ull;
         uint8_t* idx = ptr<uint8_t>(b->genVertices) + ((uint64_t)s.vertexOffset + s.nv) * 12ull;
         uint32_t run = 0;
