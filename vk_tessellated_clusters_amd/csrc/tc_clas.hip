@@ -96,40 +96,150 @@ __device__ __forceinline__ uint32_t vertex_base(const tc_scene_building* b, uint
   return vbuf >= b->genVertices ? (uint32_t)((vbuf - b->genVertices) / 12u) : 0xFFFFFFFFu;
 }
 
-// One wave per CLAS, grid-stride over tempInstantiateCounter.
-__global__ __launch_bounds__(64) void k_clas_instantiate(KArgs a) {
-  __shared__ float s_v[kMaxClusterVerts * 3];
-  __shared__ uint32_t s_t[kMaxClusterTris];
+// Small templates (at most kSubTris triangles and kSubVerts vertices: the
+// tessellated parts, a few triangles each) are built by 16-lane subgroups,
+// four per wave; larger ones (whole clusters) by the whole wave.  Both
+// paths write the same bytes (box min / max are exact in any order).
+constexpr uint32_t kSub = 16;
+constexpr uint32_t kSubTris = 16;
+constexpr uint32_t kSubVerts = 16;
+constexpr uint32_t kClasWaves = 4;
+
+__device__ __forceinline__ float sub_min(float v) {
+#pragma unroll
+  for (int d = 1; d < (int)kSub; d <<= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+__device__ __forceinline__ float sub_max(float v) {
+#pragma unroll
+  for (int d = 1; d < (int)kSub; d <<= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+// Each wave takes 4 consecutive records per step (one per subgroup),
+// grid-stride over tempInstantiateCounter.
+__global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a) {
+  __shared__ float s_vall[kClasWaves][kMaxClusterVerts * 3];
+  __shared__ uint32_t s_tall[kClasWaves][kMaxClusterTris];
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->tempInstantiateCounter, a.cfg.maxGeneratedClusters);
-  const uint32_t lane = lane_id();
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint32_t sub = lane / kSub, sl = lane % kSub;
+  float* s_v = s_vall[wave];
+  uint32_t* s_t = s_tall[wave];
+  // a subgroup stages its small block in its quarter of the wave's LDS
+  float* s_sv = s_v + sub * (kSubVerts * 3);
+  uint32_t* s_st = s_t + sub * kSubTris;
   const tc_template_instantiate_info* infos = ptr<const tc_template_instantiate_info>(b->tempInstantiations);
   const uint64_t* dsts = ptr<const uint64_t>(b->tempClusterAddresses);
   uint32_t* sizes = ptr<uint32_t>(b->tempClusterSizes);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const tc_template_instantiate_info info = infos[i];
-    const tc_clas_template T = *ptr<const tc_clas_template>(info.clusterTemplateAddress);
-    const uint32_t nt = T.numTriangles, nv = T.numVertices, ng = T.numGroups;
-    char* dst = ptr<char>(dsts[i]);
-    uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
-    float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
-    const char* src = ptr<const char>(info.vertexBufferAddress);
-    for (uint32_t k = lane; k < nv * 3; k += kWave) {
-      const uint32_t v = k / 3, c = k - v * 3;
-      const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
-      s_v[k] = f;
-      verts[k] = f;
+  const uint32_t perStep = kWave / kSub;
+  for (uint32_t base = (blockIdx.x * kClasWaves + wave) * perStep; base < n; base += gridDim.x * kClasWaves * perStep) {
+    const uint32_t i = base + sub;
+    const bool valid = i < n;
+    tc_template_instantiate_info info;
+    tc_clas_template T;
+    bool small = false;
+    if (valid) {
+      info = infos[i];
+      T = *ptr<const tc_clas_template>(info.clusterTemplateAddress);
+      small = T.numTriangles <= kSubTris && T.numVertices <= kSubVerts;
     }
-    for (uint32_t t = lane; t < nt; t += kWave) {
-      const uint32_t w = ptr<const uint32_t>(T.triangles)[t];
-      s_t[t] = w;
-      tris[t] = w;
+    const uint32_t nt = small ? T.numTriangles : 0u, nv = small ? T.numVertices : 0u;
+    const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
+    char* dst = small ? ptr<char>(dsts[i]) : nullptr;
+    const char* src = small ? ptr<const char>(info.vertexBufferAddress) : nullptr;
+    if (small) {
+      uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
+      float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
+      for (uint32_t k = sl; k < nv * 3; k += kSub) {
+        const uint32_t v = k / 3, c = k - v * 3;
+        const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
+        s_sv[k] = f;
+        verts[k] = f;
+      }
+      for (uint32_t t = sl; t < nt; t += kSub) {
+        const uint32_t w = ptr<const uint32_t>(T.triangles)[t];
+        s_st[t] = w;
+        tris[t] = w;
+      }
     }
-    __syncthreads();
-    const uint32_t size = finish_block(dst, s_v, s_t, nt, nv, T.clusterID + info.clusterIdOffset,
-                                       vertex_base(b, info.vertexBufferAddress));
-    if (lane == 0) sizes[i] = size;
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (small && sl < ng) {
+      float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      const uint32_t t1 = min(nt, (sl + 1) * TC_CLAS_GROUP_TRIANGLES);
+      for (uint32_t t = sl * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
+        const uint32_t w = s_st[t];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const uint32_t vi = (w >> (8 * k)) & 0xFF;
+#pragma unroll
+          for (int c = 0; c < 3; c++) glo[c] = fminf(glo[c], s_sv[vi * 3 + c]), ghi[c] = fmaxf(ghi[c], s_sv[vi * 3 + c]);
+        }
+      }
+      tc_aabb box;
+#pragma unroll
+      for (int c = 0; c < 3; c++) box.lo[c] = glo[c], box.hi[c] = ghi[c], lo[c] = glo[c], hi[c] = ghi[c];
+      reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header))[sl] = box;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) lo[c] = sub_min(lo[c]), hi[c] = sub_max(hi[c]);
+    if (small) {
+      // zero the 16-byte padding after each part
+      uint32_t* w32 = reinterpret_cast<uint32_t*>(dst);
+      const uint32_t g0 = (uint32_t)sizeof(tc_clas_header) + ng * 24u, g1 = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u);
+      const uint32_t t0 = g1 + nt * 4u, t1 = g1 + r16(nt * 4u), v0 = t1 + nv * 12u, v1 = t1 + r16(nv * 12u);
+      const uint32_t na = (g1 - g0) / 4, nb = na + (t1 - t0) / 4, nc = nb + (v1 - v0) / 4;
+      for (uint32_t q = sl; q < nc; q += kSub) w32[(q < na ? g0 + 4 * q : (q < nb ? t0 + 4 * (q - na) : v0 + 4 * (q - nb))) / 4] = 0u;
+      if (sl == 0) {
+        tc_clas_header h;
+        h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
+        h.hi[0] = hi[0], h.hi[1] = hi[1], h.hi[2] = hi[2];
+        h.clusterID = T.clusterID + info.clusterIdOffset;
+        h.counts = nt | (nv << 8) | (ng << 16);
+        h.vertexBase = vertex_base(b, info.vertexBufferAddress);
+        h._pad[0] = h._pad[1] = h._pad[2] = 0;
+        *reinterpret_cast<tc_clas_header*>(dst) = h;
+        sizes[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // large templates: the whole wave, one at a time
+    uint64_t big = ballot(valid && !small && sl == 0);
+    while (big) {
+      const uint32_t ij = base + (uint32_t)__builtin_ctzll(big) / kSub;
+      big &= big - 1;
+      const tc_template_instantiate_info bi = infos[ij];
+      const tc_clas_template BT = *ptr<const tc_clas_template>(bi.clusterTemplateAddress);
+      const uint32_t bnt = BT.numTriangles, bnv = BT.numVertices, bng = BT.numGroups;
+      char* bdst = ptr<char>(dsts[ij]);
+      uint32_t* tris = reinterpret_cast<uint32_t*>(bdst + sizeof(tc_clas_header) + r16(bng * 24u));
+      float* verts = reinterpret_cast<float*>(bdst + sizeof(tc_clas_header) + r16(bng * 24u) + r16(bnt * 4u));
+      const char* bsrc = ptr<const char>(bi.vertexBufferAddress);
+      for (uint32_t k = lane; k < bnv * 3; k += kWave) {
+        const uint32_t v = k / 3, c = k - v * 3;
+        const float f = reinterpret_cast<const float*>(bsrc + (uint64_t)v * bi.vertexBufferStride)[c];
+        s_v[k] = f;
+        verts[k] = f;
+      }
+      for (uint32_t t = lane; t < bnt; t += kWave) {
+        const uint32_t w = ptr<const uint32_t>(BT.triangles)[t];
+        s_t[t] = w;
+        tris[t] = w;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint32_t size = finish_block(bdst, s_v, s_t, bnt, bnv, BT.clusterID + bi.clusterIdOffset,
+                                         vertex_base(b, bi.vertexBufferAddress));
+      if (lane == 0) sizes[ij] = size;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -183,8 +293,10 @@ extern "C" tc_status tc_clas_instantiate(const tc_bindings* b, tc_config config,
   KArgs a;
   tc_status s = make_args_public(b, config, a);
   if (s != TC_OK) return s;
-  uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
-  hipLaunchKernelGGL(k_clas_instantiate, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
+  const uint32_t waves = std::min<uint32_t>((config.maxGeneratedClusters + 3) / 4,
+                                            (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
+  const uint32_t grid = (waves + kClasWaves - 1) / kClasWaves;
+  hipLaunchKernelGGL(k_clas_instantiate, dim3(grid), dim3(kClasWaves * kWave), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 
