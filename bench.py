@@ -49,10 +49,13 @@ def profiled_traffic(kernel, workload):
     return None
 
 
-def bench_config(abi):
-    return abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
-                              maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
-                              maxGeneratedVertices=1 << 25)
+def bench_config(abi, transient_1x=False):
+    c = abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
+                           maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
+                           maxGeneratedVertices=1 << 25)
+    if transient_1x:
+        c.flags |= abi.FLAG_1X_TRANSIENT
+    return c
 
 
 def make_scene(name):
@@ -305,6 +308,8 @@ def main():
     ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
                     help="pick launch shapes by measurement first (default for batched500)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--transient-1x", action="store_true",
+                    help="1X transient cluster builds (the reference renderer's default, renderer.hpp:47)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy a buffer of this size once (scale reference for PMC byte counters)")
     args = ap.parse_args()
@@ -321,7 +326,7 @@ def main():
     from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
 
     host, desc = make_scene(args.workload)
-    cfg = bench_config(abi)
+    cfg = bench_config(abi, args.transient_1x)
     ds = DeviceScene(host)
     r = Renderer(ds, cfg)
     fc = host.frame()
@@ -403,10 +408,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc, "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
+        "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else ""),
+                   "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
                    "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),
                    "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
-                   "clas_per_frame": int(sb.tempInstantiateCounter), "gen_vertices": int(sb.genVertexCounter)},
+                   "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if args.transient_1x else 0)),
+                   "transient_clas": int(sb.transBuildCounter if args.transient_1x else 0),
+                   "gen_vertices": int(sb.genVertexCounter)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "tuning": dict(autotune.get_tuning(), autotuned=bool(autotuned)),
         "cluster_bvh_build_ms_per_frame": bvh_ms,
