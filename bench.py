@@ -189,7 +189,7 @@ def algorithmic_bytes(r, host, cfg):
     nv_parts = lk["numVertices"][cfgs].astype(np.int64)
     nt_parts = lk["numTriangles"][cfgs].astype(np.int64)
     sizes = r.copy(sb.tempClusterSizes, 4 * ntemp).view(np.uint32).astype(np.int64)
-    tex_bytes = sum(a.nbytes for a, _ in host.textures)
+    tex_bytes = sum(t.nbytes if hasattr(t, "seed") else t[0].nbytes for t in host.textures)
     full_verts = int(sb.genVertexCounter - nv_parts.sum())
     n_split_items = sb.splitWriteCounter
     classify = int((8 + 16 + 4 + 12 * V + 3 * T).sum() + (12 + 8 + 12) * full_verts + 44 * int(full_mask.sum())
