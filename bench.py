@@ -49,10 +49,15 @@ def profiled_traffic(kernel, workload):
     return None
 
 
-def bench_config(abi, transient_1x=False):
-    c = abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
-                           maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
-                           maxGeneratedVertices=1 << 25)
+def bench_config(abi, transient_1x=False, large=False):
+    if large:
+        c = abi.default_config(maxVisibleClusters=1 << 18, maxPartTriangles=1 << 23, maxSplitTriangles=1 << 20,
+                               maxGeneratedClusters=1 << 23, maxGeneratedClusterMegs=12288,
+                               maxGeneratedVertices=1 << 27)
+    else:
+        c = abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
+                               maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
+                               maxGeneratedVertices=1 << 25)
     if transient_1x:
         c.flags |= abi.FLAG_1X_TRANSIENT
     return c
@@ -67,7 +72,24 @@ def make_scene(name):
     if name == "batched500":
         return scene.scene_batched(), ("500 meshes log-uniform 200..200000 tris (seed 2, 14.0M tris), grids + "
                                        "icospheres, own 1024^2 R16 fBm each, 1920x1080 @ 4px")
+    if name == "large":
+        return scene.scene_large(), ("8x8 patches of 512x256 quads (16.8M tris), 4096^2 R16 fBm, 3840x2160 @ 2px, "
+                                     "camera on a seeded spline over 100 frames")
     raise SystemExit("unknown workload " + name)
+
+
+def frame_sequence(host, n=100):
+    """Frame constants of every frame: the spline path for the large
+    workload, else the scene's one camera."""
+    from vk_tessellated_clusters_amd import scene
+    if not hasattr(host, "spline"):
+        return [host.frame()]
+    span_x, span_z, seed = host.spline
+    out = []
+    for k in range(n):
+        host.camera.eye, host.camera.center = scene.camera_spline(k, n, span_x, span_z, seed)
+        out.append(host.frame())
+    return out
 
 
 def per_mesh_ms(torch, ds, cfg, fc, stream, reps):
@@ -326,10 +348,11 @@ def main():
     from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
 
     host, desc = make_scene(args.workload)
-    cfg = bench_config(abi, args.transient_1x)
+    cfg = bench_config(abi, args.transient_1x, args.workload == "large")
     ds = DeviceScene(host)
     r = Renderer(ds, cfg)
-    fc = host.frame()
+    frames = frame_sequence(host)
+    fc = frames[0]
     # a created stream (default priority): graphs cannot be captured on the
     # null stream
     stream = torch.cuda.Stream()
@@ -342,8 +365,8 @@ def main():
     tp.width, tp.height, tp.flags = W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW
     tp.hits, tp.decoded = hits.data_ptr(), dec.data_ptr()
 
-    def frame():
-        r.tessellate(fc, stream)
+    def frame(k=0):
+        r.tessellate(frames[k % len(frames)], stream)
         r.build_accel(stream)
         r.trace(tp, stream)
 
@@ -358,18 +381,29 @@ def main():
         abi.check(r.L.tc_renderer_graph_capture(r.handle, C.byref(fc), C.byref(tp), C.c_void_p(stream.cuda_stream)),
                   "graph capture")
 
-        def frame():  # noqa: F811
-            abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)), "graph launch")
+        def frame(k=0):  # noqa: F811
+            abi.check(r.L.tc_renderer_graph_launch(r.handle, C.byref(frames[k % len(frames)]),
+                                                   C.c_void_p(stream.cuda_stream)), "graph launch")
 
         frame()
         torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for _ in range(args.steps):
-        frame()
+    for k in range(args.steps):
+        frame(k)
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
+    # displaced triangles of every timed frame (the frames are deterministic;
+    # counted in an untimed pass when the camera moves)
+    if len(frames) > 1:
+        frame_tris = []
+        for k in range(args.steps):
+            r.tessellate(frames[k % len(frames)], stream)
+            frame_tris.append(int(r.readback().numTotalTriangles))
+        mean_tris = sum(frame_tris) / len(frame_tris)
+    else:
+        frame_tris = None
 
     # stage-by-stage frames for the stage times and the roofline kernel
     stage_ms = {}
@@ -380,7 +414,7 @@ def main():
         for k, v in t.durations().items():
             stage_ms[k] = stage_ms.get(k, 0.0) + v / nstage
     alg, rb, sb = algorithmic_bytes(r, host, cfg)
-    tris = rb.numTotalTriangles
+    tris = rb.numTotalTriangles if frame_tris is None else mean_tris
     torch.cuda.synchronize()
     dec_np = dec.cpu().numpy().view(abi.HIT_DECODED_DT)
     n_primary = W * H
@@ -410,11 +444,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else ""),
                    "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
+                   "frames_on_path": len(frames),
                    "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),
                    "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
                    "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if args.transient_1x else 0)),
                    "transient_clas": int(sb.transBuildCounter if args.transient_1x else 0),
                    "gen_vertices": int(sb.genVertexCounter)},
+        "capacity_ok": bool(rb.numVisibleClusters <= cfg.maxVisibleClusters and
+                            rb.numPartTriangles <= cfg.maxPartTriangles and
+                            rb.numSplitTriangles <= cfg.maxSplitTriangles and
+                            rb.numGenVertices <= cfg.maxGeneratedVertices and
+                            rb.numBlasClusters <= cfg.maxGeneratedClusters and
+                            rb.numGenDatas <= cfg.maxGeneratedClusterMegs * (1 << 20)),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "tuning": dict(autotune.get_tuning(), autotuned=bool(autotuned)),
         "cluster_bvh_build_ms_per_frame": bvh_ms,
