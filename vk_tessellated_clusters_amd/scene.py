@@ -425,8 +425,8 @@ def scene_batched(num_meshes=500, seed=2, tex_size=1024, width=1920, height=1080
 # ------------------------------------------------------- large 4K frame
 
 
-def scene_large(patches=8, quads_x=512, quads_z=256, quad_size=1.0 / 16.0, tex_size=4096, seed=5, octaves=6,
-                width=3840, height=2160, tess_rate_pixels=2.0, disp_scale=1.5):
+def scene_large(patches=8, quads_x=512, quads_z=256, quad_size=0.25, tex_size=4096, seed=5, octaves=6,
+                width=3840, height=2160, tess_rate_pixels=2.0, disp_scale=4.0):
     """patches x patches instances of a quads_x x quads_z-quad grid patch
     (512 x 256 quads: 262,144 triangles each, 16.8M base triangles for the
     8 x 8 field) on a square lattice, one tex_size^2 R16 fBm height map,
