@@ -691,6 +691,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
 #define TC_TRACE_SHADOW (1u << 1)    /* one shadow ray toward skyParams.sunDirection per hit */
 #define TC_TRACE_PRIVATE_STACK (1u << 2) /* keep the whole traversal stack in private memory
                                             (slower; exercises the overflow part of the stack) */
+#define TC_TRACE_PACKETS (1u << 3) /* wave-wide packet traversal (shared node stack); same results,
+                                      faster for coherent rays */
 
 /* Rays are either explicit (rays != NULL, numRays) or the primary rays of
  * the frame camera, one per pixel of width x height (rgen:52-86).  Hits are

@@ -132,12 +132,16 @@ def test_trace_parity_primary_and_shadow():
     try:
         g.run(host.frame(), stages="accel")
         gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
-        # the same traversal with the whole stack in private memory
+        # the same traversal with the whole stack in private memory, and the
+        # wave-wide packet traversal
         ph, pd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW | abi.TRACE_PRIVATE_STACK)
+        kh, kd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW | abi.TRACE_PACKETS)
     finally:
         g.close()
     assert np.array_equal(ph.view(np.uint8), gh.view(np.uint8))
     assert np.array_equal(pd.view(np.uint8), gd.view(np.uint8))
+    assert np.array_equal(kh.view(np.uint8), gh.view(np.uint8))
+    assert np.array_equal(kd.view(np.uint8), gd.view(np.uint8))
     assert (oh["t"] > 0).mean() > 0.3, "camera must see geometry"
     agree = compare_hits(gh, oh, gd, od)
     print("hit agreement", agree, "hits", (oh["t"] > 0).sum(), "shadowed", od["shadowed"].sum())
@@ -179,3 +183,30 @@ def test_trace_parity_transient_1x():
     assert ((oh["clusterID"] >> 30) == abi.RT_CLUSTER_MODE_1X_SUBSET_CLUSTER).sum() > 20
     agree = compare_hits(gh, oh, gd, od)
     print("hit agreement", agree)
+
+
+def test_packet_traversal_matches_per_ray():
+    """Packet traversal gives the per-ray traversal's hits byte for byte,
+    for camera rays (ragged tiles) and explicit rays in any directions."""
+    host = scene.scene_icosphere()
+    cfg = harness.small_config()
+    g = harness.GpuFrame(host, cfg)
+    rng = np.random.default_rng(7)
+    rays = np.zeros(5000, abi.RAY_DT)
+    rays["origin"] = rng.normal(0, 0.3, (5000, 3)) + np.array([0.0, 0.0, 3.0])
+    dirs = rng.normal(size=(5000, 3)) + np.array([0.0, 0.0, -2.0])
+    rays["direction"] = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays["tMin"], rays["tMax"] = 0.0, 100.0
+    try:
+        g.run(host.frame(), stages="accel")
+        for W, H in ((100, 60), (64, 64)):
+            a = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+            b = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW | abi.TRACE_PACKETS)
+            assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
+            assert np.array_equal(a[1].view(np.uint8), b[1].view(np.uint8))
+        a = g.trace(0, 0, 0, rays=rays)
+        b = g.trace(0, 0, abi.TRACE_PACKETS, rays=rays)
+        assert (a[0]["t"] > 0).sum() > 500
+        assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
+    finally:
+        g.close()

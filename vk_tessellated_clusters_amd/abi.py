@@ -187,6 +187,7 @@ class TraceParams(C.Structure):
 TRACE_CULL_BACK = 1
 TRACE_SHADOW = 2
 TRACE_PRIVATE_STACK = 4
+TRACE_PACKETS = 8
 
 RAY_DT = np.dtype([("origin", "<f4", (3,)), ("tMin", "<f4"), ("direction", "<f4", (3,)), ("tMax", "<f4")])
 HIT_DT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instanceID", "<u4"), ("clusterID", "<u4"),

@@ -291,6 +291,104 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
   return best.h.instanceID != 0xFFFFFFFFu;
 }
 
+// Wave-wide packet traversal: the 64 lanes of a wave walk one shared node
+// stack (in LDS, uniform entries).  A node is entered when any live lane's
+// ray enters it before that lane's current closest hit, so every lane sees
+// a superset of the nodes its own traversal would visit and finds the same
+// closest hit (ties are broken by key).  Node records are fetched once per
+// wave.  Suited to coherent rays: camera rays in 8 x 8 pixel tiles and their
+// shadow rays.
+constexpr uint32_t kPacketStack = 160;
+
+__device__ __forceinline__ float wave_min_f(float v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+__device__ bool trace_packet(const TraceArgs& a, const float o[3], const float d[3], float tmin, float tmax, bool live,
+                             bool cullBack, bool anyHit, uint32_t* stk, HitState& best) {
+  best.h.t = tmax;
+  best.h.u = best.h.v = 0.0f;
+  best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
+  best.clas = nullptr;
+  best.word = 0;
+  WoopRay world, ray;
+  setup_ray(world, o, d);
+  ray = world;
+  uint32_t sp = 0;
+  uint32_t e = __builtin_amdgcn_readfirstlane(*a.tlasRoot);
+  uint32_t inst = 0xFFFFFFFFu;
+  const tc_bvh4_node* blas4 = nullptr;
+  const uint64_t* blasLeaves = nullptr;
+  bool found = false;
+  while (true) {
+    if (!ballot(live)) break;
+    bool next = false;  // pop the next entry
+    if (!(e & kBlasEntry)) {
+      if (inst != 0xFFFFFFFFu) {
+        ray = world;
+        inst = 0xFFFFFFFFu;
+      }
+      if (e & TC_BVH_LEAF) {
+        const uint32_t i = e & 0x3FFFFFFFu;
+        const tc_tlas_instance& ti = a.tlasInstances[i];
+        if (ti.numClusters != 0) {
+          float ol[3], dl[3];
+          xform34(ti.worldToObject, o, 1.0f, ol);
+          xform34(ti.worldToObject, d, 0.0f, dl);
+          setup_ray(ray, ol, dl);
+          inst = i;
+          blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
+          blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+          e = __builtin_amdgcn_readfirstlane(kBlasEntry | ti.blasRoot);
+          continue;
+        }
+        next = true;
+      }
+    } else if (e & TC_BVH_LEAF) {
+      const char* clas = reinterpret_cast<const char*>(blasLeaves[e & ~(kBlasEntry | TC_BVH_LEAF)]);
+      if (live && clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) {
+        found = true;
+        live = false;  // any-hit: this lane is done
+      }
+      next = true;
+    }
+    if (!next) {
+      const tc_bvh4_node* node = (e & kBlasEntry) ? blas4 + (e & ~kBlasEntry) : a.tlasNodes4 + e;
+      const uint32_t tag = e & kBlasEntry;
+      float lo[3][4], hi[3][4], t[4], tw[4];
+      uint32_t c[4];
+      load_node4(node, lo, hi, c);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
+        t[k] = (c[k] == TC_BVH4_EMPTY || !live) ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
+        tw[k] = wave_min_f(t[k]);  // nearest entry over the packet (uniform)
+        c[k] |= tag;
+      }
+      cswap(tw, c, 0, 1);
+      cswap(tw, c, 2, 3);
+      cswap(tw, c, 0, 2);
+      cswap(tw, c, 1, 3);
+      cswap(tw, c, 1, 2);
+      if (tw[0] == INFINITY) {
+        next = true;
+      } else {
+        if (tw[3] != INFINITY && sp < kPacketStack) stk[sp++] = c[3];
+        if (tw[2] != INFINITY && sp < kPacketStack) stk[sp++] = c[2];
+        if (tw[1] != INFINITY && sp < kPacketStack) stk[sp++] = c[1];
+        e = __builtin_amdgcn_readfirstlane(c[0]);
+      }
+    }
+    if (next) {
+      if (sp == 0) break;
+      e = __builtin_amdgcn_readfirstlane(stk[--sp]);
+    }
+  }
+  return anyHit ? found : best.h.instanceID != 0xFFFFFFFFu;
+}
+
 __device__ __forceinline__ void mul44(const float* m, const float v[4], float out[4]) {
 #pragma unroll
   for (int r = 0; r < 4; r++) out[r] = ((m[r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
@@ -437,6 +535,71 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5, 8))) void 
   }
 }
 
+// Packet form of k_trace: one wave per 64 work items (an 8 x 8 pixel tile
+// for camera rays), shared stack per wave.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_trace_packet(TraceArgs a) {
+  __shared__ uint32_t s_stk[T / kWave][kPacketStack];
+  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
+  uint32_t* stk = s_stk[wave];
+  const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
+  const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
+  const uint32_t tilesX = (a.width + 7) / 8;
+  const uint32_t wavesTotal = gridDim.x * (T / kWave);
+  for (uint32_t base = (blockIdx.x * (T / kWave) + wave) * kWave; base < a.numItems; base += wavesTotal * kWave) {
+    const uint32_t item = base + lane;
+    float o[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f}, tmin = 0.f, tmax = 0.f;
+    uint32_t r = 0xFFFFFFFFu;
+    if (item < a.numItems) {
+      if (a.rays) {
+        r = item;
+        const tc_ray ray = a.rays[r];
+        for (int k = 0; k < 3; k++) o[k] = ray.origin[k], d[k] = ray.direction[k];
+        tmin = ray.tMin;
+        tmax = ray.tMax;
+      } else {
+        const uint32_t tile = item >> 6, in = item & 63;
+        const uint32_t x = (tile % tilesX) * 8 + (in & 7), y = (tile / tilesX) * 8 + (in >> 3);
+        if (x < a.width && y < a.height) {
+          r = y * a.width + x;
+          primary_ray(a.view, x, y, a.width, a.height, o, d);
+          tmin = a.view->nearPlane;
+          tmax = a.view->farPlane;
+        }
+      }
+    }
+    const bool live = r != 0xFFFFFFFFu;
+    HitState hs;
+    trace_packet(a, o, d, tmin, tmax, live, cull, false, stk, hs);
+    tc_hit hit = hs.h;
+    if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
+    tc_hit_decoded dec;
+    if (live) {
+      if (a.hits) a.hits[r] = hit;
+      if (a.hitVertices) {
+        uint4 hv = make_uint4(~0u, ~0u, ~0u, ~0u);
+        if (hs.clas) {
+          const uint32_t vb = reinterpret_cast<const tc_clas_header*>(hs.clas)->vertexBase;
+          hv = make_uint4(vb + (hs.word & 0xFF), vb + ((hs.word >> 8) & 0xFF), vb + ((hs.word >> 16) & 0xFF), 0u);
+        }
+        reinterpret_cast<uint4*>(a.hitVertices)[r] = hv;
+      }
+      if (a.decoded) decode_hit(a, hs, dec);
+    }
+    if (shadows) {
+      // render_shading.glsl:263-274: opaque, terminate on first hit
+      const bool sLive = live && hit.instanceID != 0xFFFFFFFFu;
+      const float* sun = a.view->skyParams.sunDirection;
+      const float sd[3] = {sun[0], sun[1], sun[2]};
+      const float so[3] = {sLive ? dec.wPos[0] : 0.f, sLive ? dec.wPos[1] : 0.f, sLive ? dec.wPos[2] : 0.f};
+      HitState sh;
+      const bool shadowed = trace_packet(a, so, sd, 0.001f, 10000000.0f, sLive, false, true, stk, sh);
+      if (sLive) dec.shadowed = shadowed ? 1u : 0u;
+    }
+    if (live && a.decoded) a.decoded[r] = dec;
+  }
+}
+
 }  // namespace tc
 
 using namespace tc;
@@ -471,6 +634,15 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   const tc_tuning tn = tuning_snapshot();
   const uint32_t T = tn.traceThreads;
   const uint32_t grid = std::min<uint32_t>((a.numItems + T - 1) / T, (uint32_t)num_cu_public() * tn.traceBlocksPerCU);
+  if (a.flags & TC_TRACE_PACKETS) {
+    if (T == 64)
+      hipLaunchKernelGGL(k_trace_packet<64>, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
+    else if (T == 256)
+      hipLaunchKernelGGL(k_trace_packet<256>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+    else
+      hipLaunchKernelGGL(k_trace_packet<128>, dim3(grid), dim3(128), 0, (hipStream_t)stream, a);
+    return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+  }
   if (T == 64)
     hipLaunchKernelGGL(k_trace<64>, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
   else if (T == 256)
