@@ -49,7 +49,7 @@ def profiled_traffic(kernel, workload):
     return None
 
 
-def bench_config(abi, transient_1x=False, large=False):
+def bench_config(abi, transient_1x=False, large=False, animation=False):
     if large:
         c = abi.default_config(maxVisibleClusters=1 << 18, maxPartTriangles=1 << 23, maxSplitTriangles=1 << 20,
                                maxGeneratedClusters=1 << 23, maxGeneratedClusterMegs=12288,
@@ -60,6 +60,8 @@ def bench_config(abi, transient_1x=False, large=False):
                                maxGeneratedVertices=1 << 25)
     if transient_1x:
         c.flags |= abi.FLAG_1X_TRANSIENT
+    if animation:
+        c.flags |= abi.FLAG_ANIMATION
     return c
 
 
@@ -78,17 +80,20 @@ def make_scene(name):
     raise SystemExit("unknown workload " + name)
 
 
-def frame_sequence(host, n=100):
+def frame_sequence(host, n=100, animation=False):
     """Frame constants of every frame: the spline path for the large
-    workload, else the scene's one camera."""
+    workload, else the scene's one camera; with the ripple animation the
+    animation clock runs at half the time of 60 Hz frames
+    (tessellatedclusters.cpp:556)."""
     from vk_tessellated_clusters_amd import scene
-    if not hasattr(host, "spline"):
+    if not hasattr(host, "spline") and not animation:
         return [host.frame()]
-    span_x, span_z, seed = host.spline
     out = []
     for k in range(n):
-        host.camera.eye, host.camera.center = scene.camera_spline(k, n, span_x, span_z, seed)
-        out.append(host.frame())
+        if hasattr(host, "spline"):
+            span_x, span_z, seed = host.spline
+            host.camera.eye, host.camera.center = scene.camera_spline(k, n, span_x, span_z, seed)
+        out.append(host.frame(animation_time=k * 0.5 / 60.0) if animation else host.frame())
     return out
 
 
@@ -332,6 +337,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--transient-1x", action="store_true",
                     help="1X transient cluster builds (the reference renderer's default, renderer.hpp:47)")
+    ap.add_argument("--animation", action="store_true",
+                    help="ripple animation of the generated vertices, clock advancing every frame")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy a buffer of this size once (scale reference for PMC byte counters)")
     args = ap.parse_args()
@@ -348,10 +355,10 @@ def main():
     from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
 
     host, desc = make_scene(args.workload)
-    cfg = bench_config(abi, args.transient_1x, args.workload == "large")
+    cfg = bench_config(abi, args.transient_1x, args.workload == "large", args.animation)
     ds = DeviceScene(host)
     r = Renderer(ds, cfg)
-    frames = frame_sequence(host)
+    frames = frame_sequence(host, animation=args.animation)
     fc = frames[0]
     # a created stream (default priority): graphs cannot be captured on the
     # null stream
@@ -442,7 +449,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else ""),
+        "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else "") +
+                   (", ripple animation" if args.animation else ""),
                    "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
                    "frames_on_path": len(frames),
                    "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),

@@ -285,6 +285,44 @@ inline v3 displace(const oracle_ctx* c, const tc_render_instance& inst, v3 pos, 
   return add(pos, mul(normalize(nrm), h));
 }
 
+/* sin / cos for the ripple: the argument is reduced by multiples of pi/2 in
+ * FP64, then the Taylor series to r^9 (sin) / r^8 (cos) is evaluated in FP32.
+ * The GPU evaluates the same operations in the same order, so both sides
+ * agree bit for bit; the error against the exact function is below 1e-6 for
+ * |x| < 1e4 (pinned against libm in the tests). */
+float ripple_sincos(float x, bool cosine) {
+  const double k = rint((double)x * 0.63661977236758134308);
+  const float r = (float)((double)x - k * 1.57079632679489661923);
+  const long long q = (long long)k + (cosine ? 1 : 0);
+  const float r2 = r * r;
+  const float s = r * (1.0f + r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f)))));
+  const float co = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f))));
+  switch ((int)(q & 3)) {
+    case 0: return s;
+    case 1: return co;
+    case 2: return -s;
+    default: return -co;
+  }
+}
+
+/* displacement.glsl:106-120 rippleDeform, applied after displacement at
+ * cluster_classify.comp.glsl:483-485 and
+ * triangle_tess_template_instantiate.comp.glsl:366-368 */
+v3 ripple_deform(const oracle_ctx* c, v3 p, uint32_t seed, float size) {
+  const tc_frame_constants* view = c->view;
+  float maxCoord = std::max(fabsf(p.x), std::max(fabsf(p.y), fabsf(p.z)));
+  if (maxCoord == 0.0f) return p; /* no direction at the origin: kept, not NaN */
+  float frequency = view->animationRippleFrequency / size;
+  float phase = view->animationState * view->animationRippleSpeed;
+  float fs = (float)seed;
+  v3 wave = mk(ripple_sincos((maxCoord * frequency + fs) + phase, false),
+               ripple_sincos(((maxCoord * frequency) * 3.0f + fs) + phase, true),
+               ripple_sincos(((maxCoord * frequency) * 1.2f + fs) + phase, false));
+  float amp = view->animationRippleAmplitude * size;
+  v3 n = normalize(mk(p.z, p.y, p.x));
+  return add(p, mk(n.x * (amp * wave.x), n.y * (amp * wave.y), n.z * (amp * wave.z)));
+}
+
 inline bool has_disp(const oracle_ctx* c, tc_config cfg, const tc_render_instance& inst) {
   return (cfg.flags & TC_FLAG_DISPLACEMENT) && inst.displacementIndex >= 0 &&
          (uint32_t)inst.displacementIndex < c->numDisplacementTextures;
@@ -426,6 +464,7 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
             uint32_t vi = cl.firstLocalVertex + v;
             p = displace(c, inst, p, ld3(inst.normals, vi), ld2(inst.texcoords, vi));
           }
+          if (cfg.flags & TC_FLAG_ANIMATION) p = ripple_deform(c, p, ci.instanceID, inst.geoHi[3]);
           float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + v);
           o[0] = p.x, o[1] = p.y, o[2] = p.z;
         }
@@ -597,6 +636,7 @@ void instantiate_part(const oracle_ctx* c, tc_config cfg, uint32_t p, uint32_t g
     v3 w = interp(bb, cfg_vertex_bary(cf, vert));
     v3 pos = pn ? get_pn(basePN, w) : interp(bp, w);
     if (disp) pos = displace(c, inst, pos, interp(bn, w), interp2(bt, w));
+    if (cfg.flags & TC_FLAG_ANIMATION) pos = ripple_deform(c, pos, ti.cluster.instanceID, inst.geoHi[3]);
     float* o = P<float>(b->genVertices) + 3ull * (uint64_t(vertexOffset) + vert);
     o[0] = pos.x, o[1] = pos.y, o[2] = pos.z;
   }
@@ -820,6 +860,8 @@ void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]
   wuv[0] = w.x, wuv[1] = w.y, wuv[2] = w.z;
 }
 float oracle_sample_height(const tc_displacement_texture* tex, float u, float v) { return sample_height(*tex, u, v); }
+float oracle_ripple_sin(float x) { return ripple_sincos(x, false); }
+float oracle_ripple_cos(float x) { return ripple_sincos(x, true); }
 
 /* instances_classify.comp.glsl:100-127 with culling.glsl:30-90 (frustum and
  * size test; no HiZ pyramid exists, so the occlusion test passes). */

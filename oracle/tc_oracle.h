@@ -53,6 +53,9 @@ void oracle_config_factors(uint32_t cfg, uint32_t out[3]);
 void oracle_config_triangle_vertices(uint32_t cfg, uint32_t tri, uint32_t out[3]);
 void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]);
 float oracle_sample_height(const tc_displacement_texture* tex, float u, float v);
+/* the ripple animation's sin / cos (FP64 reduction, FP32 Taylor series) */
+float oracle_ripple_sin(float x);
+float oracle_ripple_cos(float x);
 
 /* ---- frame stages (host-address records) ---- */
 typedef struct oracle_ctx {

@@ -58,6 +58,9 @@ def oracle():
         L.oracle_config_vertex_barycentrics.argtypes = [C.c_uint32, C.c_uint32, P]
         L.oracle_sample_height.argtypes = [P, C.c_float, C.c_float]
         L.oracle_sample_height.restype = C.c_float
+        for fn in (L.oracle_ripple_sin, L.oracle_ripple_cos):
+            fn.argtypes = [C.c_float]
+            fn.restype = C.c_float
         L.oracle_table_lookup.argtypes = [P]
         L.oracle_trace.argtypes = [ctx, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
         L.oracle_trace.restype = None

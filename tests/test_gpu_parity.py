@@ -11,14 +11,14 @@ from vk_tessellated_clusters_amd import abi, scene
 pytestmark = pytest.mark.gpu
 
 
-def run_both(host, cfg):
+def run_both(host, cfg, **frame_kw):
     print("case:", host.name, len(host.instances), "instances", host.num_base_triangles, "tris", flush=True)
     of = harness.OracleFrame(host, cfg)
-    of.run(host.frame())
+    of.run(host.frame(**frame_kw))
     ref = of.results()
     g = harness.GpuFrame(host, cfg)
     try:
-        g.run(host.frame())
+        g.run(host.frame(**frame_kw))
         res = g.results()
     finally:
         g.close()
@@ -43,6 +43,22 @@ def test_grid_split_parity():
 
 def test_pn_parity():
     res, ref = run_both(scene.scene_icosphere(segments=16), harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_PN))
+    harness.compare(res, ref)
+
+
+def test_animation_parity():
+    """Ripple animation after displacement (full clusters and part
+    templates), after PN, and on 1X transient clusters."""
+    anim = abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_ANIMATION
+    res, ref = run_both(small_grid(), harness.small_config(flags=anim), animation_time=2.25)
+    assert ref.counters["partTriangleCounter"] > 0
+    harness.compare(res, ref)
+    res, ref = run_both(scene.scene_icosphere(segments=16), harness.small_config(flags=anim | abi.FLAG_PN),
+                        animation_time=0.75)
+    harness.compare(res, ref)
+    res, ref = run_both(small_grid(tess_rate_pixels=16.0), harness.small_config(flags=anim | abi.FLAG_1X_TRANSIENT),
+                        animation_time=7.5)
+    assert ref.counters["transBuildCounter"] > 0
     harness.compare(res, ref)
 
 

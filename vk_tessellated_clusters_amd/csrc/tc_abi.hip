@@ -72,7 +72,7 @@ static bool config_ok(const tc_config& c) {
 }
 
 static constexpr uint32_t kSupportedFlags =
-    TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT;
+    TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT | TC_FLAG_ANIMATION;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -81,6 +81,8 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   if (b->numRenderInstances == 0 || b->numRenderInstances > kMaxInstances) return TC_ERROR_INVALID_ARGUMENT;
   if ((cfg.flags & TC_FLAG_DISPLACEMENT) && b->numDisplacementTextures && !b->displacementTextures) return TC_ERROR_INVALID_ARGUMENT;
   if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && !b->vertexTape) return TC_ERROR_INVALID_ARGUMENT;
+  // the depth adjoint differentiates displacement only, not the ripple
+  if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && (cfg.flags & TC_FLAG_ANIMATION)) return TC_ERROR_UNSUPPORTED;
   WsLayout L = ws_layout(cfg, b->numRenderInstances);
   if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   a.view = b->view;

@@ -52,6 +52,7 @@ struct ClusterHead {
   // instance record
   uint64_t positions, normals, texcoords, localTriangles;
   float dispScale, dispOffset;
+  float geoSize;  // geoHi.w, the ripple scale
   tc_displacement_texture tex;
 };
 
@@ -105,6 +106,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       hd.localTriangles = inst.clusterLocalTriangles;
       hd.dispScale = inst.displacementScale;
       hd.dispOffset = inst.displacementOffset;
+      hd.geoSize = inst.geoHi[3];
       if (hd.disp) hd.tex = a.textures[inst.displacementIndex];
       s_h[k] = hd;
 #pragma unroll
@@ -312,6 +314,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         } else if (a.tape) {
           write_tape(a, out, p, mk(0.f, 0.f, 0.f), v2{0.f, 0.f}, hd.dispScale, hd.dispOffset, ~0u);
         }
+        if (cfg.flags & TC_FLAG_ANIMATION) p = ripple_deform(p, hd.ci.instanceID, hd.geoSize, a.view);  // :483-485
         st3(b->genVertices, out, p);
       }
     }

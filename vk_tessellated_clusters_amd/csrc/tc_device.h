@@ -254,6 +254,39 @@ __device__ __forceinline__ void bilinear_footprint(uint32_t width, uint32_t heig
   w[0] = (1.0f - fx) * (1.0f - fy), w[1] = fx * (1.0f - fy), w[2] = (1.0f - fx) * fy, w[3] = fx * fy;
 }
 
+// sin / cos that evaluate identically on the CPU oracle and the GPU: the
+// argument is reduced by multiples of pi/2 in FP64 (k = rint(x * 2/pi),
+// r = x - k * pi/2), then the Taylor series to r^9 / r^8 is evaluated in
+// FP32 (|r| <= pi/4 keeps the truncation below 3e-9).
+__device__ __forceinline__ float tc_sincos_reduced(float x, bool cosine) {
+  const double k = rint((double)x * 0.63661977236758134308);
+  const float r = (float)((double)x - k * 1.57079632679489661923);
+  long long q = (long long)k + (cosine ? 1 : 0);
+  const float r2 = r * r;
+  const float s = r * (1.0f + r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f)))));
+  const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f))));
+  const int m = (int)(q & 3);
+  return m == 0 ? s : (m == 1 ? c : (m == 2 ? -s : -c));
+}
+__device__ __forceinline__ float tc_sin(float x) { return tc_sincos_reduced(x, false); }
+__device__ __forceinline__ float tc_cos(float x) { return tc_sincos_reduced(x, true); }
+
+// displacement.glsl:106-120 ripple animation (DO_ANIMATION)
+__device__ __forceinline__ v3 ripple_deform(v3 p, uint32_t seed, float size, const tc_frame_constants* view) {
+  const float maxCoord = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
+  // normalize(p.zyx) has no direction at the object-space origin; the point
+  // stays put there instead of turning into NaN
+  if (maxCoord == 0.0f) return p;
+  const float frequency = view->animationRippleFrequency / size;
+  const float phase = view->animationState * view->animationRippleSpeed;
+  const float fs = (float)seed;
+  const v3 wave = mk(tc_sin((maxCoord * frequency + fs) + phase), tc_cos(((maxCoord * frequency) * 3.0f + fs) + phase),
+                     tc_sin(((maxCoord * frequency) * 1.2f + fs) + phase));
+  const float amp = view->animationRippleAmplitude * size;
+  const v3 n = normalize(mk(p.z, p.y, p.x));
+  return add(p, mk(n.x * (amp * wave.x), n.y * (amp * wave.y), n.z * (amp * wave.z)));
+}
+
 // displacement.glsl:30-104 (PN triangle), pow written as products
 struct PNBasis {
   v3 b030, b003, b300, b021, b012, b102, b201, b210, b120, b111;

@@ -24,7 +24,8 @@ struct PartBase {
   int32_t dispIndex;
   float dispScale;
   float dispOffset;
-  uint32_t _pad;
+  uint32_t instanceID;  // ripple seed
+  float geoSize;        // geoHi.w, the ripple scale
 };
 
 // Counters: 0 genCluster, 1 genVertex, 2 genData.
@@ -153,6 +154,8 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
       pb.dispIndex = has_disp(a, inst.displacementIndex) ? inst.displacementIndex : -1;
       pb.dispScale = inst.displacementScale;
       pb.dispOffset = inst.displacementOffset;
+      pb.instanceID = ti.cluster.instanceID;
+      pb.geoSize = inst.geoHi[3];
       parts[lane] = pb;
       if (PN) setup_pn(s_pn[PN ? wave : 0][lane], pb.pos[0], pb.pos[1], pb.pos[2], pb.nrm[0], pb.nrm[1], pb.nrm[2]);
     }
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
       } else if (a.tape) {
         write_tape(a, (uint64_t)pb.vertexOffset + vert, pos, mk(0.f, 0.f, 0.f), v2{0.f, 0.f}, 0.f, 0.f, ~0u);
       }
+      if (a.cfg.flags & TC_FLAG_ANIMATION) pos = ripple_deform(pos, pb.instanceID, pb.geoSize, a.view);  // :366-368
       st3(b->genVertices, (uint64_t)pb.vertexOffset + vert, pos);
     }
     __syncthreads();

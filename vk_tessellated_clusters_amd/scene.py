@@ -240,9 +240,15 @@ class Camera:
     far: float = 1000.0
 
 
+RIPPLE_DEFAULT = (100.0, 0.005, 3.7)  # frequency, amplitude, speed (tessellatedclusters.cpp:75-78)
+
+
 def frame_constants(cam: Camera, tess_rate_pixels: float, sun_dir=(0.3, 0.9, 0.2), disp_scale: float = 1.0,
-                    disp_offset: float = 0.0) -> abi.FrameConstants:
-    """FrameConstants as tessellatedclusters.cpp:526-613 derives them."""
+                    disp_offset: float = 0.0, animation_time: float = None,
+                    ripple=RIPPLE_DEFAULT) -> abi.FrameConstants:
+    """FrameConstants as tessellatedclusters.cpp:526-613 derives them.
+    `animation_time` (seconds of animation clock) turns the ripple state on;
+    it only deforms vertices when the frame runs with FLAG_ANIMATION."""
     fc = abi.FrameConstants()
     proj = perspective_rh_zo(math.radians(cam.fovy_deg), cam.width / cam.height, cam.near, cam.far)
     view = look_at_rh(cam.eye, cam.center, cam.up)
@@ -269,6 +275,11 @@ def frame_constants(cam: Camera, tess_rate_pixels: float, sun_dir=(0.3, 0.9, 0.2
     s /= np.linalg.norm(s)
     fc.skyParams.sunDirection = (abi.f32 * 3)(*[float(x) for x in s])
     fc.doShadow = 1
+    fc.animationRippleEnabled = 1
+    fc.animationRippleFrequency, fc.animationRippleAmplitude, fc.animationRippleSpeed = ripple
+    if animation_time is not None:
+        fc.doAnimation = 1
+        fc.animationState = animation_time
     return fc
 
 
