@@ -49,7 +49,7 @@ def profiled_traffic(kernel, workload):
     return None
 
 
-def bench_config(abi, transient_1x=False, large=False, animation=False):
+def bench_config(abi, transient_1x=False, large=False, animation=False, transient_2x=False):
     if large:
         c = abi.default_config(maxVisibleClusters=1 << 18, maxPartTriangles=1 << 23, maxSplitTriangles=1 << 20,
                                maxGeneratedClusters=1 << 23, maxGeneratedClusterMegs=12288,
@@ -60,6 +60,8 @@ def bench_config(abi, transient_1x=False, large=False, animation=False):
                                maxGeneratedVertices=1 << 25)
     if transient_1x:
         c.flags |= abi.FLAG_1X_TRANSIENT
+    if transient_2x:
+        c.flags |= abi.FLAG_2X_TRANSIENT
     if animation:
         c.flags |= abi.FLAG_ANIMATION
     return c
@@ -337,6 +339,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--transient-1x", action="store_true",
                     help="1X transient cluster builds (the reference renderer's default, renderer.hpp:47)")
+    ap.add_argument("--transient-2x", action="store_true",
+                    help="2X transient mini batches (the reference renderer's default, renderer.hpp:48)")
     ap.add_argument("--animation", action="store_true",
                     help="ripple animation of the generated vertices, clock advancing every frame")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -355,7 +359,7 @@ def main():
     from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
 
     host, desc = make_scene(args.workload)
-    cfg = bench_config(abi, args.transient_1x, args.workload == "large", args.animation)
+    cfg = bench_config(abi, args.transient_1x, args.workload == "large", args.animation, args.transient_2x)
     ds = DeviceScene(host)
     r = Renderer(ds, cfg)
     frames = frame_sequence(host, animation=args.animation)
@@ -450,13 +454,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else "") +
+                   (", 2X transient batches" if args.transient_2x else "") +
                    (", ripple animation" if args.animation else ""),
                    "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
                    "frames_on_path": len(frames),
                    "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),
                    "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
-                   "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if args.transient_1x else 0)),
-                   "transient_clas": int(sb.transBuildCounter if args.transient_1x else 0),
+                   "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if (args.transient_1x or args.transient_2x) else 0)),
+                   "transient_clas": int(sb.transBuildCounter if (args.transient_1x or args.transient_2x) else 0),
                    "gen_vertices": int(sb.genVertexCounter)},
         "capacity_ok": bool(rb.numVisibleClusters <= cfg.maxVisibleClusters and
                             rb.numPartTriangles <= cfg.maxPartTriangles and

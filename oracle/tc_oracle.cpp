@@ -352,6 +352,9 @@ struct ClassifyCounters {
  * y the counters before it (the reference decides per subgroup batch). */
 inline bool dual_part_ok(uint64_t x, uint64_t y, uint64_t n, uint32_t maxParts) { return x + y + n + 1 <= maxParts; }
 
+/* TESS_USE_TRANSIENTBUILDS (shaderio.h:140-141): 1X or 2X transient builds */
+inline bool transient_builds(tc_config cfg) { return (cfg.flags & (TC_FLAG_1X_TRANSIENT | TC_FLAG_2X_TRANSIENT)) != 0; }
+
 /* 1X transient cluster data: the exact size of our CLAS block for the
  * cluster's simple triangles and all its vertices (the reference reserves
  * basicClusterSizes[simpleCount], a worst case for NV's opaque format). */
@@ -361,12 +364,39 @@ inline uint32_t clas_size(uint32_t nt, uint32_t nv) {
   return (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
 }
 
-/* cluster_classify.comp.glsl:150-666 for one visible cluster (2X transient
- * builds off, no animation).  With TC_FLAG_1X_TRANSIENT a cluster with more
- * than one but not all triangles simple (:261-276) gets a transient CLAS
- * over its simple triangles (:314-447, :489-535) and only its other
- * triangles are split or instantiated (:576-584).  WRITE=false only
- * advances the counters. */
+/* 2X mini batches (cluster_classify.comp.glsl:667-905; shaderio.h:149-158):
+ * up to 8 base triangles with every factor <= 2 share one transient CLAS of
+ * 6 vertex slots and up to 4 triangles each. */
+constexpr uint32_t kMiniBatch = 8, kMiniTris = 4, kMiniVerts = 6;
+constexpr uint32_t kMiniBatchTris = kMiniBatch * kMiniTris, kMiniBatchVerts = kMiniBatch * kMiniVerts;
+/* the reference's SUBGROUP_SIZE: batches never span two 32-triangle groups */
+constexpr uint32_t kMiniGroup = 32;
+/* worst-case CLAS bytes / vertex slots / part slots of a batch (:732-737) */
+inline uint32_t mini_data_size() { return clas_size(kMiniBatchTris, kMiniBatchVerts); }
+constexpr uint32_t kMiniVertexSize = kMiniBatchVerts + (kMiniBatchTris * 3 + 11) / 12;
+constexpr uint32_t kMiniPartSize = (8 + kMiniBatchTris * 2 + 24 - 1) / 24;
+
+/* One generated vertex of a sub-triangle: barycentric `w` of the base
+ * triangle -> base surface (flat or PN), displacement, ripple
+ * (triangle_tess_template_instantiate.comp.glsl:346-368, and the same steps
+ * at cluster_classify.comp.glsl:846-876). */
+v3 generate_vertex(const oracle_ctx* c, tc_config cfg, const tc_render_instance& inst, uint32_t instanceID, const v3 bp[3],
+                   const v3 bn[3], const v2 bt[3], const PN* pn, v3 w) {
+  v3 pos = pn ? get_pn(*pn, w) : interp(bp, w);
+  if (has_disp(c, cfg, inst)) pos = displace(c, inst, pos, interp(bn, w), interp2(bt, w));
+  if (cfg.flags & TC_FLAG_ANIMATION) pos = ripple_deform(c, pos, instanceID, inst.geoHi[3]);
+  return pos;
+}
+
+/* cluster_classify.comp.glsl:150-905 for one visible cluster.  With
+ * TC_FLAG_1X_TRANSIENT a cluster with more than one but not all triangles
+ * simple (:261-276) gets a transient CLAS over its simple triangles
+ * (:314-447, :489-535) and only its other triangles are split or
+ * instantiated (:576-584).  With TC_FLAG_2X_TRANSIENT the remaining
+ * triangles whose factors are all <= 2 go into 2X mini batches instead of
+ * the part list (:586-592, :667-905).  Within a cluster the transient
+ * allocations (1X, then the 2X batches in triangle order) come before the
+ * split / part allocations.  WRITE=false only advances the counters. */
 template <bool WRITE>
 void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, ClassifyCounters& k) {
   tc_scene_building* b = c->build;
@@ -489,17 +519,117 @@ void classify_cluster(const oracle_ctx* c, tc_config cfg, uint32_t visIndex, Cla
     if (full) return;
   }
 
+  const bool flag2x = (cfg.flags & TC_FLAG_2X_TRANSIENT) != 0;
+  auto is_mini = [&](uint32_t t) {
+    uint32_t m = amax3(fac[t][0], fac[t][1], fac[t][2]);
+    return flag2x && m <= 2 && !(trans && m == 1);
+  };
+  if (flag2x) {
+    /* :694-790 batches of 8 consecutive mini triangles of a 32-triangle
+     * group, one allocation each */
+    for (uint32_t g0 = 0; g0 < nt; g0 += kMiniGroup) {
+      uint32_t members[kMiniGroup], nm = 0;
+      for (uint32_t t = g0; t < std::min(nt, g0 + kMiniGroup); t++)
+        if (is_mini(t)) members[nm++] = t;
+      for (uint32_t m0 = 0; m0 < nm; m0 += kMiniBatch) {
+        const uint32_t cnt = std::min(kMiniBatch, nm - m0);
+        uint32_t mcfg[kMiniBatch], mvtx[kMiniBatch][3], batchTris = 0;
+        for (uint32_t r = 0; r < cnt; r++) {
+          uint32_t f[3] = {fac[members[m0 + r]][0], fac[members[m0 + r]][1], fac[members[m0 + r]][2]};
+          mvtx[r][0] = 0, mvtx[r][1] = 32768u, mvtx[r][2] = 32768u << 16;
+          mcfg[r] = get_config(f, mvtx[r]);
+          batchTris += cfg_entry(mcfg[r]).numTriangles;
+        }
+        const uint32_t genOffset = k.genCluster++;
+        const uint32_t partOffset = dual_part_ok(k.part, k.partY, kMiniPartSize, cfg.maxPartTriangles)
+                                        ? cfg.maxPartTriangles - k.partY - kMiniPartSize
+                                        : cfg.maxPartTriangles;
+        k.partY += kMiniPartSize;
+        const uint64_t dataOffset = k.genData;
+        k.genData += mini_data_size();
+        const uint32_t vertexOffset = k.genVertex;
+        k.genVertex += kMiniVertexSize;
+        if (!WRITE) continue;
+        const bool ok = !(uint64_t(vertexOffset) + kMiniVertexSize > cfg.maxGeneratedVertices ||
+                          uint64_t(genOffset) + 1 > cfg.maxGeneratedClusters ||
+                          dataOffset + mini_data_size() > max_data_bytes(cfg) ||
+                          uint64_t(partOffset) + kMiniPartSize > cfg.maxPartTriangles);
+        if (!ok) continue;
+        /* :750-790 the transient build record */
+        const uint64_t vbuf = gen_vertices_value(c) + uint64_t(vertexOffset) * 12ull;
+        uint32_t transOffset = fadd(&b->transBuildCounter, 1u);
+        tc_clas_build_info bi;
+        memset(&bi, 0, sizeof(bi));
+        bi.clusterID = (TC_RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED << 30) | partOffset;
+        bi.packed = batchTris | (kMiniBatchVerts << 9) | ((b->positionTruncateBitCount & 63u) << 18) | (1u << 24);
+        bi.baseGeometryIndexAndFlags = TC_CLAS_GEOMETRY_OPAQUE;
+        bi.indexBufferStride = 1;
+        bi.vertexBufferStride = 12;
+        bi.vertexBuffer = vbuf;
+        bi.indexBuffer = vbuf + uint64_t(kMiniBatchVerts) * 12ull;
+        P<tc_clas_build_info>(b->transBuilds)[transOffset] = bi;
+        P<uint32_t>(b->transInstanceIDs)[transOffset] = ci.instanceID;
+        P<uint64_t>(b->transClusterAddresses)[transOffset] = b->genClusterData + dataOffset;
+        P<tc_tess_triangle_info>(b->partTriangles)[partOffset].cluster = ci;
+        fadd(&P<tc_blas_build_info>(b->blasBuildInfos)[ci.instanceID].clusterReferencesCount, 1u);
+        fadd(&c->readback->numTotalTriangles, batchTris);
+        /* :794-903 per member: its config's vertices in 6 slots, its
+         * triangles' u8 indices after the 48 vertex slots and u16 mappings
+         * (triangle | sub-triangle << 8 | (factor - 1) bits << 12) after the
+         * ClusterInfo of the part slot.  Slots and bytes no member uses are
+         * zero (the reference leaves them unwritten). */
+        float* vout = P<float>(b->genVertices) + 3ull * vertexOffset;
+        uint8_t* idx = P<uint8_t>(b->genVertices) + (uint64_t(vertexOffset) + kMiniBatchVerts) * 12ull;
+        uint16_t* map = reinterpret_cast<uint16_t*>(P<uint8_t>(b->partTriangles) + uint64_t(partOffset) * 24ull + 8ull);
+        for (uint32_t j = 0; j < kMiniBatchVerts * 3; j++) vout[j] = 0.0f;
+        for (uint32_t j = 0; j < (kMiniVertexSize - kMiniBatchVerts) * 12; j++) idx[j] = 0;
+        const bool pn = (cfg.flags & TC_FLAG_PN) != 0;
+        uint32_t o = 0;
+        for (uint32_t r = 0; r < cnt; r++) {
+          const uint32_t t = members[m0 + r];
+          v3 bb[3], bp[3], bn[3];
+          v2 bt[3];
+          for (int v = 0; v < 3; v++) {
+            bb[v] = decode_bary(mvtx[r][v]);
+            uint32_t vi = lt[t * 3 + v] + cl.firstLocalVertex;
+            bp[v] = ld3(inst.positions, vi);
+            bn[v] = normalize(ld3(inst.normals, vi));
+            bt[v] = ld2(inst.texcoords, vi);
+          }
+          PN basePN;
+          if (pn) setup_pn(basePN, bp, bn);
+          const tc_tess_table_entry& e = cfg_entry(mcfg[r]);
+          for (uint32_t vert = 0; vert < e.numVertices; vert++) {
+            v3 w = interp(bb, cfg_vertex_bary(mcfg[r], vert));
+            v3 pos = generate_vertex(c, cfg, inst, ci.instanceID, bp, bn, bt, pn ? &basePN : nullptr, w);
+            float* q = vout + 3 * (r * kMiniVerts + vert);
+            q[0] = pos.x, q[1] = pos.y, q[2] = pos.z;
+          }
+          const uint32_t packedFactors = (fac[t][0] - 1) | ((fac[t][1] - 1) << 1) | ((fac[t][2] - 1) << 2);
+          for (uint32_t i = 0; i < e.numTriangles; i++, o++) {
+            map[o] = (uint16_t)(t | (i << 8) | (packedFactors << 12));
+            uint32_t tv[3];
+            cfg_tri_vertices(mcfg[r], i, tv);
+            for (int j = 0; j < 3; j++) idx[o * 3 + j] = (uint8_t)(tv[j] + r * kMiniVerts);
+          }
+        }
+      }
+    }
+  }
+
   /* :543-666 per-triangle split / part emission */
   for (uint32_t t = 0; t < nt; t++) {
     uint32_t f[3] = {fac[t][0], fac[t][1], fac[t][2]};
     uint32_t m = amax3(f[0], f[1], f[2]);
     if (trans && m == 1) continue; /* :576-584 in the transient CLAS */
+    if (is_mini(t)) continue;      /* :586-592 in a 2X batch */
     bool needSplit = m > TC_TESSTABLE_SIZE;
     bool needPart = m <= TC_TESSTABLE_SIZE;
     uint32_t offSplit = needSplit ? k.split++ : 0;
     uint32_t offPart = needPart ? k.part++ : 0;
     if (!WRITE) continue;
-    bool partOk = flag1x ? dual_part_ok(offPart, k.partY, 1, cfg.maxPartTriangles) : offPart < cfg.maxPartTriangles;
+    bool partOk = (flag1x || flag2x) ? dual_part_ok(offPart, k.partY, 1, cfg.maxPartTriangles)
+                                     : offPart < cfg.maxPartTriangles;
     tc_tess_triangle_info ti;
     ti.cluster = ci;
     ti.subTriangle.vtxEncoded[0] = 0;
@@ -574,7 +704,7 @@ void split_item(const oracle_ctx* c, tc_config cfg, uint32_t itemIndex, SplitCou
     ti.subTriangle.triangleID_config = (ti.subTriangle.triangleID_config & 0xFFFFu) | (cf << 16);
     /* with transient builds the part list is shared with their slots at
      * its top (build.glsl:68-83) */
-    bool partOk = (cfg.flags & TC_FLAG_1X_TRANSIENT)
+    bool partOk = transient_builds(cfg)
                       ? dual_part_ok(offPart, uint32_t(b->dualPartTriangleCounter >> 32), 1, cfg.maxPartTriangles)
                       : offPart < cfg.maxPartTriangles;
     if (needSplit && offSplit < cfg.maxSplitTriangles)
@@ -958,7 +1088,7 @@ void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode) {
     uint32_t count = end - start;
     b->dispatchTriangleSplit = tc_dispatch_indirect{(count + 63) / 64, 1, 1};
   } else if (mode == TC_BUILD_SETUP_BUILD_BLAS) {
-    const bool trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) != 0;
+    const bool trans = transient_builds(cfg);
     uint32_t temp = b->tempInstantiateCounter;
     uint32_t tr = trans ? b->transBuildCounter : 0u;
     rb->numBlasClusters = temp + tr;
@@ -980,7 +1110,7 @@ void oracle_build_setup(const oracle_ctx* c, tc_config cfg, int mode) {
   } else if (mode == TC_BUILD_SETUP_INSTANTIATE_TESS) {
     uint32_t part = b->partTriangleCounter;
     rb->numSplitTriangles = b->splitWriteCounter;
-    if (cfg.flags & TC_FLAG_1X_TRANSIENT) {
+    if (transient_builds(cfg)) {
       /* :240-253; the instantiated parts are the accepted prefix
        * x + y + 2 <= MAX_PART_TRIANGLES of the lower counter */
       uint32_t y = uint32_t(b->dualPartTriangleCounter >> 32);
@@ -1019,7 +1149,7 @@ void oracle_blas_setup_insertion(const oracle_ctx* c, tc_config) {
  * holds the address of the per-instance list in the build's address space;
  * blasClusterAddresses is the host pointer of the same array. */
 void oracle_blas_clusters_insert(const oracle_ctx* c, tc_config cfg, int transient) {
-  if (transient && !(cfg.flags & TC_FLAG_1X_TRANSIENT)) return;
+  if (transient && !transient_builds(cfg)) return;
   tc_scene_building* b = c->build;
   uint32_t n = transient ? b->transBuildCounter : b->tempInstantiateCounter;
   const uint32_t* ids = P<const uint32_t>(transient ? b->transInstanceIDs : b->tempInstanceIDs);
@@ -1099,7 +1229,7 @@ void oracle_clas_instantiate(const oracle_ctx* c, tc_config cfg) {
  * boxes of TC_CLAS_GROUP_TRIANGLES triangles, all the record's vertices. */
 void oracle_clas_build_transient(const oracle_ctx* c, tc_config cfg) {
   tc_scene_building* b = c->build;
-  if (!(cfg.flags & TC_FLAG_1X_TRANSIENT)) return;
+  if (!transient_builds(cfg)) return;
   uint32_t n = std::min(b->transBuildCounter, cfg.maxGeneratedClusters);
   auto r16 = [](uint32_t v) { return (v + 15u) & ~15u; };
   for (uint32_t i = 0; i < n; i++) {
@@ -1313,6 +1443,7 @@ void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded&
     return;
   }
   uint32_t mode = h.clusterID >> 30, clusterID = h.clusterID & 0x3FFFFFFFu, triangleID = h.primitiveID;
+  uint32_t subTriangle = h.primitiveID;
   v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v), base = bw;
   if (mode == TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER) {
     /* rchit:173-178: the part slot holds the ClusterInfo, then one byte per
@@ -1322,6 +1453,28 @@ void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded&
     memcpy(&ci, slot, sizeof(ci));
     triangleID = slot[8 + h.primitiveID];
     clusterID = ci.clusterID;
+  } else if (mode == TC_RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED) {
+    /* rchit:152-170: a u16 mapping per batch triangle after the ClusterInfo
+     * names the cluster triangle, the sub-triangle of its config and the
+     * (factor - 1) bits the config is rebuilt from; the sub-triangle field
+     * is masked with 0xF (the reference's `& 4` drops sub-triangles 1-3) */
+    const uint8_t* slot = P<const uint8_t>(c->build->partTriangles) + uint64_t(clusterID) * 24ull;
+    tc_cluster_info ci;
+    memcpy(&ci, slot, sizeof(ci));
+    uint16_t packed;
+    memcpy(&packed, slot + 8 + 2 * h.primitiveID, 2);
+    triangleID = packed & 0xFFu;
+    subTriangle = (packed >> 8) & 0xFu;
+    uint32_t f[3] = {1u + ((packed >> 12) & 1u), 1u + ((packed >> 13) & 1u), 1u + (uint32_t(packed) >> 14)};
+    uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
+    uint32_t cfg = get_config(f, vtx);
+    clusterID = ci.clusterID;
+    uint32_t idx[3];
+    cfg_tri_vertices(cfg, subTriangle, idx);
+    v3 vb[3] = {cfg_vertex_bary(cfg, idx[0]), cfg_vertex_bary(cfg, idx[1]), cfg_vertex_bary(cfg, idx[2])};
+    v3 nb = interp(vb, bw);
+    v3 bb[3] = {decode_bary(vtx[0]), decode_bary(vtx[1]), decode_bary(vtx[2])};
+    base = interp(bb, nb);
   } else if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
     const tc_tess_triangle_info& ti = P<const tc_tess_triangle_info>(c->build->partTriangles)[clusterID];
     triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
@@ -1337,7 +1490,7 @@ void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded&
   }
   out.clusterID = clusterID;
   out.triangleID = triangleID;
-  out.subTriangleID = h.primitiveID;
+  out.subTriangleID = subTriangle;
   out.baryBase[0] = base.x, out.baryBase[1] = base.y, out.baryBase[2] = base.z;
   tc_clas_header hdr;
   memcpy(&hdr, b.clas, sizeof(hdr));

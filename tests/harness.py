@@ -161,7 +161,7 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
     nref = b.blasClusterCounter
     refs = fetch(base["blasClusterAddresses"], 8 * nref).view(np.uint64)
     R.arrays["blasClusterOffsets"] = refs - np.uint64(base["genClusterData"])
-    if cfg.flags & abi.FLAG_1X_TRANSIENT:
+    if cfg.flags & abi.TRANSIENT_FLAGS:
         ntr = min(b.transBuildCounter, cfg.maxGeneratedClusters)
         tb = fetch(base["transBuilds"], 64 * ntr).view(abi.CLAS_BUILD_INFO_DT).copy()
         for f in ("vertexBuffer", "indexBuffer"):
@@ -175,8 +175,12 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
         blocks = [fetch(int(base["genClusterData"] + int(o)), int(sz)) for o, sz in zip(R.arrays["transClusterOffsets"], tsz)]
         R.arrays["transClasBlocks"] = np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
         # written bytes of every transient part slot: ClusterInfo + mappings
-        slots = [fetch(int(base["partTriangles"] + 24 * (int(r["clusterID"]) & 0x3FFFFFFF)), 8 + (int(r["packed"]) & 511))
-                 for r in tb]
+        # (u8 per triangle for 1X subsets, u16 for 2X batches)
+        def slot_bytes(r):
+            width = 2 if int(r["clusterID"]) >> 30 == abi.RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED else 1
+            return 8 + width * (int(r["packed"]) & 511)
+
+        slots = [fetch(int(base["partTriangles"] + 24 * (int(r["clusterID"]) & 0x3FFFFFFF)), slot_bytes(r)) for r in tb]
         R.arrays["transSlots"] = np.concatenate(slots) if slots else np.zeros(0, np.uint8)
     return R
 
@@ -265,7 +269,7 @@ class OracleFrame:
         sb.blasBuildInfos = alloc("blasBuildInfos", 16 * n)
         sb.blasBuildSizes = alloc("blasBuildSizes", 4 * n)
         sb.blasClusterAddresses = alloc("blasClusterAddresses", 8 * cfg.maxGeneratedClusters)
-        if cfg.flags & abi.FLAG_1X_TRANSIENT:
+        if cfg.flags & abi.TRANSIENT_FLAGS:
             sb.transBuilds = alloc("transBuilds", 64 * cfg.maxGeneratedClusters)
             sb.transInstanceIDs = alloc("transInstanceIDs", 4 * cfg.maxGeneratedClusters)
             sb.transClusterAddresses = alloc("transClusterAddresses", 8 * cfg.maxGeneratedClusters)

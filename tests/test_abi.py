@@ -83,10 +83,11 @@ def test_argument_errors_without_gpu_work():
     assert L.tc_frame_tessellate(None, cfg, 0, None) == 1
     assert L.tc_memcpy(None, None, 16) == 1
     assert L.tc_cluster_template_order(0, 0, None, None, None) == 1
-    unsupported = abi.default_config(flags=abi.FLAG_2X_TRANSIENT)
-    b.view = b.tessTable = b.readback = b.instances = b.build = b.workspace = 16
+    b.view = b.tessTable = b.readback = b.instances = b.build = b.workspace = b.vertexTape = 16
     b.numRenderInstances = 1
-    assert L.tc_cluster_classify(C.byref(b), unsupported, None) == 5
+    # an unknown flag bit, and the vertex tape with flags it does not cover
+    for flags in (1 << 20, abi.FLAG_VERTEX_TAPE | abi.FLAG_2X_TRANSIENT, abi.FLAG_VERTEX_TAPE | abi.FLAG_ANIMATION):
+        assert L.tc_cluster_classify(C.byref(b), abi.default_config(flags=flags), None) == 5, hex(flags)
 
 
 def test_cluster_template_order_is_a_permutation():

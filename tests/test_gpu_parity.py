@@ -89,6 +89,30 @@ def test_transient_1x_parity():
     harness.compare(res, ref)
 
 
+def test_transient_2x_parity():
+    """2X mini batches: batch records and part slots, member vertices,
+    index and u16 mapping bytes, CLAS blocks and both insert passes; alone,
+    with 1X subsets, with PN and with the ripple."""
+    base = abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_2X_TRANSIENT
+    for flags, kw in ((base, {}), (base | abi.FLAG_1X_TRANSIENT, {}), (base | abi.FLAG_PN, {}),
+                      (base | abi.FLAG_1X_TRANSIENT | abi.FLAG_ANIMATION, {"animation_time": 3.5})):
+        print("flags", hex(flags), flush=True)
+        res, ref = run_both(small_grid(tess_rate_pixels=8.0), harness.small_config(flags=flags), **kw)
+        modes = ref.arrays["transBuilds"]["clusterID"] >> 30
+        assert (modes == abi.RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED).sum() > 100
+        harness.compare(res, ref)
+
+
+def test_transient_2x_capacity_overflow_parity():
+    # batch slots and parts share the part list; vertices run out part way
+    flags = abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_1X_TRANSIENT | abi.FLAG_2X_TRANSIENT
+    for kw in ({"maxPartTriangles": 12000}, {"maxGeneratedVertices": 60000}, {"maxGeneratedClusters": 1500},
+               {"maxGeneratedClusterMegs": 1}):
+        print("capacity", kw, flush=True)
+        res, ref = run_both(small_grid(tess_rate_pixels=8.0), harness.small_config(flags=flags, **kw))
+        harness.compare(res, ref)
+
+
 def test_transient_1x_capacity_overflow_parity():
     # the part list is shared by parts (from the bottom) and transient slots
     # (from the top); acceptance stays a prefix of the canonical order

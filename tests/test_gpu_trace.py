@@ -185,6 +185,27 @@ def test_trace_parity_transient_1x():
     print("hit agreement", agree)
 
 
+def test_trace_parity_transient_2x():
+    """Hits on 2X batch CLAS decode to the base cluster, triangle and
+    sub-triangle (with 1X subsets in the same frame)."""
+    host = scene.scene_grid_instances(num_instances=4, quads=64, patch_size=8.0, tex_size=256, tess_rate_pixels=8.0)
+    cfg = harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_1X_TRANSIENT |
+                               abi.FLAG_2X_TRANSIENT)
+    of = harness.OracleFrame(host, cfg)
+    of.run(host.frame())
+    W, H = 160, 90
+    oh, od = of.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(host.frame(), stages="accel")
+        gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    finally:
+        g.close()
+    assert ((oh["clusterID"] >> 30) == abi.RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED).sum() > 50
+    agree = compare_hits(gh, oh, gd, od)
+    print("hit agreement", agree)
+
+
 def test_packet_traversal_matches_per_ray():
     """Packet traversal gives the per-ray traversal's hits byte for byte,
     for camera rays (ragged tiles) and explicit rays in any directions."""

@@ -35,6 +35,7 @@ BUILD_SETUP_BUILD_BLAS = 5
 RT_CLUSTER_MODE_FULL_CLUSTER = 0
 RT_CLUSTER_MODE_SINGLE_TESSELLATED = 1
 RT_CLUSTER_MODE_1X_SUBSET_CLUSTER = 2
+RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED = 3
 
 TEXTURE_R16_UNORM = 0
 TEXTURE_R32_FLOAT = 1
@@ -45,6 +46,7 @@ FLAG_ANIMATION = 1 << 2
 FLAG_PN = 1 << 3
 FLAG_1X_TRANSIENT = 1 << 4
 FLAG_2X_TRANSIENT = 1 << 5
+TRANSIENT_FLAGS = FLAG_1X_TRANSIENT | FLAG_2X_TRANSIENT  # TESS_USE_TRANSIENTBUILDS
 FLAG_PERSISTENT_SPLIT = 1 << 6
 FLAG_VERTEX_TAPE = 1 << 7
 

@@ -72,7 +72,8 @@ static bool config_ok(const tc_config& c) {
 }
 
 static constexpr uint32_t kSupportedFlags =
-    TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT | TC_FLAG_ANIMATION;
+    TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT |
+    TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -81,8 +82,9 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   if (b->numRenderInstances == 0 || b->numRenderInstances > kMaxInstances) return TC_ERROR_INVALID_ARGUMENT;
   if ((cfg.flags & TC_FLAG_DISPLACEMENT) && b->numDisplacementTextures && !b->displacementTextures) return TC_ERROR_INVALID_ARGUMENT;
   if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && !b->vertexTape) return TC_ERROR_INVALID_ARGUMENT;
-  // the depth adjoint differentiates displacement only, not the ripple
-  if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && (cfg.flags & TC_FLAG_ANIMATION)) return TC_ERROR_UNSUPPORTED;
+  // the depth adjoint differentiates displacement only (not the ripple) on
+  // cluster and part vertices (not on 2X batch vertices)
+  if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && (cfg.flags & (TC_FLAG_ANIMATION | TC_FLAG_2X_TRANSIENT))) return TC_ERROR_UNSUPPORTED;
   WsLayout L = ws_layout(cfg, b->numRenderInstances);
   if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   a.view = b->view;
@@ -449,7 +451,10 @@ tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream 
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)0);
   uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().classifyBlocksPerCU);
-  hipLaunchKernelGGL(k_cluster_classify, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+  if (config.flags & TC_FLAG_2X_TRANSIENT)
+    hipLaunchKernelGGL(k_cluster_classify<true>, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cluster_classify<false>, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
   return launched();
 }
 
@@ -493,7 +498,7 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
   WsLayout L;
   tc_status s = make_args(b, config, a, &L);
   if (s != TC_OK) return s;
-  if (transient && !(config.flags & TC_FLAG_1X_TRANSIENT)) return TC_OK;  // no transient builds configured
+  if (transient && !transient_builds(config.flags)) return TC_OK;  // no transient builds configured
   const int tr = transient ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   SortBuffers sb = sort_buffers(b, L);

@@ -304,7 +304,7 @@ extern "C" tc_status tc_clas_build_transient(const tc_bindings* b, tc_config con
   KArgs a;
   tc_status s = make_args_public(b, config, a);
   if (s != TC_OK) return s;
-  if (!(config.flags & TC_FLAG_1X_TRANSIENT)) return TC_OK;
+  if (!transient_builds(config.flags)) return TC_OK;
   uint32_t grid = std::min<uint32_t>(config.maxGeneratedClusters, (uint32_t)num_cu_public() * tuning_snapshot().clasWavesPerCU);
   hipLaunchKernelGGL(k_clas_build_transient, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;

@@ -15,7 +15,14 @@
 //   3. the 256 threads run flat over the displaced vertices of the accepted
 //      clusters; each wave then emits the split / part records (and the
 //      transient index and mapping bytes) of its share of the clusters.
-// Offsets are those of a serial run in (visible cluster, triangle) order.
+// With TC_FLAG_2X_TRANSIENT (the TWO_X instance of the kernel) the
+// triangles whose factors are all <= 2 form mini batches of up to 8 base
+// triangles, each one transient CLAS (:667-905): phase 1 counts the
+// batches, phase 2 allocates them after the cluster's own allocation and
+// phase 3 writes their vertices, indices, mappings and build records.
+// Offsets are those of a serial run in (visible cluster, triangle) order;
+// within a cluster the transient allocations (1X, then the 2X batches in
+// triangle order) come before the split / part allocations.
 #include "tc_stage.h"
 
 namespace tc {
@@ -34,6 +41,11 @@ struct ClusterSlot {
   uint32_t yIncl;       // upper part counter after this cluster
   uint32_t ok;
   uint64_t dataOffset;
+  // 2X mini batches: count, accepted count (a prefix), and the vertex slot,
+  // CLAS byte offset, upper part counter and transient rank of batch 0
+  uint32_t nb, nbOk;
+  uint32_t batchV0, batchY0, batchRank0;
+  uint64_t batchD0;
 };
 
 constexpr uint32_t kSimpleCode = 1u | (1u << 4) | (1u << 8);  // factors (1,1,1), no split
@@ -52,10 +64,139 @@ struct ClusterHead {
   // instance record
   uint64_t positions, normals, texcoords, localTriangles;
   float dispScale, dispOffset;
-  float geoSize;  // geoHi.w, the ripple scale
   tc_displacement_texture tex;
+  float geoSize;  // geoHi.w, the ripple scale
 };
 
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// :794-903 the mini triangles of one cluster, one lane per base triangle:
+// each writes its config's vertices into its 6 slots (unused slots zero),
+// its triangles' u8 indices after the batch's 48 vertex slots and its u16
+// mappings (triangle | sub-triangle << 8 | (factor - 1) bits << 12) after
+// the ClusterInfo of the batch's part slot; the first lane of a batch writes
+// the build record and zeroes the slots and index bytes no member uses.
+// Returns the wave's sum of batch triangles (lane 0).
+__device__ __forceinline__ uint32_t emit_mini_batches(const KArgs& a, const TableView& T, const ClusterSlot& s,
+                                                      const ClusterHead& hd, const uint16_t* code) {
+  tc_scene_building* b = a.build;
+  const tc_config& cfg = a.cfg;
+  const uint32_t lane = lane_id();
+  const uint32_t half = lane >> 5;
+  const uint64_t halfMask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+  uint32_t batchTotal = 0;
+  uint32_t batchBase = 0;
+  for (uint32_t t0 = 0; t0 < s.nt && batchBase < s.nbOk; t0 += kWave) {
+    const uint32_t t = t0 + lane;
+    const uint32_t c = t < s.nt ? (uint32_t)code[t] : 0x8000u;
+    uint32_t f[3] = {c & 15u, (c >> 4) & 15u, (c >> 8) & 15u};
+    const uint32_t mx = umax3(f[0], f[1], f[2]);
+    const bool mini = !(c & 0x8000u) && mx <= 2 && !(s.trans && mx == 1);
+    const uint64_t vm = ballot(mini);
+    const uint32_t nLo = __popcll(vm & 0xFFFFFFFFull), nHi = __popcll(vm >> 32);
+    const uint32_t bLo = (nLo + kMiniBatch - 1) / kMiniBatch, bHi = (nHi + kMiniBatch - 1) / kMiniBatch;
+    const uint32_t r = __popcll(vm & halfMask & lanes_below());  // rank in the 32-triangle group
+    const uint32_t j = batchBase + (half ? bLo : 0u) + r / kMiniBatch;
+    const uint32_t rel = r % kMiniBatch;
+    batchBase += bLo + bHi;
+    uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
+    uint32_t cf = 0;
+    tc_tess_table_entry e = {0, 0, 0, 0};
+    if (mini) {
+      uint32_t fr[3] = {f[0], f[1], f[2]};  // get_config rotates its factors
+      cf = get_config(fr, vtx);
+      e = cfg_entry(T, cf);
+    }
+    const uint32_t ntri = mini ? (uint32_t)e.numTriangles : 0u;
+    const uint32_t incl = wave_incl_scan(ntri);
+    const bool start = mini && rel == 0;
+    const uint64_t starts = ballot(start);
+    const uint64_t upTo = starts & (lanes_below() | (1ull << lane));
+    const uint32_t startLane = upTo ? 63u - (uint32_t)__builtin_clzll(upTo) : lane;
+    const uint32_t firstTris = shfl(incl - ntri, startLane);
+    // the batch ends before the next start of the same group
+    const uint64_t later = starts & ~(lanes_below() | (1ull << lane)) & halfMask;
+    const uint32_t endLane = later ? (uint32_t)__builtin_ctzll(later) - 1u : (half ? 63u : 31u);
+    const uint32_t endIncl = shfl(incl, endLane);
+    const bool ok = mini && j < s.nbOk;
+    if (ok) {
+      const uint32_t vOff = s.batchV0 + kMiniVertexSize * j;
+      const uint32_t pOff = cfg.maxPartTriangles - (s.batchY0 + kMiniPartSize * j) - kMiniPartSize;
+      const uint32_t baseTris = incl - ntri - firstTris;
+      // base triangle
+      const uint8_t* lt = ptr<const uint8_t>(hd.localTriangles) + hd.firstLocalTriangle + 3 * t;
+      v3 bp[3], bn[3];
+      v2 bt[3];
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        const uint32_t vi = hd.firstLocalVertex + lt[v];
+        bp[v] = ld3(hd.positions, vi);
+        bn[v] = normalize(ld3(hd.normals, vi));
+        bt[v] = ld2(hd.texcoords, vi);
+      }
+      const v3 b0 = decode_bary(vtx[0]), b1 = decode_bary(vtx[1]), b2 = decode_bary(vtx[2]);
+      const bool pn = (cfg.flags & TC_FLAG_PN) != 0;
+      PNBasis pnb;
+      if (pn) setup_pn(pnb, bp[0], bp[1], bp[2], bn[0], bn[1], bn[2]);
+      for (uint32_t vert = 0; vert < kMiniVerts; vert++) {
+        v3 pos = mk(0.f, 0.f, 0.f);
+        if (vert < e.numVertices) {
+          const v3 w = interp(b0, b1, b2, cfg_vertex_bary(T, cf, e.firstVertex, vert));
+          pos = pn ? get_pn(pnb, w) : interp(bp[0], bp[1], bp[2], w);
+          if (hd.disp) {
+            const v3 n = interp(bn[0], bn[1], bn[2], w);
+            const v2 uv = interp2(bt[0], bt[1], bt[2], w);
+            const float hgt = displaced_height(a, hd.tex, hd.dispScale, hd.dispOffset, uv);
+            pos = add(pos, mul(normalize(n), hgt));
+          }
+          if (cfg.flags & TC_FLAG_ANIMATION) pos = ripple_deform(pos, hd.ci.instanceID, hd.geoSize, a.view);
+        }
+        st3(b->genVertices, (uint64_t)vOff + rel * kMiniVerts + vert, pos);
+      }
+      const uint32_t packedFactors = (f[0] - 1) | ((f[1] - 1) << 1) | ((f[2] - 1) << 2);
+      uint16_t* map = reinterpret_cast<uint16_t*>(ptr<uint8_t>(b->partTriangles) + (uint64_t)pOff * 24ull + 8ull);
+      uint8_t* idx = ptr<uint8_t>(b->genVertices) + ((uint64_t)vOff + kMiniBatchVerts) * 12ull;
+      for (uint32_t i = 0; i < ntri; i++) {
+        map[baseTris + i] = (uint16_t)(t | (i << 8) | (packedFactors << 12));
+        uint32_t tv[3];
+        cfg_tri_vertices(T, cf, e.firstTriangle, i, tv);
+#pragma unroll
+        for (int q = 0; q < 3; q++) idx[(baseTris + i) * 3 + q] = (uint8_t)(tv[q] + rel * kMiniVerts);
+      }
+      if (start) {
+        // :750-790 the batch's build record
+        const uint32_t batchTris = endIncl - (incl - ntri);
+        const uint32_t members = min(kMiniBatch, (half ? nHi : nLo) - r);
+        const uint64_t vbuf = b->genVertices + (uint64_t)vOff * 12ull;
+        tc_clas_build_info bi;
+        bi.clusterID = (TC_RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED << 30) | pOff;
+        bi.clusterFlags = 0;
+        bi.packed = batchTris | (kMiniBatchVerts << 9) | ((b->positionTruncateBitCount & 63u) << 18) | (1u << 24);
+        bi.baseGeometryIndexAndFlags = TC_CLAS_GEOMETRY_OPAQUE;
+        bi.indexBufferStride = 1;
+        bi.vertexBufferStride = 12;
+        bi.geometryIndexAndFlagsBufferStride = 0;
+        bi.opacityMicromapIndexBufferStride = 0;
+        bi.indexBuffer = vbuf + (uint64_t)kMiniBatchVerts * 12ull;
+        bi.vertexBuffer = vbuf;
+        bi.geometryIndexAndFlagsBuffer = 0;
+        bi.opacityMicromapArray = 0;
+        bi.opacityMicromapIndexBuffer = 0;
+        const uint32_t transOffset = s.batchRank0 + j;
+        ptr<tc_clas_build_info>(b->transBuilds)[transOffset] = bi;
+        ptr<uint32_t>(b->transInstanceIDs)[transOffset] = s.ci.instanceID;
+        ptr<uint64_t>(b->transClusterAddresses)[transOffset] = b->genClusterData + s.batchD0 + (uint64_t)kMiniDataSize * j;
+        ptr<tc_tess_triangle_info>(b->partTriangles)[pOff].cluster = s.ci;
+        for (uint32_t q = members * kMiniVerts; q < kMiniBatchVerts; q++) st3(b->genVertices, (uint64_t)vOff + q, mk(0.f, 0.f, 0.f));
+        for (uint32_t q = batchTris * 3; q < (kMiniVertexSize - kMiniBatchVerts) * 12; q++) idx[q] = 0;
+        batchTotal += batchTris;
+      }
+    }
+  }
+  return wave_sum(batchTotal);
+}
+
+template <bool TWO_X>
 __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
   __shared__ uint16_t s_code[kClassifyTile][kMaxClusterTris];
   __shared__ ClusterSlot s_cl[kClassifyTile];
@@ -65,6 +206,10 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
   __shared__ uint32_t s_vstart[kClassifyTile], s_vcluster[kClassifyTile];
   __shared__ uint32_t s_vtotal;
   __shared__ uint32_t s_tile;
+  // 2X: batches / mini triangles counting all triangles with factors <= 2,
+  // and counting only those with a factor 2 (the rest of a 1X cluster)
+  __shared__ uint32_t s_nb12[TWO_X ? kClassifyTile : 1], s_nb2[TWO_X ? kClassifyTile : 1];
+  __shared__ uint32_t s_nm12[TWO_X ? kClassifyTile : 1], s_nm2[TWO_X ? kClassifyTile : 1];
 
   WsHeader* h = a.ws;
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
@@ -74,6 +219,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
   const ViewParams vp = view_params(a);
   tc_scene_building* b = a.build;
   const tc_config cfg = a.cfg;
+  const TableView T = load_table(a.tessTable);
 
   while (true) {
     if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -107,13 +253,20 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       hd.dispScale = inst.displacementScale;
       hd.dispOffset = inst.displacementOffset;
       hd.geoSize = inst.geoHi[3];
-      if (hd.disp) hd.tex = a.textures[inst.displacementIndex];
+      // field by field, so the record stays in registers
+      const tc_displacement_texture* tp = a.textures + (hd.disp ? inst.displacementIndex : 0);
+      hd.tex.texels = hd.disp ? tp->texels : 0ull;
+      hd.tex.width = hd.disp ? tp->width : 0u;
+      hd.tex.height = hd.disp ? tp->height : 0u;
+      hd.tex.format = hd.disp ? tp->format : 0u;
+      hd.tex._pad = 0;
       s_h[k] = hd;
 #pragma unroll
       for (int i = 0; i < 16; i++) s_m[k][i] = inst.worldMatrix[i];
       s_simple[k] = hd.invisible ? hd.nt : 0u;  // :203-211 invisible: all 1x1x1
       s_ns[k] = 0;
       s_np[k] = 0;
+      if (TWO_X) s_nb12[k] = s_nb2[k] = s_nm12[k] = s_nm2[k] = 0;
     }
     __syncthreads();
 
@@ -149,6 +302,20 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
           if (ns) atomicAdd(&s_ns[k], ns);
           if (np) atomicAdd(&s_np[k], np);
         }
+        if (TWO_X) {
+          // a wave covers two whole 32-triangle groups of one cluster
+          const uint64_t m12 = ballot(valid && mx <= 2), m2 = ballot(valid && mx == 2);
+          const uint32_t l12 = __popcll(m12 & 0xFFFFFFFFull), h12 = __popcll(m12 >> 32);
+          const uint32_t l2 = __popcll(m2 & 0xFFFFFFFFull), h2 = __popcll(m2 >> 32);
+          if (lane == 0 && (l12 | h12)) {
+            atomicAdd(&s_nb12[k], (l12 + kMiniBatch - 1) / kMiniBatch + (h12 + kMiniBatch - 1) / kMiniBatch);
+            atomicAdd(&s_nm12[k], l12 + h12);
+            if (l2 | h2) {
+              atomicAdd(&s_nb2[k], (l2 + kMiniBatch - 1) / kMiniBatch + (h2 + kMiniBatch - 1) / kMiniBatch);
+              atomicAdd(&s_nm2[k], l2 + h2);
+            }
+          }
+        }
       }
     }
     __syncthreads();
@@ -168,7 +335,10 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       // :261-276 transientSimpleThreshold = 1
       s.trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) && !s.full && simple > 1;
       s.ns = s.full ? 0 : ns;
-      s.np = s.full ? 0 : (s.trans ? np - simple : np);
+      // :586-592 mini triangles go to 2X batches, not to the part list
+      const uint32_t mini = TWO_X && !s.full ? (s.trans ? s_nm2[k] : s_nm12[k]) : 0u;
+      s.nb = TWO_X && !s.full ? (s.trans ? s_nb2[k] : s_nb12[k]) : 0u;
+      s.np = s.full ? 0 : (s.trans ? np - simple : np) - mini;
       s.dataSize = s.full ? ptr<const uint32_t>(a.instances[hd.ci.instanceID].clusterTemplateInstantiatonSizes)[hd.ci.clusterID]
                           : (s.trans ? clas_block_size(simple, hd.nv) : 0u);
       s.vertexSize = s.full ? hd.nv : (s.trans ? hd.nv + (simple * 3 + 11) / 12 : 0u);
@@ -185,11 +355,12 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       ClusterSlot s;
       if (valid) s = s_cl[k];
       const bool alloc = valid && (s.full || s.trans);
+      const uint32_t nb = TWO_X && valid ? s.nb : 0u;
       const uint32_t cs = valid ? s.ns : 0u, cp = valid ? s.np : 0u;
-      const uint32_t cg = alloc ? 1u : 0u, cv = alloc ? s.vertexSize : 0u;
-      const uint64_t cd = alloc ? (uint64_t)s.dataSize : 0ull;
-      const uint32_t cy = valid ? s.partSize : 0u;
-      const uint32_t cf = valid && s.full ? 1u : 0u, ct = valid && s.trans ? 1u : 0u;
+      const uint32_t cg = (alloc ? 1u : 0u) + nb, cv = (alloc ? s.vertexSize : 0u) + nb * kMiniVertexSize;
+      const uint64_t cd = (alloc ? (uint64_t)s.dataSize : 0ull) + (uint64_t)nb * kMiniDataSize;
+      const uint32_t cy = (valid ? s.partSize : 0u) + nb * kMiniPartSize;
+      const uint32_t cf = valid && s.full ? 1u : 0u, ct1 = valid && s.trans ? 1u : 0u, ct = ct1 + nb;
       const uint32_t is = wave_incl_scan(cs), ip = wave_incl_scan(cp), ig = wave_incl_scan(cg), iv = wave_incl_scan(cv);
       const uint64_t id = wave_incl_scan64(cd);
       const uint32_t iy = wave_incl_scan(cy), iff = wave_incl_scan(cf), it = wave_incl_scan(ct);
@@ -215,6 +386,28 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
                         s.dataOffset + s.dataSize > max_data_bytes(cfg) ||
                         (s.trans && (uint64_t)s.partOffset + s.partSize > cfg.maxPartTriangles));
         s.ok = ok;
+        if (TWO_X) {
+          // :711-748 the cluster's mini batches, after its own allocation;
+          // accepted batches are a prefix
+          s.batchV0 = s.vertexOffset + (alloc ? s.vertexSize : 0u);
+          s.batchD0 = s.dataOffset + (alloc ? s.dataSize : 0u);
+          s.batchY0 = (uint32_t)(yBefore + s.partSize);
+          s.batchRank0 = (uint32_t)(B7 + excl[7] + (it - ct)) + ct1;
+          const uint32_t gen0 = s.genOffset + (alloc ? 1u : 0u);
+          uint32_t nbOk = 0;
+          for (uint32_t j = 0; j < nb; j++) {
+            const uint64_t y = (uint64_t)s.batchY0 + (uint64_t)kMiniPartSize * j;
+            if ((uint64_t)s.offPart + y + kMiniPartSize + 1 > cfg.maxPartTriangles ||
+                (uint64_t)s.batchV0 + (uint64_t)kMiniVertexSize * (j + 1) > cfg.maxGeneratedVertices ||
+                (uint64_t)gen0 + j + 1 > cfg.maxGeneratedClusters ||
+                s.batchD0 + (uint64_t)kMiniDataSize * (j + 1) > max_data_bytes(cfg))
+              break;
+            nbOk++;
+          }
+          s.nbOk = nbOk;
+        } else {
+          s.nb = s.nbOk = 0;
+        }
         s_cl[k] = s;
         if (ok) {
           // accepted template / transient clusters are prefixes of their
@@ -273,9 +466,12 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         if (lane == kWave - 1) s_vtotal = vincl;
       }
       // :302, :442-444 statistics and BLAS reference counts, aggregated
-      add_blas_reference_counts(ptr<tc_blas_build_info>(b->blasBuildInfos), valid ? s.ci.instanceID : 0u, ok ? 1u : 0u, ok);
+      const uint32_t nbOk = TWO_X && valid ? s.nbOk : 0u;
+      const uint32_t refs = (ok ? 1u : 0u) + nbOk;
+      add_blas_reference_counts(ptr<tc_blas_build_info>(b->blasBuildInfos), valid ? s.ci.instanceID : 0u, refs, refs != 0);
       const uint32_t nFull = __popcll(ballot(cf != 0));
-      const uint32_t nOkFull = __popcll(ballot(ok && cf)), nOkTrans = __popcll(ballot(ok && ct));
+      const uint32_t nOkFull = __popcll(ballot(ok && cf));
+      const uint32_t nOkTrans = __popcll(ballot(ok && ct1)) + (TWO_X ? wave_sum(nbOk) : 0u);
       const uint32_t okTris = wave_sum(ok ? s.simple : 0u);
       if (lane == 0) {
         if (nFull) atomicAdd(&a.readback->numFullClusters, nFull);
@@ -319,10 +515,12 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       }
     }
 
-    // ---- 3b. per cluster: transient bytes, split / part records
+    // ---- 3b. per cluster: transient bytes, 2X batches, split / part records
+    uint32_t miniTris = 0;
     for (uint32_t k = wave; k < cnt; k += kClassifyWaves) {
       const ClusterSlot s = s_cl[k];
       if (s.full) continue;
+      if (TWO_X && s.nbOk) miniTris += emit_mini_batches(a, T, s, s_h[k], s_code[k]);
       if (s.ok && s.trans) {
         // :497-534 triangle mapping bytes after the ClusterInfo of the part
         // slot, u8 indices after the vertices, in triangle order
@@ -351,7 +549,13 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
       uint32_t runSplit = s.offSplit, runPart = s.offPart;
       for (uint32_t t0 = 0; t0 < s.nt; t0 += kWave) {
         const uint32_t t = t0 + lane;
-        const bool valid = t < s.nt && !(s.trans && s_code[k][t] == kSimpleCode);
+        bool valid = t < s.nt && !(s.trans && s_code[k][t] == kSimpleCode);
+        if (TWO_X && valid) {
+          // :586-592 in a 2X batch
+          const uint32_t c = s_code[k][t];
+          const uint32_t mx = umax3(c & 15u, (c >> 4) & 15u, (c >> 8) & 15u);
+          if (!(c & 0x8000u) && mx <= 2) valid = false;
+        }
         const uint32_t code = valid ? (uint32_t)s_code[k][t] : 0u;
         const bool split = valid && (code & 0x8000u);
         const bool part = valid && !split;
@@ -362,7 +566,7 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         runPart += __popcll(vpm);
         const bool writeSplit = split && offSplit < cfg.maxSplitTriangles;
         // build.glsl:68-83 with transient builds: x + y + 2 <= MAX_PART_TRIANGLES
-        const bool writePart = part && ((cfg.flags & TC_FLAG_1X_TRANSIENT)
+        const bool writePart = part && (transient_builds(cfg.flags)
                                             ? (uint64_t)offPart + s.yIncl + 2 <= cfg.maxPartTriangles
                                             : offPart < cfg.maxPartTriangles);
         if (writeSplit || writePart) {
@@ -382,8 +586,12 @@ __global__ __launch_bounds__(256) void k_cluster_classify(KArgs a) {
         }
       }
     }
+    if (TWO_X && lane == 0 && miniTris) atomicAdd(&a.readback->numTotalTriangles, miniTris);
     __syncthreads();
   }
 }
+
+template __global__ void k_cluster_classify<false>(KArgs);
+template __global__ void k_cluster_classify<true>(KArgs);
 
 }  // namespace tc

@@ -8,6 +8,20 @@ namespace tc {
 constexpr uint32_t kMaxClusterVerts = 256;
 constexpr uint32_t kMaxClusterTris = 128;
 
+// TESS_USE_TRANSIENTBUILDS (shaderio.h:140-141): 1X or 2X transient builds
+__host__ __device__ inline bool transient_builds(uint32_t flags) {
+  return (flags & (TC_FLAG_1X_TRANSIENT | TC_FLAG_2X_TRANSIENT)) != 0;
+}
+
+// 2X mini batches (shaderio.h:149-158, cluster_classify.comp.glsl:686-737):
+// up to 8 base triangles whose factors are all <= 2 share one transient
+// CLAS of 6 vertex slots and up to 4 triangles per member; a batch never
+// spans two 32-triangle groups of a cluster (the reference's subgroup).
+constexpr uint32_t kMiniBatch = 8, kMiniTris = 4, kMiniVerts = 6, kMiniGroup = 32;
+constexpr uint32_t kMiniBatchTris = kMiniBatch * kMiniTris, kMiniBatchVerts = kMiniBatch * kMiniVerts;
+constexpr uint32_t kMiniVertexSize = kMiniBatchVerts + (kMiniBatchTris * 3 + 11) / 12;  // vertex slots
+constexpr uint32_t kMiniPartSize = (8 + kMiniBatchTris * 2 + 24 - 1) / 24;             // part slots
+
 // Everything a stage kernel reads: the bindings (device pointers), the
 // configuration and the look-back workspace.
 struct KArgs {
@@ -53,6 +67,7 @@ tc_tuning tuning_snapshot();
 
 __global__ void k_prologue(KArgs a, uint32_t stage);
 __global__ void k_build_setup(KArgs a, int mode);
+template <bool TWO_X>
 __global__ void k_cluster_classify(KArgs a);
 __global__ void k_triangle_split(KArgs a);
 template <bool PN>

@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
 
 // ---------------------------------------------------------- build_setup
 
-// build_setup.comp.glsl:102-264 (1X transient builds with
-// TC_FLAG_1X_TRANSIENT)
+// build_setup.comp.glsl:102-264 (transient builds with TC_FLAG_1X_TRANSIENT
+// or TC_FLAG_2X_TRANSIENT)
 __global__ void k_build_setup(KArgs a, int mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   tc_scene_building* b = a.build;
@@ -110,7 +110,7 @@ __global__ void k_build_setup(KArgs a, int mode) {
     b->splitPassEnd = end;
     b->dispatchTriangleSplit = tc_dispatch_indirect{(end - start + 63) / 64, 1, 1};
   } else if (mode == TC_BUILD_SETUP_BUILD_BLAS) {
-    const bool trans = (cfg.flags & TC_FLAG_1X_TRANSIENT) != 0;
+    const bool trans = transient_builds(cfg.flags);
     uint32_t temp = b->tempInstantiateCounter;
     uint32_t tr = trans ? b->transBuildCounter : 0u;
     rb->numBlasClusters = temp + tr;
@@ -132,7 +132,7 @@ __global__ void k_build_setup(KArgs a, int mode) {
   } else if (mode == TC_BUILD_SETUP_INSTANTIATE_TESS) {
     uint32_t part = b->partTriangleCounter;
     rb->numSplitTriangles = b->splitWriteCounter;
-    if (cfg.flags & TC_FLAG_1X_TRANSIENT) {
+    if (transient_builds(cfg.flags)) {
       // :240-253: the instantiated parts are the accepted prefix
       // x + y + 2 <= MAX_PART_TRIANGLES of the lower counter
       const uint32_t y = (uint32_t)(b->dualPartTriangleCounter >> 32);
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
         ti.subTriangle.triangleID_config = (itp->triangleID_config & 0xFFFFu) | (cf << 16);
         // with transient builds the part list shares its top with their
         // slots (build.glsl:68-83): x + y + 2 <= MAX_PART_TRIANGLES
-        const bool partOk = (cfg.flags & TC_FLAG_1X_TRANSIENT)
+        const bool partOk = transient_builds(cfg.flags)
                                 ? (uint64_t)offPart + transY + 2 <= cfg.maxPartTriangles
                                 : offPart < cfg.maxPartTriangles;
         if (needSplit && offSplit < cfg.maxSplitTriangles)

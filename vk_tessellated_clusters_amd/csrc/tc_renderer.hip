@@ -200,10 +200,10 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&B.blasBuildSizes, 4ull * numInstances},
       {&B.blasClusterAddresses, 8ull * c.maxGeneratedClusters},
       // renderer_raytrace_clusters_tess.cpp:224-229 (1X transient builds)
-      {&B.transBuilds, (c.flags & TC_FLAG_1X_TRANSIENT) ? sizeof(tc_clas_build_info) * (uint64_t)c.maxGeneratedClusters : 0},
-      {&B.transInstanceIDs, (c.flags & TC_FLAG_1X_TRANSIENT) ? 4ull * c.maxGeneratedClusters : 0},
-      {&B.transClusterAddresses, (c.flags & TC_FLAG_1X_TRANSIENT) ? 8ull * c.maxGeneratedClusters : 0},
-      {&B.transClusterSizes, (c.flags & TC_FLAG_1X_TRANSIENT) ? 4ull * c.maxGeneratedClusters : 0},
+      {&B.transBuilds, transient_builds(c.flags) ? sizeof(tc_clas_build_info) * (uint64_t)c.maxGeneratedClusters : 0},
+      {&B.transInstanceIDs, transient_builds(c.flags) ? 4ull * c.maxGeneratedClusters : 0},
+      {&B.transClusterAddresses, transient_builds(c.flags) ? 8ull * c.maxGeneratedClusters : 0},
+      {&B.transClusterSizes, transient_builds(c.flags) ? 4ull * c.maxGeneratedClusters : 0},
       {&B.genClusterData, (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull},
       {&oWs, wsBytes},
       {&oXf, 24ull * 4 * numInstances},
@@ -245,7 +245,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
                       &B.blasBuildInfos, &B.blasBuildSizes, &B.blasClusterAddresses, &B.genClusterData, &B.transBuilds,
                       &B.transInstanceIDs, &B.transClusterAddresses, &B.transClusterSizes})
     *f += base;
-  if (!(c.flags & TC_FLAG_1X_TRANSIENT)) B.transBuilds = B.transInstanceIDs = B.transClusterAddresses = B.transClusterSizes = 0;
+  if (!transient_builds(c.flags)) B.transBuilds = B.transInstanceIDs = B.transClusterAddresses = B.transClusterSizes = 0;
   // :251, :293 the transient index and mapping bytes alias these lists
   B.transTriIndices = B.genVertices;
   B.transTriMappings = B.partTriangles;

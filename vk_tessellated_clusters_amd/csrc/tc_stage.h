@@ -42,10 +42,14 @@ __device__ __forceinline__ void write_tape(const KArgs& a, uint64_t index, v3 ba
 }
 
 // Bytes of our CLAS block for nt triangles and nv vertices (tc_clas_size).
-__device__ __forceinline__ uint32_t clas_block_size(uint32_t nt, uint32_t nv) {
-  const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
-  return (uint32_t)sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u) + ((nv * 12u + 15u) & ~15u);
+__host__ __device__ constexpr uint32_t clas_block_size(uint32_t nt, uint32_t nv) {
+  return (uint32_t)sizeof(tc_clas_header) + ((((nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES) * 24u + 15u) & ~15u) +
+         ((nt * 4u + 15u) & ~15u) + ((nv * 12u + 15u) & ~15u);
 }
+
+// CLAS bytes reserved per 2X mini batch: the worst case of the batch
+// (:732-733 reserves basicClusterSizes[miniBatchTriangles])
+constexpr uint32_t kMiniDataSize = clas_block_size(kMiniBatchTris, kMiniBatchVerts);
 
 __device__ __forceinline__ uint64_t max_data_bytes(const tc_config& c) {
   return (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull;

@@ -435,6 +435,7 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   const uint32_t mode = h.clusterID >> 30;
   uint32_t clusterID = h.clusterID & 0x3FFFFFFFu;
   uint32_t triangleID = h.primitiveID;
+  uint32_t subTriangle = h.primitiveID;
   const v3 bw = mk((1.0f - h.u) - h.v, h.u, h.v);
   v3 base = bw;
   if (mode == TC_RT_CLUSTER_MODE_1X_SUBSET_CLUSTER) {
@@ -444,6 +445,27 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
     const tc_cluster_info ci = *reinterpret_cast<const tc_cluster_info*>(slot);
     triangleID = slot[8 + h.primitiveID];
     clusterID = ci.clusterID;
+  } else if (mode == TC_RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED) {
+    // :152-170 a u16 mapping per batch triangle after the ClusterInfo: the
+    // cluster triangle, the sub-triangle of its config (masked with 0xF; the
+    // reference's `& 4` drops sub-triangles 1-3) and the (factor - 1) bits
+    // the config is rebuilt from
+    const uint8_t* slot = ptr<const uint8_t>(a.build->partTriangles) + (uint64_t)clusterID * 24ull;
+    const tc_cluster_info ci = *reinterpret_cast<const tc_cluster_info*>(slot);
+    const uint32_t packed = reinterpret_cast<const uint16_t*>(slot + 8)[h.primitiveID];
+    triangleID = packed & 0xFFu;
+    subTriangle = (packed >> 8) & 0xFu;
+    uint32_t f[3] = {1u + ((packed >> 12) & 1u), 1u + ((packed >> 13) & 1u), 1u + (packed >> 14)};
+    uint32_t vtx[3] = {0u, 32768u, 32768u << 16};
+    const uint32_t cfg = get_config(f, vtx);
+    clusterID = ci.clusterID;
+    const TableView T = load_table(a.tessTable);
+    const tc_tess_table_entry e = cfg_entry(T, cfg);
+    uint32_t idx[3];
+    cfg_tri_vertices(T, cfg, e.firstTriangle, subTriangle, idx);
+    const v3 nb = interp(cfg_vertex_bary(T, cfg, e.firstVertex, idx[0]), cfg_vertex_bary(T, cfg, e.firstVertex, idx[1]),
+                         cfg_vertex_bary(T, cfg, e.firstVertex, idx[2]), bw);
+    base = interp(decode_bary(vtx[0]), decode_bary(vtx[1]), decode_bary(vtx[2]), nb);
   } else if (mode != TC_RT_CLUSTER_MODE_FULL_CLUSTER) {
     const tc_tess_triangle_info ti = ptr<const tc_tess_triangle_info>(a.build->partTriangles)[clusterID];
     triangleID = ti.subTriangle.triangleID_config & 0xFFFF;
@@ -460,7 +482,7 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   }
   out.clusterID = clusterID;
   out.triangleID = triangleID;
-  out.subTriangleID = h.primitiveID;
+  out.subTriangleID = subTriangle;
   out.baryBase[0] = base.x, out.baryBase[1] = base.y, out.baryBase[2] = base.z;
   // :248-249 object-space hit point from the hit triangle, then world
   const tc_clas_header hdr = *reinterpret_cast<const tc_clas_header*>(hs.clas);
