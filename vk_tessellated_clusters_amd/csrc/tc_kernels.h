@@ -69,6 +69,7 @@ __global__ void k_prologue(KArgs a, uint32_t stage);
 __global__ void k_build_setup(KArgs a, int mode);
 template <bool TWO_X>
 __global__ void k_cluster_classify(KArgs a);
+__global__ void k_mini_vertices(KArgs a);
 __global__ void k_triangle_split(KArgs a);
 template <bool PN>
 __global__ void k_instantiate(KArgs a);
