@@ -624,7 +624,8 @@ tc_status tc_build_setup(const tc_bindings* b, tc_config config, int mode, tc_st
 tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream);
 
 /* triangle_split.comp.glsl:598-632, one pass over
- * [build.splitPassStart, build.splitPassEnd). */
+ * [build.splitPassStart, build.splitPassEnd); with TC_FLAG_PERSISTENT_SPLIT
+ * (:476-594) every pass, SPLIT_PASS setups included, in one launch. */
 tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream);
 
 /* triangle_tess_template_instantiate.comp.glsl:126-373 over build.partTriangles. */

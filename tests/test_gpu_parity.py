@@ -3,6 +3,7 @@ library and the CPU oracle run the same seeded frames and must agree bit
 for bit on every counter, list, record, CLAS block and BLAS reference list,
 and within 2 ULP on generated positions (the kernels reproduce the
 oracle's float evaluation order, so the observed difference is 0 ULP)."""
+import numpy as np
 import pytest
 
 import harness
@@ -111,6 +112,44 @@ def test_transient_2x_capacity_overflow_parity():
         print("capacity", kw, flush=True)
         res, ref = run_both(small_grid(tess_rate_pixels=8.0), harness.small_config(flags=flags, **kw))
         harness.compare(res, ref)
+
+
+def split_queue_error(g):
+    """The persistent split's error word (a wait hit its bound)."""
+    ws = g.r.bindings.workspace
+    return int(g.r.copy(ws + 256 + 12, 4).view(np.uint32)[0])
+
+
+def test_persistent_split_parity():
+    """TC_FLAG_PERSISTENT_SPLIT runs every split pass in one launch; the
+    frame equals the multipass frame and the oracle bit for bit (lists,
+    counters and SPLIT_PASS state), for 4 and 14 passes, with the split list
+    overflowing, and with transient builds."""
+    base = abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING
+    cases = [(4.0, {}, base), (1.0, {}, base), (1.0, {"maxSplitFactor": 2}, base),
+             (1.0, {"maxSplitTriangles": 300}, base), (1.0, {}, base | abi.FLAG_1X_TRANSIENT | abi.FLAG_2X_TRANSIENT)]
+    for rate, kw, flags in cases:
+        host = small_grid(tess_rate_pixels=rate)
+        multi = harness.small_config(flags=flags, **kw)
+        pers = harness.small_config(flags=flags | abi.FLAG_PERSISTENT_SPLIT, **kw)
+        print("case", rate, kw, hex(flags), flush=True)
+        of = harness.OracleFrame(host, multi)
+        of.run(host.frame())
+        ref = of.results()
+        out = []
+        for cfg in (multi, pers):
+            g = harness.GpuFrame(host, cfg)
+            try:
+                g.run(host.frame())
+                g.run(host.frame())  # a second frame reuses the queue state
+                out.append(g.results())
+                if cfg is pers:
+                    assert split_queue_error(g) == 0
+            finally:
+                g.close()
+        assert ref.counters["splitWriteCounter"] > 0
+        harness.compare(out[1], out[0], max_ulp=0)
+        harness.compare(out[1], ref)
 
 
 def test_transient_1x_capacity_overflow_parity():

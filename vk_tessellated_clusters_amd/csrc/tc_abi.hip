@@ -73,7 +73,7 @@ static bool config_ok(const tc_config& c) {
 
 static constexpr uint32_t kSupportedFlags =
     TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT |
-    TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION;
+    TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION | TC_FLAG_PERSISTENT_SPLIT;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -462,16 +462,28 @@ tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream 
   return launched();
 }
 
-tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream) {
+// One split pass, or with TC_FLAG_PERSISTENT_SPLIT all `passes` passes in
+// one persistent launch (renderer_raytrace_clusters_tess.cpp:510-513).
+static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t passes, tc_stream stream) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)1);
   uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
-  uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU);
-  hipLaunchKernelGGL(k_triangle_split, dim3(grid), dim3(64), 0, st, a);
+  uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU));
+  if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
+    if (passes > kMaxSplitPasses) return TC_ERROR_UNSUPPORTED;
+    hipLaunchKernelGGL(k_split_queue_init, dim3(1), dim3(256), 0, st, a, passes);
+    hipLaunchKernelGGL(k_triangle_split_persistent, dim3(grid), dim3(64), 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_triangle_split, dim3(grid), dim3(64), 0, st, a);
+  }
   return launched();
+}
+
+tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream) {
+  return launch_split(b, config, tc_num_split_passes(config.maxSplitFactor), stream);
 }
 
 tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
@@ -529,9 +541,14 @@ tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t n
   if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_CLASSIFY, stream)) != TC_OK) return s;
   if ((s = tc_cluster_classify(b, config, stream)) != TC_OK) return s;
   if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT, stream)) != TC_OK) return s;
-  for (uint32_t p = 0; p < numSplitPasses; p++) {
-    if ((s = tc_triangle_split(b, config, stream)) != TC_OK) return s;
-    if (p + 1 < numSplitPasses && (s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT_PASS, stream)) != TC_OK) return s;
+  if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
+    // every pass, with its SPLIT_PASS setup, in one launch
+    if ((s = launch_split(b, config, numSplitPasses, stream)) != TC_OK) return s;
+  } else {
+    for (uint32_t p = 0; p < numSplitPasses; p++) {
+      if ((s = tc_triangle_split(b, config, stream)) != TC_OK) return s;
+      if (p + 1 < numSplitPasses && (s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT_PASS, stream)) != TC_OK) return s;
+    }
   }
   if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_INSTANTIATE_TESS, stream)) != TC_OK) return s;
   if ((s = tc_triangle_tess_template_instantiate(b, config, stream)) != TC_OK) return s;

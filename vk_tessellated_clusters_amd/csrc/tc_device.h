@@ -428,7 +428,32 @@ struct WsHeader {
   uint32_t _pad[3];
   uint64_t base[8];   // counter values at launch start
 };
-constexpr uint64_t kWsHeaderBytes = 256;
+constexpr uint64_t kWsHeaderBytes = 1024;
+
+// Pass table of the persistent split kernel (TC_FLAG_PERSISTENT_SPLIT),
+// after the header: pass p covers global tickets [tileBegin, tileBegin +
+// numTiles); a pass is published (passReady > p) once the pass before it
+// has completed, and `finished` is set when no pass with tiles is left.
+constexpr uint32_t kMaxSplitPasses = 16;
+struct SplitPass {
+  uint32_t tileBegin, numTiles;
+  uint32_t itemBegin, itemEnd;
+  uint32_t done;   // completed tiles
+  uint32_t epoch;  // look-back epoch of the pass
+  uint64_t b0, b1; // split / part counters at pass start
+};
+struct SplitQueue {
+  uint32_t passReady;
+  uint32_t finished;
+  uint32_t numPasses;
+  uint32_t error;  // a wait exceeded its spin limit
+  uint32_t _pad[4];
+  SplitPass pass[kMaxSplitPasses];
+};
+static_assert(256 + sizeof(SplitQueue) <= kWsHeaderBytes, "split queue exceeds the workspace header");
+__device__ __forceinline__ SplitQueue* split_queue(WsHeader* h) {
+  return reinterpret_cast<SplitQueue*>(reinterpret_cast<char*>(h) + 256);
+}
 
 __device__ __forceinline__ uint64_t atomic_load_agent(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -451,9 +476,12 @@ __device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
 // wave that publishes its aggregate before it waits on anyone; the wait
 // therefore ends.  Status words carry the launch epoch, so words from
 // earlier launches read as "not yet published".
+// `abort` (optional): a word another wave sets to end every wait early; the
+// results are then invalid (the persistent split flags the launch).
 template <int N>
 __device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
-                                         const uint64_t (&agg)[N], uint64_t (&excl)[N]) {
+                                         const uint64_t (&agg)[N], uint64_t (&excl)[N],
+                                         const uint32_t* abort = nullptr) {
   const uint32_t lane = lane_id();
   if (tile == 0) {
     if (lane < (uint32_t)N) atomic_store_agent(&status[lane * (uint64_t)maxTiles], pack_status(epoch, 2, agg[lane]));
@@ -482,6 +510,7 @@ __device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, ui
       }
     }
     if (ballot(!ok) != 0) {
+      if (abort && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
       __builtin_amdgcn_s_sleep(1);
       continue;
     }

@@ -180,134 +180,283 @@ __device__ __forceinline__ void split_child(const TableView& T, const SplitItem&
 }
 
 
-// triangle_split.comp.glsl:598-632 + processAllSubTasks :350-472, one tile
-// of 64 split items per wave; children packed densely over the wave.
-// Counters: 0 split, 1 part.
+// triangle_split.comp.glsl:598-632 + processAllSubTasks :350-472 for one
+// tile of 64 split items (one wave); children packed densely over the wave.
+// Counters: 0 split, 1 part.  The last tile of a pass writes the pass's
+// split / part totals to SceneBuilding.
+__device__ __forceinline__ void split_tile(const KArgs& a, const TableView& T, const ViewParams& vp, SplitItem* s_items,
+                                           uint32_t* s_start, uint32_t tile, uint32_t numTiles, uint32_t epoch,
+                                           uint32_t itemBegin, uint32_t itemEnd, uint64_t B0, uint64_t B1, uint64_t transY,
+                                           const uint32_t* abort = nullptr) {
+  const uint32_t lane = lane_id();
+  tc_scene_building* b = a.build;
+  const tc_config& cfg = a.cfg;
+  // setupTask + fillBaseVertices (:146-199) once per item
+  const uint32_t idx = itemBegin + tile * kWave + lane;
+  bool valid = idx < itemEnd;
+  uint32_t count = 0;
+  {
+    SplitItem it;
+    tc_tess_triangle_info ti;
+    if (valid) {
+      ti = ptr<const tc_tess_triangle_info>(b->splitTriangles)[idx];
+      valid = ti.cluster.instanceID < a.numInstances && cfg_index(ti.subTriangle.triangleID_config >> 16) < TC_TESSTABLE_NUM_LOOKUP &&
+              ti.cluster.clusterID < a.instances[ti.cluster.instanceID].numClusters;
+    }
+    if (valid) {
+      it.cluster = ti.cluster;
+      it.triangleID_config = ti.subTriangle.triangleID_config;
+      it.cfg = ti.subTriangle.triangleID_config >> 16;
+      tc_tess_table_entry e = cfg_entry(T, it.cfg);
+      it.firstTriangle = e.firstTriangle;
+      it.firstVertex = e.firstVertex;
+      count = e.numTriangles;
+      const tc_render_instance& inst = a.instances[ti.cluster.instanceID];
+      const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
+      const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
+      uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        it.pos[v] = xform_point(inst.worldMatrix, ld3(inst.positions, lt[tri * 3 + v] + cl.firstLocalVertex));
+        it.bary[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
+      }
+    }
+    uint32_t incl = wave_incl_scan(count);
+    it.start = incl - count;
+    it.count = count;
+    s_items[lane] = it;
+    s_start[lane] = valid ? it.start : 0xFFFFFFFFu;
+  }
+  const uint32_t total = shfl(wave_incl_scan(count), kWave - 1);
+  __syncthreads();
+
+  // phase A: count children going to the split and part lists
+  uint32_t ns = 0, np = 0;
+  for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+    const uint32_t t = t0 + lane;
+    bool needSplit = false, needPart = false;
+    if (t < total) {
+      const SplitItem& it = s_items[find_owner(s_start, t)];
+      uint32_t vtx[3], f[3];
+      split_child(T, it, t - it.start, vp, vtx, f);
+      uint32_t mx = umax3(f[0], f[1], f[2]);
+      needSplit = mx > TC_TESSTABLE_SIZE;
+      needPart = !needSplit;
+    }
+    ns += __popcll(ballot(needSplit));
+    np += __popcll(ballot(needPart));
+  }
+  uint64_t agg[2] = {ns, np}, excl[2];
+  lookback<2>(a.status, a.maxTiles, epoch, tile, agg, excl, abort);
+
+  // phase B: recompute and emit in (item, child) order
+  uint32_t runSplit = (uint32_t)(B0 + excl[0]), runPart = (uint32_t)(B1 + excl[1]);
+  for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+    const uint32_t t = t0 + lane;
+    bool needSplit = false, needPart = false;
+    uint32_t vtx[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    const SplitItem* itp = nullptr;
+    if (t < total) {
+      itp = &s_items[find_owner(s_start, t)];
+      split_child(T, *itp, t - itp->start, vp, vtx, f);
+      uint32_t mx = umax3(f[0], f[1], f[2]);
+      needSplit = mx > TC_TESSTABLE_SIZE;
+      needPart = !needSplit;
+    }
+    uint64_t vs = ballot(needSplit), vpm = ballot(needPart);
+    uint32_t offSplit = runSplit + mask_prefix(vs);
+    uint32_t offPart = runPart + mask_prefix(vpm);
+    runSplit += __popcll(vs);
+    runPart += __popcll(vpm);
+    if (needSplit || needPart) {
+      if (needSplit) {
+        f[0] = split_factor(f[0], cfg.maxSplitFactor);
+        f[1] = split_factor(f[1], cfg.maxSplitFactor);
+        f[2] = split_factor(f[2], cfg.maxSplitFactor);
+      }
+      uint32_t cf = get_config(f, vtx);
+      tc_tess_triangle_info ti;
+      ti.cluster = itp->cluster;
+      ti.subTriangle.vtxEncoded[0] = vtx[0];
+      ti.subTriangle.vtxEncoded[1] = vtx[1];
+      ti.subTriangle.vtxEncoded[2] = vtx[2];
+      ti.subTriangle.triangleID_config = (itp->triangleID_config & 0xFFFFu) | (cf << 16);
+      // with transient builds the part list shares its top with their
+      // slots (build.glsl:68-83): x + y + 2 <= MAX_PART_TRIANGLES
+      const bool partOk = transient_builds(cfg.flags)
+                              ? (uint64_t)offPart + transY + 2 <= cfg.maxPartTriangles
+                              : offPart < cfg.maxPartTriangles;
+      if (needSplit && offSplit < cfg.maxSplitTriangles)
+        ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
+      else if (needPart && partOk)
+        ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
+    }
+  }
+
+  if (tile == numTiles - 1 && lane == 0) {
+    b->splitWriteCounter = (uint32_t)(B0 + excl[0] + agg[0]);
+    b->partTriangleCounter = (uint32_t)(B1 + excl[1] + agg[1]);
+    b->dualPartTriangleCounter = (transY << 32) | (uint32_t)(B1 + excl[1] + agg[1]);
+  }
+  __syncthreads();
+}
+
+// One split pass (multipass form, the reference default): tiles by ticket.
 __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
   __shared__ SplitItem s_items[kWave];
   __shared__ uint32_t s_start[kWave];
-
   WsHeader* h = a.ws;
-  const uint32_t lane = lane_id();
   const uint32_t epoch = h->epoch, numTiles = h->numTiles, itemBegin = h->itemBegin, itemEnd = h->itemEnd;
   const uint64_t B0 = h->base[0], B1 = h->base[1], transY = h->base[2];
   const ViewParams vp = view_params(a);
   const TableView T = load_table(a.tessTable);
-  tc_scene_building* b = a.build;
-  const tc_config cfg = a.cfg;
-
   while (true) {
     const uint32_t tile = take_ticket(h);
     if (tile >= numTiles) break;
+    split_tile(a, T, vp, s_items, s_start, tile, numTiles, epoch, itemBegin, itemEnd, B0, B1, transY);
+  }
+}
 
-    // setupTask + fillBaseVertices (:146-199) once per item
-    const uint32_t idx = itemBegin + tile * kWave + lane;
-    bool valid = idx < itemEnd;
-    uint32_t count = 0;
-    {
-      SplitItem it;
-      tc_tess_triangle_info ti;
-      if (valid) {
-        ti = ptr<const tc_tess_triangle_info>(b->splitTriangles)[idx];
-        valid = ti.cluster.instanceID < a.numInstances && cfg_index(ti.subTriangle.triangleID_config >> 16) < TC_TESSTABLE_NUM_LOOKUP &&
-                ti.cluster.clusterID < a.instances[ti.cluster.instanceID].numClusters;
-      }
-      if (valid) {
-        it.cluster = ti.cluster;
-        it.triangleID_config = ti.subTriangle.triangleID_config;
-        it.cfg = ti.subTriangle.triangleID_config >> 16;
-        tc_tess_table_entry e = cfg_entry(T, it.cfg);
-        it.firstTriangle = e.firstTriangle;
-        it.firstVertex = e.firstVertex;
-        count = e.numTriangles;
-        const tc_render_instance& inst = a.instances[ti.cluster.instanceID];
-        const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
-        const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
-        uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
-#pragma unroll
-        for (int v = 0; v < 3; v++) {
-          it.pos[v] = xform_point(inst.worldMatrix, ld3(inst.positions, lt[tri * 3 + v] + cl.firstLocalVertex));
-          it.bary[v] = decode_bary(ti.subTriangle.vtxEncoded[v]);
-        }
-      }
-      uint32_t incl = wave_incl_scan(count);
-      it.start = incl - count;
-      it.count = count;
-      s_items[lane] = it;
-      s_start[lane] = valid ? it.start : 0xFFFFFFFFu;
+// ---------------------------------------------- persistent triangle_split
+
+// Loads of words other waves write during the launch (never cached in the
+// scalar cache).
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// build_setup SPLIT_PASS (build_setup.comp.glsl:168-190) for the passes
+// after `p`, publishing the first of them that has items; when none is
+// left the queue is finished.  One thread, after pass p completed.
+__device__ void publish_split_passes(KArgs a, SplitQueue* q, WsHeader* h, uint32_t p) {
+  tc_scene_building* b = a.build;
+  const tc_config& cfg = a.cfg;
+  uint32_t tileEnd = ld_agent(&q->pass[p].tileBegin) + ld_agent(&q->pass[p].numTiles), epoch = ld_agent(&q->pass[p].epoch);
+  const uint32_t numPasses = ld_agent(&q->numPasses);
+  for (uint32_t np = p + 1; np < numPasses; np++) {
+    b->splitPass = ld_agent(&b->splitPass) + 1;
+    const uint32_t start = min(ld_agent(&b->splitPassEnd), cfg.maxSplitTriangles);
+    const uint32_t writeCounter = ld_agent(&b->splitWriteCounter);
+    const uint32_t end = min(writeCounter, cfg.maxSplitTriangles);
+    b->splitPassStart = start;
+    b->splitPassEnd = end;
+    b->dispatchTriangleSplit = tc_dispatch_indirect{(end - start + 63) / 64, 1, 1};
+    const uint32_t n = end > start ? end - start : 0u;
+    SplitPass& nx = q->pass[np];
+    nx.tileBegin = tileEnd;
+    nx.numTiles = min((n + kWave - 1) / kWave, a.maxTiles);
+    nx.itemBegin = start;
+    nx.itemEnd = start + n;
+    nx.done = 0;
+    nx.epoch = ++epoch;
+    nx.b0 = writeCounter;
+    nx.b1 = ld_agent(&b->partTriangleCounter);
+    h->epoch = epoch;  // the next stage's prologue continues from here
+    if (nx.numTiles) {
+      __hip_atomic_store(&q->passReady, np + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return;
     }
-    const uint32_t total = shfl(wave_incl_scan(count), kWave - 1);
+  }
+  __hip_atomic_store(&q->passReady, numPasses, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&q->finished, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass 0 of the persistent split from the prologue's snapshot (workspace
+// header), with look-back epochs for every pass.
+__global__ __launch_bounds__(256) void k_split_queue_init(KArgs a, uint32_t numPasses) {
+  __shared__ uint32_t s_wrap;
+  WsHeader* h = a.ws;
+  SplitQueue* q = split_queue(h);
+  if (threadIdx.x == 0) s_wrap = h->epoch + numPasses >= (1u << kEpochBits);
+  __syncthreads();
+  if (s_wrap) {
+    // the passes' epochs would wrap: clear the status words, restart at 1
+    const uint64_t n = (uint64_t)a.maxTiles * kMaxCounters;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) a.status[i] = 0;
+    __threadfence();
     __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (s_wrap) h->epoch = 1;
+  q->numPasses = min(numPasses, kMaxSplitPasses);
+  q->error = 0;
+  q->finished = 0;
+  SplitPass& p0 = q->pass[0];
+  p0.tileBegin = 0;
+  p0.numTiles = h->numTiles;
+  p0.itemBegin = h->itemBegin;
+  p0.itemEnd = h->itemEnd;
+  p0.done = 0;
+  p0.epoch = h->epoch;
+  p0.b0 = h->base[0];
+  p0.b1 = h->base[1];
+  if (q->numPasses == 0) {
+    q->passReady = 0;
+    q->finished = 1;
+  } else if (p0.numTiles) {
+    q->passReady = 1;
+  } else {
+    q->passReady = 1;
+    publish_split_passes(a, q, h, 0);
+  }
+  __threadfence();
+}
 
-    // phase A: count children going to the split and part lists
-    uint32_t ns = 0, np = 0;
-    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
-      const uint32_t t = t0 + lane;
-      bool needSplit = false, needPart = false;
-      if (t < total) {
-        const SplitItem& it = s_items[find_owner(s_start, t)];
-        uint32_t vtx[3], f[3];
-        split_child(T, it, t - it.start, vp, vtx, f);
-        uint32_t mx = umax3(f[0], f[1], f[2]);
-        needSplit = mx > TC_TESSTABLE_SIZE;
-        needPart = !needSplit;
-      }
-      ns += __popcll(ballot(needSplit));
-      np += __popcll(ballot(needPart));
-    }
-    uint64_t agg[2] = {ns, np}, excl[2];
-    lookback<2>(a.status, a.maxTiles, epoch, tile, agg, excl);
-
-    // phase B: recompute and emit in (item, child) order
-    uint32_t runSplit = (uint32_t)(B0 + excl[0]), runPart = (uint32_t)(B1 + excl[1]);
-    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
-      const uint32_t t = t0 + lane;
-      bool needSplit = false, needPart = false;
-      uint32_t vtx[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-      const SplitItem* itp = nullptr;
-      if (t < total) {
-        itp = &s_items[find_owner(s_start, t)];
-        split_child(T, *itp, t - itp->start, vp, vtx, f);
-        uint32_t mx = umax3(f[0], f[1], f[2]);
-        needSplit = mx > TC_TESSTABLE_SIZE;
-        needPart = !needSplit;
-      }
-      uint64_t vs = ballot(needSplit), vpm = ballot(needPart);
-      uint32_t offSplit = runSplit + mask_prefix(vs);
-      uint32_t offPart = runPart + mask_prefix(vpm);
-      runSplit += __popcll(vs);
-      runPart += __popcll(vpm);
-      if (needSplit || needPart) {
-        if (needSplit) {
-          f[0] = split_factor(f[0], cfg.maxSplitFactor);
-          f[1] = split_factor(f[1], cfg.maxSplitFactor);
-          f[2] = split_factor(f[2], cfg.maxSplitFactor);
+// triangle_split.comp.glsl:476-594 (TESS_USE_PERSISTENT_KERNEL) as one
+// launch over every pass: tickets are global over the passes, a ticket of
+// a pass not yet published waits for it, and the wave completing a pass
+// publishes the next.  Every ticket before a waiting one is held by a
+// running wave that does not wait on anything later, so the waits end; a
+// wait is still bounded and sets `error` when the bound is hit.  The split
+// and part lists, counters and SPLIT_PASS state equal the multipass form's.
+__global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
+  __shared__ SplitItem s_items[kWave];
+  __shared__ uint32_t s_start[kWave];
+  WsHeader* h = a.ws;
+  SplitQueue* q = split_queue(h);
+  const uint32_t lane = lane_id();
+  const uint64_t transY = h->base[2];
+  const ViewParams vp = view_params(a);
+  const TableView T = load_table(a.tessTable);
+  uint32_t p = 0;
+  while (true) {
+    const uint32_t g = take_ticket(h);
+    uint32_t found = 0xFFFFFFFFu;
+    if (lane == 0) {
+      for (uint32_t spins = 0;; spins++) {
+        const uint32_t ready = __hip_atomic_load(&q->passReady, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        while (p < ready && g >= ld_agent(&q->pass[p].tileBegin) + ld_agent(&q->pass[p].numTiles)) p++;
+        if (p < ready) {
+          found = p;
+          break;
         }
-        uint32_t cf = get_config(f, vtx);
-        tc_tess_triangle_info ti;
-        ti.cluster = itp->cluster;
-        ti.subTriangle.vtxEncoded[0] = vtx[0];
-        ti.subTriangle.vtxEncoded[1] = vtx[1];
-        ti.subTriangle.vtxEncoded[2] = vtx[2];
-        ti.subTriangle.triangleID_config = (itp->triangleID_config & 0xFFFFu) | (cf << 16);
-        // with transient builds the part list shares its top with their
-        // slots (build.glsl:68-83): x + y + 2 <= MAX_PART_TRIANGLES
-        const bool partOk = transient_builds(cfg.flags)
-                                ? (uint64_t)offPart + transY + 2 <= cfg.maxPartTriangles
-                                : offPart < cfg.maxPartTriangles;
-        if (needSplit && offSplit < cfg.maxSplitTriangles)
-          ptr<tc_tess_triangle_info>(b->splitTriangles)[offSplit] = ti;
-        else if (needPart && partOk)
-          ptr<tc_tess_triangle_info>(b->partTriangles)[offPart] = ti;
+        if (__hip_atomic_load(&q->finished, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
+        // a wait far beyond any frame's split time: flag the launch and end
+        // every wait (look-backs included) so the grid drains
+        if (spins >= (1u << 24) || ld_agent(&q->error)) {
+          __hip_atomic_store(&q->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
       }
     }
-
-    if (tile == numTiles - 1 && lane == 0) {
-      b->splitWriteCounter = (uint32_t)(B0 + excl[0] + agg[0]);
-      b->partTriangleCounter = (uint32_t)(B1 + excl[1] + agg[1]);
-      b->dualPartTriangleCounter = (transY << 32) | (uint32_t)(B1 + excl[1] + agg[1]);
+    found = uniform(found);
+    if (found == 0xFFFFFFFFu) break;
+    p = found;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const SplitPass& Q = q->pass[p];
+    const uint32_t tileBegin = ld_agent(&Q.tileBegin), numTiles = ld_agent(&Q.numTiles);
+    split_tile(a, T, vp, s_items, s_start, g - tileBegin, numTiles, ld_agent(&Q.epoch), ld_agent(&Q.itemBegin),
+               ld_agent(&Q.itemEnd), ld_agent(&Q.b0), ld_agent(&Q.b1), transY, &q->error);
+    // completion: this tile's lists and counters are visible before it counts
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) {
+      const uint32_t old = __hip_atomic_fetch_add(&q->pass[p].done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == numTiles) publish_split_passes(a, q, h, p);
     }
-    __syncthreads();
   }
 }
 
