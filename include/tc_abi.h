@@ -603,6 +603,22 @@ tc_status tc_device_synchronize(void);
  * reference takes from its template builds (raytracing_cluster_data.cpp:362-388). */
 uint32_t tc_clas_size(uint32_t numTriangles, uint32_t numVertices);
 
+/* Host-side mesh clustering (stands for meshopt_buildMeshletsSpatial and
+ * the cluster bounds / per-cluster vertices of scene.cpp:396-552):
+ * clusters of at most maxTriangles (<= 128) triangles and maxVertices
+ * (<= 256) distinct vertices by recursive bisection of the triangle
+ * centroids, deterministic.  Outputs, sized by the caller: clusters and
+ * clusterBboxes numTriangles entries, clusterLocalVertices and
+ * clusterLocalTriangles 3 x numTriangles entries.  clusterLocalVertices[i]
+ * is the mesh vertex behind cluster vertex i (the caller gathers the
+ * per-cluster vertex arrays from it); clusterLocalTriangles holds 3 u8
+ * cluster-local indices per triangle.  Returns the cluster and cluster
+ * vertex counts. */
+tc_status tc_clusterize_mesh(const float* positions, uint32_t numVertices, const uint32_t* indices,
+                             uint32_t numTriangles, uint32_t maxTriangles, uint32_t maxVertices, tc_cluster* clusters,
+                             uint32_t* clusterLocalVertices, uint8_t* clusterLocalTriangles, tc_bbox* clusterBboxes,
+                             uint32_t* numClusters, uint32_t* numClusterVertices);
+
 /* Builds a per-cluster template in host memory: writes the packed triangle
  * words in group order into trianglesOut (numTriangles u32).
  * localTriangles: 3 x u8 per triangle; positions: float3 per vertex. */

@@ -114,6 +114,18 @@ def test_transient_2x_capacity_overflow_parity():
         harness.compare(res, ref)
 
 
+def test_loaded_gltf_scene_parity(tmp_path):
+    """A glTF scene (clusters from tc_clusterize_mesh, R16 height maps
+    from PNG files, node transforms) through the GPU frame and the oracle."""
+    from test_assets import gltf_doc
+    from vk_tessellated_clusters_amd import assets
+    path, _ = gltf_doc(tmp_path)
+    host = assets.load_gltf_scene(str(path), width=640, height=360, tess_rate_pixels=2.0)
+    res, ref = run_both(host, harness.small_config())
+    assert ref.readback["numTotalTriangles"] > host.num_base_triangles
+    harness.compare(res, ref)
+
+
 def split_queue_error(g):
     """The persistent split's error word (a wait hit its bound)."""
     ws = g.r.bindings.workspace

@@ -281,6 +281,7 @@ EXPORTS = [
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
     "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
     "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning", "tc_clas_build_transient",
+    "tc_clusterize_mesh",
 ]
 
 
@@ -302,6 +303,7 @@ def lib():
         "tc_table_lookup_host": (i32, [P]),
         "tc_clas_size": (u32, [u32, u32]),
         "tc_cluster_template_order": (i32, [u32, u32, P, P, P]),
+        "tc_clusterize_mesh": (i32, [P, u32, P, u32, u32, u32, P, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
         "tc_instances_classify": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_clusters_cull": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_build_setup": (i32, [C.POINTER(Bindings), Config, C.c_int, P]),

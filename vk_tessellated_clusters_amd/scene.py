@@ -92,6 +92,54 @@ def grid_patch(nx: int, nz: int, size_x: float, size_z: float, tile: int = 8) ->
     return _finish(P, N, U, T)
 
 
+def vertex_normals(positions: np.ndarray, indices: np.ndarray) -> np.ndarray:
+    """Area-weighted vertex normals (FP64 accumulation) for meshes that
+    carry none."""
+    p = positions.astype(np.float64)
+    t = indices.reshape(-1, 3).astype(np.int64)
+    fn = np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, t[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    n = np.where(ln > 0, n / np.where(ln > 0, ln, 1.0), np.array([0.0, 1.0, 0.0]))
+    return n.astype(np.float32)
+
+
+def mesh_geometry(positions, indices, normals=None, texcoords=None, max_triangles: int = 128,
+                  max_vertices: int = 128) -> Geometry:
+    """Geometry of an indexed triangle mesh: clusters from the library's
+    host clusterizer (tc_clusterize_mesh, the counterpart of the reference's
+    meshoptimizer step, scene.cpp:396-552) with per-cluster vertex ranges and
+    8-bit local indices.  Missing normals are area-weighted vertex normals,
+    missing texture coordinates zero."""
+    import ctypes as C
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    if idx.size % 3:
+        raise ValueError("indices must hold whole triangles")
+    nt, nv = idx.size // 3, pos.shape[0]
+    nrm = vertex_normals(pos, idx) if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    uv = np.zeros((nv, 2), np.float32) if texcoords is None else np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+    if nrm.shape[0] != nv or uv.shape[0] != nv:
+        raise ValueError("normals / texcoords must have one entry per position")
+    clusters = np.zeros(max(nt, 1), abi.CLUSTER_DT)
+    bboxes = np.zeros(max(nt, 1), abi.BBOX_DT)
+    lverts = np.zeros(max(3 * nt, 1), np.uint32)
+    ltris = np.zeros(max(3 * nt, 1), np.uint8)
+    ncl, ncv = C.c_uint32(0), C.c_uint32(0)
+    abi.check(abi.lib().tc_clusterize_mesh(pos.ctypes.data, nv, idx.ctypes.data, nt, max_triangles, max_vertices,
+                                           clusters.ctypes.data, lverts.ctypes.data, ltris.ctypes.data,
+                                           bboxes.ctypes.data, C.byref(ncl), C.byref(ncv)), "tc_clusterize_mesh")
+    nc, ncv = ncl.value, ncv.value
+    lv = lverts[:ncv]
+    g = Geometry(np.ascontiguousarray(pos[lv]), np.ascontiguousarray(nrm[lv]), np.ascontiguousarray(uv[lv]),
+                 clusters[:nc].copy(), np.ascontiguousarray(ltris[:3 * nt]), bboxes[:nc].copy())
+    g.lo, g.hi = pos.min(0), pos.max(0)
+    g.source_vertices = lv.copy()  # mesh vertex behind every cluster vertex
+    return g
+
+
 _ICO_V = None
 
 
