@@ -595,7 +595,8 @@ tc_status tc_table_lookup_host(tc_tess_table_entry* out4096);
  * words; pass words = NULL to query the count), sizes4096 the CLAS sizes. */
 tc_status tc_table_templates_host(tc_clas_template* out4096, uint32_t* sizes4096, uint32_t* words, uint32_t* numWords);
 
-/* Synchronous copy between any host / device pointers (hipMemcpyDefault). */
+/* Synchronous copy between any host / device pointers (hipMemcpyDefault),
+ * after all device work queued so far on any stream has finished. */
 tc_status tc_memcpy(void* dst, const void* src, uint64_t bytes);
 tc_status tc_device_synchronize(void);
 

@@ -341,6 +341,9 @@ tc_status tc_table_templates_host(tc_clas_template* out4096, uint32_t* sizes4096
 tc_status tc_memcpy(void* dst, const void* src, uint64_t bytes) {
   if ((!dst || !src) && bytes) return TC_ERROR_INVALID_ARGUMENT;
   if (!bytes) return TC_OK;
+  // after all queued device work, on every stream (readbacks follow frames
+  // queued on non-blocking streams)
+  if (hipDeviceSynchronize() != hipSuccess) return TC_ERROR_LAUNCH;
   return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 
