@@ -497,6 +497,12 @@ typedef struct tc_hit_decoded {
    per generated vertex (tc_bindings.vertexTape), the input of
    tc_depth_adjoint. */
 #define TC_FLAG_VERTEX_TAPE (1u << 7)
+/* Occlusion culling of instances against the far HiZ of the previous
+   frame's traced depth (instances_classify.comp.glsl:120 with
+   culling.glsl:96-114), bound as tc_bindings.hiz; the renderer builds the
+   HiZ after every camera trace.  The reference does this whenever culling
+   is on; here it is a flag so frames without a HiZ stay reproducible. */
+#define TC_FLAG_HIZ (1u << 8)
 
 /* The reference's specialisation macros
  * (renderer_raytrace_clusters_tess.cpp:117-138). */
@@ -531,6 +537,22 @@ typedef struct tc_vertex_tape {
   uint32_t _pad;
 } tc_vertex_tape;
 
+/* Far HiZ pyramid (nvhiz_vk.cpp:278-320 with hizFarLevel 0,
+ * nvhiz-update.comp.glsl): level 0 holds the maximum depth of each 2 x 2
+ * block of the width x height depth image, every further level the maximum
+ * of 2 x 2 texels of the level before, in a size x size allocation (size the
+ * power of two >= max(width, height) / 2); levels are stored back to back,
+ * level k being (size >> k)^2 floats. */
+typedef struct tc_hiz {
+  float* texels;       /* device memory, tc_hiz_setup's byte count */
+  uint32_t size;
+  uint32_t mipLevels;
+  uint32_t usedWidth;  /* width / 2 */
+  uint32_t usedHeight; /* height / 2 */
+  uint32_t width;      /* source depth image */
+  uint32_t height;
+} tc_hiz;
+
 typedef struct tc_bindings {
   const tc_frame_constants* view;                      /* FRAME_UBO        */
   const tc_tessellation_table* tessTable;              /* TESSTABLE_UBO    */
@@ -544,6 +566,10 @@ typedef struct tc_bindings {
   uint64_t workspaceBytes;
   struct tc_vertex_tape* vertexTape; /* maxGeneratedVertices records, only read
                                         with TC_FLAG_VERTEX_TAPE */
+  /* culling.glsl's viewLast: the frame constants the HiZ depth was traced
+   * with (NULL: view); used by tc_instances_classify */
+  const tc_frame_constants* viewLast;
+  tc_hiz hiz; /* texels NULL: no occlusion test (with TC_FLAG_HIZ) */
 } tc_bindings;
 
 /* ------------------------------------------------------------ entry points */
@@ -626,9 +652,27 @@ tc_status tc_clusterize_mesh(const float* positions, uint32_t numVertices, const
 tc_status tc_cluster_template_order(uint32_t numTriangles, uint32_t numVertices, const uint8_t* localTriangles,
                                     const float* positions, uint32_t* trianglesOut);
 
-/* instances_classify.comp.glsl:100-127: per-instance frustum state bits
- * (no HiZ pyramid exists here, so the occlusion test passes). */
+/* instances_classify.comp.glsl:100-127: per-instance frustum, size and
+ * (with TC_FLAG_HIZ and a bound HiZ) occlusion state bits against viewLast. */
 tc_status tc_instances_classify(const tc_bindings* b, tc_config config, tc_stream stream);
+
+/* Host: the HiZ layout for a width x height depth image (6 .. 65536 per
+ * axis; nvhiz_vk.cpp:
+ * 278-320): size, levels and used texels into *hiz (texels untouched), its
+ * byte count, and the frame constants' hizSizeFactors / hizSizeMax
+ * (nvhiz_vk.cpp:29-40; tessellatedclusters.cpp:615-619). */
+tc_status tc_hiz_setup(uint32_t width, uint32_t height, tc_hiz* hiz, uint64_t* bytes, float sizeFactors[4],
+                       float* sizeMax);
+
+/* Fills every HiZ texel with the far depth 1.0 (an unbuilt HiZ occludes
+ * nothing). */
+tc_status tc_hiz_clear(const tc_hiz* hiz, tc_stream stream);
+
+/* nvhiz-update.comp.glsl: builds every level of the far HiZ from a depth
+ * image of hiz->width x hiz->height floats, pixel p at
+ * depth + p * pixelStride bytes (row-major; e.g. tc_hit_decoded.depth with
+ * pixelStride sizeof(tc_hit_decoded)); source reads clamp to the image. */
+tc_status tc_hiz_build(const tc_hiz* hiz, const void* depth, uint32_t pixelStride, tc_stream stream);
 
 /* clusters_cull.comp.glsl:112-161 in ray-tracing mode: appends every
  * cluster of every instance, instance by instance, to build.visibleClusters. */
@@ -842,7 +886,8 @@ static_assert(sizeof(tc_bvh4_node) == 128, "bvh4 node");
 static_assert(sizeof(tc_clas_build_info) == 64, "ClasBuildInfo");
 static_assert(sizeof(tc_clas_template) == 32, "clas template");
 static_assert(sizeof(tc_vertex_tape) == 48, "vertex tape");
-static_assert(sizeof(tc_bindings) == 80, "bindings");
+static_assert(sizeof(tc_bindings) == 120, "bindings");
+static_assert(sizeof(tc_hiz) == 32, "hiz");
 static_assert(sizeof(tc_adjoint_params) == 80, "adjoint params");
 #endif
 

@@ -947,6 +947,57 @@ void mat4_mul(const float* a, const float* b, float* o) {
       o[col * 4 + r] = ((a[r] * b[col * 4 + 0] + a[4 + r] * b[col * 4 + 1]) + a[8 + r] * b[col * 4 + 2]) + a[12 + r] * b[col * 4 + 3];
 }
 
+/* ---------------------------------------------------------------- HiZ */
+
+/* ceil(log2(x)) from the binary exponent, 0 for x <= 0 */
+int ceil_log2(float x) {
+  if (!(x > 0.0f)) return 0;
+  if (std::isinf(x)) return 1 << 20;
+  int e;
+  float m = frexpf(x, &e);
+  return m == 0.5f ? e - 1 : e;
+}
+
+uint64_t hiz_level_offset(uint32_t size, uint32_t level) {
+  uint64_t o = 0;
+  for (uint32_t k = 0; k < level; k++) o += uint64_t(size >> k) * (size >> k);
+  return o;
+}
+
+/* textureLod through the far sampler (nvhiz_vk.cpp:84-114: linear filter
+ * with MAX reduction, nearest mip, clamp to edge): the maximum of the 2x2
+ * footprint texels with non-zero weight */
+float hiz_sample(const tc_hiz& h, float u, float v, int level) {
+  uint32_t lv = (uint32_t)std::min(std::max(level, 0), (int)h.mipLevels - 1);
+  int dim = (int)(h.size >> lv);
+  const float* t = h.texels + hiz_level_offset(h.size, lv);
+  float x = u * (float)dim - 0.5f, y = v * (float)dim - 0.5f;
+  float x0 = floorf(x), y0 = floorf(y);
+  bool wx = x - x0 > 0.0f, wy = y - y0 > 0.0f;
+  int ix0 = std::min(std::max((int)x0, 0), dim - 1), ix1 = std::min(std::max((int)x0 + 1, 0), dim - 1);
+  int iy0 = std::min(std::max((int)y0, 0), dim - 1), iy1 = std::min(std::max((int)y0 + 1, 0), dim - 1);
+  float m = t[uint64_t(iy0) * dim + ix0];
+  if (wx) m = std::max(m, t[uint64_t(iy0) * dim + ix1]);
+  if (wy) {
+    m = std::max(m, t[uint64_t(iy1) * dim + ix0]);
+    if (wx) m = std::max(m, t[uint64_t(iy1) * dim + ix1]);
+  }
+  return m;
+}
+
+/* culling.glsl:96-114 intersectHiz */
+bool intersect_hiz(const tc_hiz& h, const tc_frame_constants* vl, const float cmin[4], const float cmax[4]) {
+  float x0 = cmin[0] * 0.5f + 0.5f, y0 = cmin[1] * 0.5f + 0.5f;
+  float x1 = cmax[0] * 0.5f + 0.5f, y1 = cmax[1] * 0.5f + 0.5f;
+  x0 *= vl->hizSizeFactors[0], y0 *= vl->hizSizeFactors[1];
+  x1 *= vl->hizSizeFactors[0], y1 *= vl->hizSizeFactors[1];
+  x0 = std::min(x0, vl->hizSizeFactors[2]), y0 = std::min(y0, vl->hizSizeFactors[3]);
+  x1 = std::min(x1, vl->hizSizeFactors[2]), y1 = std::min(y1, vl->hizSizeFactors[3]);
+  float maxsize = std::max(x1 - x0, y1 - y0) * vl->hizSizeMax;
+  float depth = hiz_sample(h, (x0 + x1) * 0.5f, (y0 + y1) * 0.5f, ceil_log2(maxsize));
+  return cmin[2] <= depth + 2.0f / (float)(1 << 24);
+}
+
 }  // namespace
 
 /* ======================================================== extern "C" */
@@ -991,13 +1042,44 @@ void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]
 }
 float oracle_sample_height(const tc_displacement_texture* tex, float u, float v) { return sample_height(*tex, u, v); }
 float oracle_ripple_sin(float x) { return ripple_sincos(x, false); }
+
+/* nvhiz-update.comp.glsl: level 0 the max of 2x2 source blocks (reads
+ * clamped to the image), each further level the max of 2x2 texels */
+void oracle_hiz_build(const tc_hiz* h, const void* depth, uint32_t stride) {
+  const char* d = static_cast<const char*>(depth);
+  auto at = [&](uint32_t x, uint32_t y) {
+    float v;
+    memcpy(&v, d + (uint64_t(std::min(y, h->height - 1)) * h->width + std::min(x, h->width - 1)) * stride, 4);
+    return v;
+  };
+  for (uint32_t y = 0; y < h->size; y++)
+    for (uint32_t x = 0; x < h->size; x++)
+      h->texels[uint64_t(y) * h->size + x] =
+          std::max(std::max(std::max(at(2 * x, 2 * y), at(2 * x + 1, 2 * y)), at(2 * x, 2 * y + 1)), at(2 * x + 1, 2 * y + 1));
+  for (uint32_t k = 1; k < h->mipLevels; k++) {
+    uint32_t dim = h->size >> k;
+    const float* s = h->texels + hiz_level_offset(h->size, k - 1);
+    float* o = h->texels + hiz_level_offset(h->size, k);
+    for (uint32_t y = 0; y < dim; y++)
+      for (uint32_t x = 0; x < dim; x++)
+        o[uint64_t(y) * dim + x] = std::max(std::max(std::max(s[2ull * y * 2 * dim + 2 * x], s[2ull * y * 2 * dim + 2 * x + 1]),
+                                                     s[(2ull * y + 1) * 2 * dim + 2 * x]),
+                                            s[(2ull * y + 1) * 2 * dim + 2 * x + 1]);
+  }
+}
+
+int oracle_intersect_hiz(const tc_hiz* hiz, const tc_frame_constants* viewLast, const float clipMin[4],
+                         const float clipMax[4]) {
+  return intersect_hiz(*hiz, viewLast, clipMin, clipMax) ? 1 : 0;
+}
 float oracle_ripple_cos(float x) { return ripple_sincos(x, true); }
 
 /* instances_classify.comp.glsl:100-127 with culling.glsl:30-90 (frustum and
  * size test; no HiZ pyramid exists, so the occlusion test passes). */
-void oracle_instances_classify(const oracle_ctx* c, tc_config) {
+void oracle_instances_classify(const oracle_ctx* c, tc_config cfg) {
   tc_scene_building* b = c->build;
-  const tc_frame_constants* vw = c->view;
+  const tc_frame_constants* vw = c->viewLast ? c->viewLast : c->view; /* culling.glsl reads viewLast */
+  const bool hiz = (cfg.flags & TC_FLAG_HIZ) && c->hiz.texels;
   for (uint32_t i = 0; i < c->numRenderInstances; i++) {
     const tc_render_instance& inst = c->instances[i];
     float wvp[16];
@@ -1034,7 +1116,9 @@ void oracle_instances_classify(const oracle_ctx* c, tc_config) {
     }
     bool inFrustum = bits == 0;
     bool sizeOk = (cmax[0] - cmin[0]) > 2.0f / vw->viewportf[0] || (cmax[1] - cmin[1]) > 2.0f / vw->viewportf[1];
-    bool visible = inFrustum && (!clipValid || sizeOk);
+    /* :120 occlusion against the previous frame's far HiZ */
+    bool hizOk = !hiz || !(inFrustum && clipValid && sizeOk) || intersect_hiz(c->hiz, vw, cmin, cmax);
+    bool visible = inFrustum && (!clipValid || (sizeOk && hizOk));
     P<uint32_t>(b->instanceStates)[i] = (inFrustum ? TC_INSTANCE_FRUSTUM_BIT : 0u) | (visible ? TC_INSTANCE_VISIBLE_BIT : 0u);
     P<tc_blas_build_info>(b->blasBuildInfos)[i].clusterReferencesCount = 0;
   }

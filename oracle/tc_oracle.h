@@ -70,7 +70,17 @@ typedef struct oracle_ctx {
   /* value (not dereferenced) the genVertices field stands for in the
    * vertexBufferAddress records; 0 means use build->genVertices */
   uint64_t genVerticesValue;
+  /* culling constants of the frame the HiZ was traced with (NULL: view)
+   * and the far HiZ (host texels; NULL: no occlusion test) */
+  const tc_frame_constants* viewLast;
+  tc_hiz hiz;
 } oracle_ctx;
+
+/* nvhiz-update.comp.glsl: the far HiZ pyramid of a depth image (host) */
+void oracle_hiz_build(const tc_hiz* hiz, const void* depth, uint32_t pixelStride);
+/* culling.glsl:96-114 for one clip-space rectangle (xy clamped to [-1, 1]) */
+int oracle_intersect_hiz(const tc_hiz* hiz, const tc_frame_constants* viewLast, const float clipMin[4],
+                         const float clipMax[4]);
 
 void oracle_instances_classify(const oracle_ctx* c, tc_config cfg);
 void oracle_clusters_cull(const oracle_ctx* c, tc_config cfg);

@@ -22,7 +22,7 @@ P = C.c_void_p
 class OracleCtx(C.Structure):
     _fields_ = [("view", P), ("tessTable", P), ("readback", P), ("instances", P), ("build", P),
                 ("displacementTextures", P), ("numRenderInstances", C.c_uint32), ("numDisplacementTextures", C.c_uint32),
-                ("genVerticesValue", C.c_uint64)]
+                ("genVerticesValue", C.c_uint64), ("viewLast", P), ("hiz", abi.Hiz)]
 
 
 _oracle = None
@@ -58,6 +58,10 @@ def oracle():
         L.oracle_config_vertex_barycentrics.argtypes = [C.c_uint32, C.c_uint32, P]
         L.oracle_sample_height.argtypes = [P, C.c_float, C.c_float]
         L.oracle_sample_height.restype = C.c_float
+        L.oracle_hiz_build.argtypes = [C.POINTER(abi.Hiz), P, C.c_uint32]
+        L.oracle_hiz_build.restype = None
+        L.oracle_intersect_hiz.argtypes = [C.POINTER(abi.Hiz), P, P, P]
+        L.oracle_intersect_hiz.restype = C.c_int
         for fn in (L.oracle_ripple_sin, L.oracle_ripple_cos):
             fn.argtypes = [C.c_float]
             fn.restype = C.c_float
@@ -281,9 +285,33 @@ class OracleFrame:
         self.sb = sb
         self.rb = abi.Readback()
         self.view = abi.FrameConstants()
+        self.view_last = abi.FrameConstants()
         self.ctx = OracleCtx(C.addressof(self.view), C.addressof(self.table), C.addressof(self.rb),
                              C.addressof(self.instances), C.addressof(self.sb), C.addressof(self.textures), n,
                              len(host_scene.textures), 0)
+        self.hiz_texels = None
+
+    def set_culling_history(self, fc_last: abi.FrameConstants = None, depth: np.ndarray = None):
+        """Culls against `fc_last` (culling.glsl's viewLast) and, with a
+        depth image (H, W float32), the far HiZ the oracle builds from it."""
+        if fc_last is None:
+            self.ctx.viewLast = None
+        else:
+            C.memmove(C.addressof(self.view_last), C.addressof(fc_last), C.sizeof(fc_last))
+            self.ctx.viewLast = C.addressof(self.view_last)
+        if depth is None:
+            self.ctx.hiz = abi.Hiz()
+            self.hiz_texels = None
+            return
+        h, w = depth.shape
+        hz = abi.Hiz()
+        nbytes = C.c_uint64()
+        abi.check(abi.lib().tc_hiz_setup(w, h, C.byref(hz), C.byref(nbytes), None, None), "tc_hiz_setup")
+        self.hiz_texels = np.zeros(nbytes.value // 4, np.float32)
+        hz.texels = self.hiz_texels.ctypes.data
+        self.hiz_depth = np.ascontiguousarray(depth, np.float32)
+        oracle().oracle_hiz_build(C.byref(hz), self.hiz_depth.ctypes.data, 4)
+        self.ctx.hiz = hz
 
     def run(self, fc: abi.FrameConstants, nthreads: int = 1, stages: str = "all"):
         O = oracle()

@@ -37,7 +37,8 @@ def test_record_sizes():
         assert C.sizeof(getattr(abi, name)) == size, name
     assert abi.TESS_TRIANGLE_INFO.itemsize == 24 and abi.TEMPLATE_INSTANTIATE_INFO.itemsize == 32
     assert abi.BLAS_BUILD_INFO.itemsize == 16 and abi.CLUSTER_DT.itemsize == 16 and abi.BBOX_DT.itemsize == 32
-    assert C.sizeof(abi.Bindings) == 80
+    assert C.sizeof(abi.Bindings) == 120
+    assert C.sizeof(abi.Hiz) == 32
     assert C.sizeof(abi.AdjointParams) == 80
     assert abi.VERTEX_TAPE_DT.itemsize == 48
 

@@ -49,6 +49,7 @@ FLAG_2X_TRANSIENT = 1 << 5
 TRANSIENT_FLAGS = FLAG_1X_TRANSIENT | FLAG_2X_TRANSIENT  # TESS_USE_TRANSIENTBUILDS
 FLAG_PERSISTENT_SPLIT = 1 << 6
 FLAG_VERTEX_TAPE = 1 << 7
+FLAG_HIZ = 1 << 8
 
 CLAS_GROUP_TRIANGLES = 4
 
@@ -221,11 +222,16 @@ class Config(C.Structure):
                 ("clusterVertexCount", u32), ("clusterTriangleCount", u32), ("maxSplitFactor", u32)]
 
 
+class Hiz(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("size", u32), ("mipLevels", u32), ("usedWidth", u32), ("usedHeight", u32),
+                ("width", u32), ("height", u32)]
+
+
 class Bindings(C.Structure):
     _fields_ = [("view", C.c_void_p), ("tessTable", C.c_void_p), ("readback", C.c_void_p), ("instances", C.c_void_p),
                 ("build", C.c_void_p), ("displacementTextures", C.c_void_p), ("numRenderInstances", u32),
                 ("numDisplacementTextures", u32), ("workspace", C.c_void_p), ("workspaceBytes", u64),
-                ("vertexTape", C.c_void_p)]
+                ("vertexTape", C.c_void_p), ("viewLast", C.c_void_p), ("hiz", Hiz)]
 
 
 class Tuning(C.Structure):
@@ -281,7 +287,7 @@ EXPORTS = [
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
     "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
     "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning", "tc_clas_build_transient",
-    "tc_clusterize_mesh",
+    "tc_clusterize_mesh", "tc_hiz_setup", "tc_hiz_clear", "tc_hiz_build",
 ]
 
 
@@ -304,6 +310,9 @@ def lib():
         "tc_clas_size": (u32, [u32, u32]),
         "tc_cluster_template_order": (i32, [u32, u32, P, P, P]),
         "tc_clusterize_mesh": (i32, [P, u32, P, u32, u32, u32, P, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
+        "tc_hiz_setup": (i32, [u32, u32, C.POINTER(Hiz), C.POINTER(u64), P, C.POINTER(f32)]),
+        "tc_hiz_clear": (i32, [C.POINTER(Hiz), P]),
+        "tc_hiz_build": (i32, [C.POINTER(Hiz), P, u32, P]),
         "tc_instances_classify": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_clusters_cull": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_build_setup": (i32, [C.POINTER(Bindings), Config, C.c_int, P]),

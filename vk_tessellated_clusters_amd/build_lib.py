@@ -17,7 +17,7 @@ OBJ = PKG / "_build"
 LIB = PKG / "libtc_hip.so"
 SOURCES = ["tc_pipeline.hip", "tc_sort.hip", "tc_abi.hip", "tc_renderer.hip", "tc_clas.hip", "tc_classify.hip",
            "tc_instantiate.hip", "tc_bvh.hip", "tc_trace.hip",
-           "tc_scene.hip", "tc_adjoint.hip", "tc_mesh.hip"]
+           "tc_scene.hip", "tc_adjoint.hip", "tc_mesh.hip", "tc_hiz.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -73,7 +73,7 @@ static bool config_ok(const tc_config& c) {
 
 static constexpr uint32_t kSupportedFlags =
     TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT |
-    TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION | TC_FLAG_PERSISTENT_SPLIT;
+    TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION | TC_FLAG_PERSISTENT_SPLIT | TC_FLAG_HIZ;
 
 static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
   if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
@@ -101,6 +101,14 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   a._pad = 0;
   a.tape = (cfg.flags & TC_FLAG_VERTEX_TAPE) ? b->vertexTape : nullptr;
   a.cfg = cfg;
+  a.viewLast = b->viewLast ? b->viewLast : b->view;
+  a.hiz = b->hiz;
+  // a bound HiZ must be a whole pyramid
+  if ((cfg.flags & TC_FLAG_HIZ) && a.hiz.texels &&
+      (a.hiz.size == 0 || (a.hiz.size & (a.hiz.size - 1)) || a.hiz.mipLevels == 0 || a.hiz.mipLevels > 16 ||
+       (a.hiz.size >> (a.hiz.mipLevels - 1)) != 1))
+    return TC_ERROR_INVALID_ARGUMENT;
+  if (!(cfg.flags & TC_FLAG_HIZ)) a.hiz.texels = nullptr;
   if (layout) *layout = L;
   return TC_OK;
 }

@@ -287,6 +287,60 @@ __device__ __forceinline__ v3 ripple_deform(v3 p, uint32_t seed, float size, con
   return add(p, mk(n.x * (amp * wave.x), n.y * (amp * wave.y), n.z * (amp * wave.z)));
 }
 
+// ------------------------------------------------------------------ HiZ
+
+// ceil(log2(x)) exactly, from the binary exponent (x = m 2^e, m in
+// [0.5, 1)); 0 for x <= 0 (a zero-sized footprint samples level 0)
+__device__ __forceinline__ int ceil_log2(float x) {
+  if (!(x > 0.0f)) return 0;
+  if (isinf(x)) return 1 << 20;
+  int e;
+  const float m = frexpf(x, &e);
+  return m == 0.5f ? e - 1 : e;
+}
+
+__device__ __forceinline__ uint64_t hiz_level_offset(uint32_t size, uint32_t level) {
+  uint64_t o = 0;
+  for (uint32_t k = 0; k < level; k++) o += (uint64_t)(size >> k) * (size >> k);
+  return o;
+}
+
+// textureLod with the far sampler (nvhiz_vk.cpp:84-114: linear filter with
+// MAX reduction, nearest mip, clamp to edge): the maximum of the texels of
+// the 2 x 2 bilinear footprint that carry a non-zero weight
+__device__ __forceinline__ float hiz_sample(const tc_hiz& h, float u, float v, int level) {
+  const uint32_t lv = (uint32_t)min(max(level, 0), (int)h.mipLevels - 1);
+  const int dim = (int)(h.size >> lv);
+  const float* t = h.texels + hiz_level_offset(h.size, lv);
+  const float x = u * (float)dim - 0.5f, y = v * (float)dim - 0.5f;
+  const float x0 = floorf(x), y0 = floorf(y);
+  const bool wx = x - x0 > 0.0f, wy = y - y0 > 0.0f;
+  const int ix0 = min(max((int)x0, 0), dim - 1), ix1 = min(max((int)x0 + 1, 0), dim - 1);
+  const int iy0 = min(max((int)y0, 0), dim - 1), iy1 = min(max((int)y0 + 1, 0), dim - 1);
+  float m = t[(uint64_t)iy0 * dim + ix0];
+  if (wx) m = fmaxf(m, t[(uint64_t)iy0 * dim + ix1]);
+  if (wy) {
+    m = fmaxf(m, t[(uint64_t)iy1 * dim + ix0]);
+    if (wx) m = fmaxf(m, t[(uint64_t)iy1 * dim + ix1]);
+  }
+  return m;
+}
+
+// culling.glsl:96-114 intersectHiz (clipMin / clipMax with xy already
+// clamped to [-1, 1], as intersectFrustum returns them)
+__device__ __forceinline__ bool intersect_hiz(const tc_hiz& h, const tc_frame_constants* vl, const float cmin[4],
+                                              const float cmax[4]) {
+  float x0 = cmin[0] * 0.5f + 0.5f, y0 = cmin[1] * 0.5f + 0.5f;
+  float x1 = cmax[0] * 0.5f + 0.5f, y1 = cmax[1] * 0.5f + 0.5f;
+  x0 *= vl->hizSizeFactors[0], y0 *= vl->hizSizeFactors[1];
+  x1 *= vl->hizSizeFactors[0], y1 *= vl->hizSizeFactors[1];
+  x0 = fminf(x0, vl->hizSizeFactors[2]), y0 = fminf(y0, vl->hizSizeFactors[3]);
+  x1 = fminf(x1, vl->hizSizeFactors[2]), y1 = fminf(y1, vl->hizSizeFactors[3]);
+  const float maxsize = fmaxf(x1 - x0, y1 - y0) * vl->hizSizeMax;
+  const float depth = hiz_sample(h, (x0 + x1) * 0.5f, (y0 + y1) * 0.5f, ceil_log2(maxsize));
+  return cmin[2] <= depth + 2.0f / (float)(1 << 24);
+}
+
 // displacement.glsl:30-104 (PN triangle), pow written as products
 struct PNBasis {
   v3 b030, b003, b300, b021, b012, b102, b201, b210, b120, b111;

@@ -39,6 +39,8 @@ struct KArgs {
   uint32_t _pad;
   tc_vertex_tape* tape;  // non-null with TC_FLAG_VERTEX_TAPE
   tc_config cfg;
+  const tc_frame_constants* viewLast;  // culling constants (view when unbound)
+  tc_hiz hiz;                          // texels null: no occlusion test
 };
 
 // Workspace layout (bytes from the workspace start).

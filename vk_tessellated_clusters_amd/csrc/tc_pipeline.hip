@@ -521,7 +521,7 @@ __global__ void k_instances_classify(KArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.numInstances) return;
   const tc_render_instance& inst = a.instances[i];
-  const tc_frame_constants* vw = a.view;
+  const tc_frame_constants* vw = a.viewLast;  // culling.glsl reads viewLast
   float wvp[16];
   for (int col = 0; col < 4; col++)
     for (int r = 0; r < 4; r++)
@@ -560,7 +560,9 @@ __global__ void k_instances_classify(KArgs a) {
   }
   const bool inFrustum = bits == 0;
   const bool sizeOk = (cmax[0] - cmin[0]) > 2.0f / vw->viewportf[0] || (cmax[1] - cmin[1]) > 2.0f / vw->viewportf[1];
-  const bool visible = inFrustum && (!clipValid || sizeOk);
+  // :120 the occlusion test against the previous frame's far HiZ
+  const bool hizOk = !a.hiz.texels || !(inFrustum && clipValid && sizeOk) || intersect_hiz(a.hiz, vw, cmin, cmax);
+  const bool visible = inFrustum && (!clipValid || (sizeOk && hizOk));
   ptr<uint32_t>(a.build->instanceStates)[i] = (inFrustum ? TC_INSTANCE_FRUSTUM_BIT : 0u) | (visible ? TC_INSTANCE_VISIBLE_BIT : 0u);
   ptr<tc_blas_build_info>(a.build->blasBuildInfos)[i].clusterReferencesCount = 0;
 }

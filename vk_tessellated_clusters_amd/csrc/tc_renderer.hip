@@ -1,6 +1,7 @@
 // tc_renderer.hip -- C++ frame driver: owns the per-frame buffers and
 // records the stage chain (renderer_raytrace_clusters_tess.cpp:166-310 for
 // the buffer layout, :410-780 for the frame).
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -17,6 +18,7 @@ struct Renderer {
   void* arena = nullptr;
   uint64_t arenaBytes = 0;
   tc_frame_constants* view = nullptr;
+  tc_frame_constants* viewLast = nullptr;  // constants of the frame before (culling)
   tc_readback* readback = nullptr;
   tc_render_instance* instances = nullptr;
   tc_displacement_texture* textures = nullptr;
@@ -47,20 +49,33 @@ struct Renderer {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   tc_trace_params graphParams{};
+  // with TC_FLAG_HIZ: the far HiZ of the latest camera trace, sized for the
+  // frame constants' viewport, and its shader factors
+  tc_hiz hiz{};
+  float hizFactors[4] = {0, 0, 0, 0};
+  float hizSizeMax = 0.0f;
 };
 
 static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
 constexpr uint32_t kViewSlots = 8;
 
 // Copies the pinned slot the device frame counter names into the frame
-// constants and advances the counter (one workgroup).
-__global__ __launch_bounds__(256) void k_fetch_view(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view) {
-  const uint32_t slot = counter[0] % kViewSlots;
+// constants and advances the counter (one workgroup).  The constants they
+// replace become viewLast (the first frame's viewLast is its own view).
+__global__ __launch_bounds__(256) void k_fetch_view(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view,
+                                                    tc_frame_constants* viewLast) {
+  const uint32_t frame = counter[0];
+  const uint32_t slot = frame % kViewSlots;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(ring + slot);
   uint32_t* dst = reinterpret_cast<uint32_t*>(view);
-  for (uint32_t i = threadIdx.x; i < sizeof(tc_frame_constants) / 4; i += blockDim.x) dst[i] = src[i];
+  uint32_t* last = reinterpret_cast<uint32_t*>(viewLast);
+  for (uint32_t i = threadIdx.x; i < sizeof(tc_frame_constants) / 4; i += blockDim.x) {
+    const uint32_t v = src[i];
+    last[i] = frame ? dst[i] : v;
+    dst[i] = v;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) counter[0] = counter[0] + 1;
+  if (threadIdx.x == 0) counter[0] = frame + 1;
 }
 
 // inverse of a 4x4 row-major matrix in double precision (cofactors)
@@ -112,7 +127,39 @@ static bool fill_slot(Renderer* r, uint32_t slot, const tc_frame_constants* fc) 
     r->slotPending[slot] = false;
   }
   memcpy(&r->pinnedRing[slot], fc, sizeof(tc_frame_constants));
+  if (r->hiz.texels) {
+    // tessellatedclusters.cpp:615-619 the HiZ factors of the frame
+    memcpy(r->pinnedRing[slot].hizSizeFactors, r->hizFactors, sizeof(r->hizFactors));
+    r->pinnedRing[slot].hizSizeMax = r->hizSizeMax;
+  }
   if (fc != &r->lastFrame) r->lastFrame = *fc;
+  return true;
+}
+
+// With TC_FLAG_HIZ: (re)allocates the HiZ for the viewport of `fc`, filled
+// with the far depth (outside graph capture; a captured graph that used an
+// older HiZ is dropped).
+static bool ensure_hiz(Renderer* r, const tc_frame_constants* fc) {
+  if (!(r->cfg.flags & TC_FLAG_HIZ) || !fc) return true;
+  const uint32_t w = (uint32_t)fc->viewport[0], h = (uint32_t)fc->viewport[1];
+  if (r->hiz.texels && r->hiz.width == w && r->hiz.height == h) return true;
+  tc_hiz nh{};
+  uint64_t bytes = 0;
+  float factors[4], sizeMax;
+  if (tc_hiz_setup(w, h, &nh, &bytes, factors, &sizeMax) != TC_OK) return false;
+  if (hipDeviceSynchronize() != hipSuccess) return false;
+  if (r->hiz.texels) (void)hipFree(r->hiz.texels);
+  r->hiz.texels = nullptr;
+  if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec), r->graphExec = nullptr;
+  if (r->graph) (void)hipGraphDestroy(r->graph), r->graph = nullptr;
+  if (hipMalloc((void**)&nh.texels, bytes) != hipSuccess) return false;
+  if (tc_hiz_clear(&nh, nullptr) != TC_OK || hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(nh.texels);
+    return false;
+  }
+  r->hiz = nh;
+  memcpy(r->hizFactors, factors, sizeof(factors));
+  r->hizSizeMax = sizeMax;
   return true;
 }
 
@@ -179,9 +226,10 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter;
+  uint64_t oView, oViewLast, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
+      {&oViewLast, sizeof(tc_frame_constants)},
       {&oRb, sizeof(tc_readback)},
       {&oInst, sizeof(tc_render_instance) * (uint64_t)numInstances},
       {&oTex, sizeof(tc_displacement_texture) * (uint64_t)std::max(numTextures, 1u)},
@@ -224,6 +272,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   const uint64_t base = reinterpret_cast<uint64_t>(r->arena);
   char* mem = static_cast<char*>(r->arena);
   r->view = reinterpret_cast<tc_frame_constants*>(mem + oView);
+  r->viewLast = reinterpret_cast<tc_frame_constants*>(mem + oViewLast);
   r->readback = reinterpret_cast<tc_readback*>(mem + oRb);
   r->instances = reinterpret_cast<tc_render_instance*>(mem + oInst);
   r->textures = reinterpret_cast<tc_displacement_texture*>(mem + oTex);
@@ -293,6 +342,7 @@ tc_status tc_renderer_destroy(tc_renderer* rr) {
     (void)hipHostFree(r->pinnedRing);
   }
   if (r->arena) (void)hipFree(r->arena);
+  if (r->hiz.texels) (void)hipFree(r->hiz.texels);
   delete r;
   return TC_OK;
 }
@@ -311,6 +361,8 @@ tc_status tc_renderer_get_bindings(tc_renderer* rr, tc_bindings* out) {
   out->workspace = r->workspace;
   out->workspaceBytes = r->workspaceBytes;
   out->vertexTape = r->tape;
+  out->viewLast = r->viewLast;
+  out->hiz = r->hiz;
   return TC_OK;
 }
 
@@ -339,9 +391,18 @@ tc_status tc_renderer_build_accel(tc_renderer* rr, tc_stream stream) {
 
 tc_status tc_renderer_trace(tc_renderer* rr, const tc_trace_params* params, tc_stream stream) {
   if (!rr) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
   tc_bindings b;
   tc_renderer_get_bindings(rr, &b);
-  return tc_trace_rays(&b, R(rr)->cfg, &R(rr)->accel, params, stream);
+  tc_status s = tc_trace_rays(&b, r->cfg, &r->accel, params, stream);
+  if (s != TC_OK) return s;
+  // resources.cpp:503-512: the HiZ of this frame's depth, for the next
+  // frame's instance culling (camera rays of the HiZ's viewport only)
+  if (r->hiz.texels && params && !params->rays && params->decoded && params->width == r->hiz.width &&
+      params->height == r->hiz.height)
+    s = tc_hiz_build(&r->hiz, reinterpret_cast<const char*>(params->decoded) + offsetof(tc_hit_decoded, depth),
+                     sizeof(tc_hit_decoded), stream);
+  return s;
 }
 
 // One whole frame: tessellation, cluster acceleration structures, rays.
@@ -382,6 +443,7 @@ tc_status tc_renderer_graph_capture(tc_renderer* rr, const tc_frame_constants* f
   if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec), r->graphExec = nullptr;
   if (r->graph) (void)hipGraphDestroy(r->graph), r->graph = nullptr;
   if (params) r->graphParams = *params;
+  if (!ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return TC_ERROR_LAUNCH;
   r->capturing = true;
   r->lastFrame = *frameHost;
@@ -426,8 +488,10 @@ tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* fra
   // during graph capture only the fetch is recorded; each replay fills its
   // slot in tc_renderer_graph_launch
   const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
+  if (!r->capturing && !ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
   if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
-  hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter, r->view);
+  hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter, r->view,
+                     r->viewLast);
   if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
   hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
                      (const tc_frame_constants*)r->view, r->readback, reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles),
