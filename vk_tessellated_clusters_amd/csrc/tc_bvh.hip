@@ -44,6 +44,24 @@ __host__ __device__ __forceinline__ uint32_t morton_of(const float* blo, const f
   return code;
 }
 
+// 24-bit Morton code (8 bits per axis) of the box centre, normalised to
+// [lo, hi]: the BLAS key below the instance, three radix digits
+__device__ __forceinline__ uint32_t morton24_of(const float* blo, const float* bhi, const float* lo, const float* hi) {
+  uint32_t code = 0;
+  for (int k = 0; k < 3; k++) {
+    const float c = (blo[k] + bhi[k]) * 0.5f;
+    const float ext = hi[k] - lo[k];
+    float n = ext > 0.0f ? (c - lo[k]) / ext : 0.5f;
+    n = fminf(fmaxf(n, 0.0f), 1.0f);
+    uint32_t q = (uint32_t)(n * 255.0f + 0.5f);
+    q = (q | (q << 8)) & 0x0300F00Fu;
+    q = (q | (q << 4)) & 0x030C30C3u;
+    q = (q | (q << 2)) & 0x09249249u;
+    code |= q << k;
+  }
+  return code;
+}
+
 // instance owning global reference position k (lists are contiguous in
 // instance order after blas_setup_insertion)
 __device__ __forceinline__ uint32_t owner_instance(const tc_blas_build_info* infos, uint32_t numInstances, uint64_t base,
@@ -70,7 +88,7 @@ __global__ void k_blas_keys(BvhArgs a) {
     const uint32_t inst = owner_instance(infos, a.numInstances, b->blasClusterAddresses, k);
     const tc_clas_header h = *ptr<const tc_clas_header>(refs[k]);
     const tc_render_instance& ri = a.instances[inst];
-    const uint32_t m = morton_of(h.lo, h.hi, ri.geoLo, ri.geoHi);
+    const uint32_t m = morton24_of(h.lo, h.hi, ri.geoLo, ri.geoHi);
     a.keys[k] = ((uint64_t)inst << 32) | m;
     a.vals[k] = k;
   }
@@ -426,6 +444,25 @@ __global__ void k_tlas_collapse4(BvhArgs a) {
   if (i + 1 < a.numInstances) collapse4(a.tlasNodes, i, a.tlasNodes4);
 }
 
+// TLAS keys sorted in one workgroup (up to kTlasRankSort instances): the
+// keys are distinct (the instance index is their low word), so every key's
+// rank is the count of smaller keys; in place
+constexpr uint32_t kTlasRankSort = 1024;
+__global__ __launch_bounds__(1024) void k_tlas_rank_sort(BvhArgs a) {
+  __shared__ uint64_t s_key[kTlasRankSort];
+  __shared__ uint32_t s_val[kTlasRankSort];
+  const uint32_t n = min(a.numInstances, kTlasRankSort);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_key[i] = a.tlasKeys[i], s_val[i] = a.tlasVals[i];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t k = s_key[i];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; j++) rank += s_key[j] < k ? 1u : 0u;
+    a.tlasKeys[rank] = k;
+    a.tlasVals[rank] = s_val[i];
+  }
+}
+
 // TLAS root word (internal node 0, or the single instance) and the TLAS
 // element count read by the sort
 __global__ void k_tlas_root(BvhArgs a) {
@@ -559,7 +596,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   uint32_t instBits = 1;
   while ((1u << instBits) < a.numInstances) instBits++;
   uint32_t res = 0;
-  radix_sort_pairs(sb, &b->build->blasClusterCounter, 0, 32 + instBits, st, &res);
+  // digits 0-2: the 24-bit Morton code; digits 4-5: the instance
+  radix_sort_digits(sb, &b->build->blasClusterCounter, 0x7u | (instBits > 8 ? 0x30u : 0x10u), st, &res);
   a.sortedKeys = sb.keys[res];
   a.sortedVals = sb.vals[res];
   hipLaunchKernelGGL(k_blas_hierarchy, dim3(grid), dim3(256), 0, st, a);
@@ -583,10 +621,16 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   tb.capacity = a.numInstances;
   tb.maxTiles = (a.numInstances + kSortTile - 1) / kSortTile;
   hipLaunchKernelGGL(k_tlas_root, dim3(1), dim3(64), 0, st, a);
-  uint32_t tres = 0;
-  radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);
-  a.tlasSortedKeys = tb.keys[tres];
-  a.tlasSortedVals = tb.vals[tres];
+  if (a.numInstances <= kTlasRankSort) {
+    hipLaunchKernelGGL(k_tlas_rank_sort, dim3(1), dim3(kTlasRankSort), 0, st, a);
+    a.tlasSortedKeys = a.tlasKeys;
+    a.tlasSortedVals = a.tlasVals;
+  } else {
+    uint32_t tres = 0;
+    radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);
+    a.tlasSortedKeys = tb.keys[tres];
+    a.tlasSortedVals = tb.vals[tres];
+  }
   const uint32_t tgrid = (a.numInstances + 255) / 256;
   hipLaunchKernelGGL(k_tlas_hierarchy, dim3(tgrid), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_tlas_refit, dim3(tgrid), dim3(256), 0, st, a);

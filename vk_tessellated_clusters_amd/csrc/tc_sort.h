@@ -18,6 +18,11 @@ struct SortBuffers {
 // bits); *outIndex receives the ping-pong slot holding the result.
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
                       uint32_t* outIndex);
+// The same over the 8-bit digits named by digitMask (bit d: key bits
+// [8d, 8d + 8)), lowest digit first; digits where every key agrees may be
+// left out.
+void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t digitMask, hipStream_t stream,
+                       uint32_t* outIndex);
 
 __global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity, int transient);
 __global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity, int transient);

@@ -120,10 +120,25 @@ __global__ __launch_bounds__(64) void k_sort_scatter(const uint64_t* keysIn, con
 
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
                       uint32_t* outIndex) {
-  uint32_t passes = (bits + 7) / 8;
   uint32_t cur = 0;
-  for (uint32_t p = 0; p < passes; p++) {
-    uint32_t shift = bit0 + p * 8;
+  for (uint32_t shift = bit0; shift < bit0 + bits; shift += 8) {
+    hipLaunchKernelGGL(k_sort_hist, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], count, sb.capacity, sb.hist,
+                       sb.maxTiles, shift);
+    hipLaunchKernelGGL(k_sort_scan, dim3(kSortRadix), dim3(256), 0, stream, sb.hist, sb.digitTotals, count, sb.capacity,
+                       sb.maxTiles);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], sb.vals[cur], sb.keys[cur ^ 1],
+                       sb.vals[cur ^ 1], sb.hist, sb.digitTotals, count, sb.capacity, sb.maxTiles, shift);
+    cur ^= 1;
+  }
+  if (outIndex) *outIndex = cur;
+}
+
+void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t digitMask, hipStream_t stream,
+                       uint32_t* outIndex) {
+  uint32_t cur = 0;
+  for (uint32_t d = 0; d < 8; d++) {
+    if (!((digitMask >> d) & 1u)) continue;
+    const uint32_t shift = 8 * d;
     hipLaunchKernelGGL(k_sort_hist, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], count, sb.capacity, sb.hist,
                        sb.maxTiles, shift);
     hipLaunchKernelGGL(k_sort_scan, dim3(kSortRadix), dim3(256), 0, stream, sb.hist, sb.digitTotals, count, sb.capacity,
