@@ -203,8 +203,8 @@ def test_launch_shapes_do_not_change_results():
     cfg = harness.small_config()
     shapes = [None, dict(classifyBlocksPerCU=1, splitBlocksPerCU=1, instantiateBlocksPerCU=1, clasWavesPerCU=4,
                          traceBlocksPerCU=2, traceThreads=64, bvhBlocksPerCU=1),
-              dict(classifyBlocksPerCU=12, splitBlocksPerCU=16, instantiateBlocksPerCU=8, clasWavesPerCU=64,
-                   traceBlocksPerCU=32, traceThreads=256, bvhBlocksPerCU=16)]
+              dict(classifyBlocksPerCU=12, splitBlocksPerCU=16, instantiateBlocksPerCU=8, clasWavesPerCU=128,
+                   traceBlocksPerCU=128, traceThreads=128, bvhBlocksPerCU=16)]
     out = []
     try:
         for shape in shapes:

@@ -15,9 +15,9 @@ CANDIDATES = {
     "classifyBlocksPerCU": [2, 3, 5, 8, 12],
     "splitBlocksPerCU": [2, 4, 8, 16],
     "instantiateBlocksPerCU": [2, 3, 4, 6, 8],
-    "clasWavesPerCU": [8, 16, 32, 64],
+    "clasWavesPerCU": [16, 32, 64, 128],
     "traceThreads": [64, 128, 256],
-    "traceBlocksPerCU": [4, 8, 16, 32],
+    "traceBlocksPerCU": [8, 16, 32, 64, 128],
     "bvhBlocksPerCU": [4, 8, 16],
 }
 

@@ -130,7 +130,10 @@ static SortBuffers sort_buffers(const tc_bindings* b, const WsLayout& L) {
 tc_status make_args_public(const tc_bindings* b, const tc_config& cfg, KArgs& a) { return make_args(b, cfg, a); }
 int num_cu_public() { return num_cu(); }
 
-static constexpr tc_tuning kDefaultTuning = {5, 8, 3, 32, 16, 128, 8, 0};
+// defaults from the autotuner on the 64-instance grid frame: traversal with
+// about one ray per thread (256-thread workgroups, 32 per CU) hides its
+// memory latency best
+static constexpr tc_tuning kDefaultTuning = {5, 8, 3, 64, 32, 256, 8, 0};
 static std::mutex g_tuningMutex;
 static tc_tuning g_tuning = kDefaultTuning;
 
