@@ -17,7 +17,7 @@ CANDIDATES = {
     "instantiateBlocksPerCU": [2, 3, 4, 6, 8],
     "clasWavesPerCU": [16, 32, 64, 128],
     "traceThreads": [64, 128, 256],
-    "traceBlocksPerCU": [8, 16, 32, 64, 128],
+    "traceBlocksPerCU": [32, 64, 128, 256, 512],
     "bvhBlocksPerCU": [4, 8, 16],
 }
 

@@ -5,9 +5,9 @@
 // glsl:128-236, 338-339 (decode, depth) and render_shading.glsl:263-274
 // (shadow rays).  One ray per lane over the four-wide views of the TLAS and
 // the BLAS; the lane's stack (LDS, overflowing into private memory) holds
-// TLAS and BLAS entries (bit 30 marks BLAS entries; BLAS entries above a
-// TLAS entry belong to the instance entered last).  Triangles use the watertight test of Woop,
-// Benthin and Wald (JCGT 2013) with an FP64 recomputation of zero edge
+// TLAS and BLAS entries (the BLAS entries of the instance being walked lie
+// above the depth it was entered at).  Triangles use the watertight test of
+// Woop, Benthin and Wald (JCGT 2013) with an FP64 recomputation of zero edge
 // functions; closest-hit ties are broken by (instance, cluster, primitive)
 // so the result does not depend on traversal order.
 #include <algorithm>
@@ -16,7 +16,6 @@
 
 namespace tc {
 
-constexpr uint32_t kBlasEntry = 1u << 30;
 
 struct WoopRay {
   float o[3], d[3], inv[3];
@@ -24,8 +23,17 @@ struct WoopRay {
   float Sx, Sy, Sz;
 };
 
-__device__ __forceinline__ void setup_ray(WoopRay& r, const float o[3], const float d[3]) {
-  for (int k = 0; k < 3; k++) r.o[k] = o[k], r.d[k] = d[k], r.inv[k] = 1.0f / d[k];
+// (x, y, z)[k] for a lane-varying k in {0, 1, 2} as two selects of
+// registers (a dynamic index into an array would go through private memory)
+__device__ __forceinline__ float pick3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ void setup_ray(WoopRay& r, const float o[3], const float dir[3]) {
+  float d[3];
+  for (int k = 0; k < 3; k++) {
+    d[k] = dir[k];
+    asm volatile("" : "+v"(d[k]));  // register values: the picks below stay selects
+    r.o[k] = o[k], r.d[k] = d[k], r.inv[k] = 1.0f / d[k];
+  }
   const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
   r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
   r.kx = r.kz == 2 ? 0 : r.kz + 1;
@@ -35,9 +43,10 @@ __device__ __forceinline__ void setup_ray(WoopRay& r, const float o[3], const fl
     r.kx = r.ky;
     r.ky = t;
   }
-  r.Sx = d[r.kx] / d[r.kz];
-  r.Sy = d[r.ky] / d[r.kz];
-  r.Sz = 1.0f / d[r.kz];
+  const float dz = pick3(d[0], d[1], d[2], r.kz);
+  r.Sx = pick3(d[0], d[1], d[2], r.kx) / dz;
+  r.Sy = pick3(d[0], d[1], d[2], r.ky) / dz;
+  r.Sz = 1.0f / dz;
 }
 
 // conservative slab test; returns entry distance or +inf
@@ -60,12 +69,13 @@ struct TriHit {
 // Woop et al. 2013 with barycentrics (u: vertex 1, v: vertex 2)
 __device__ __forceinline__ bool tri_hit(const WoopRay& r, const float* p0, const float* p1, const float* p2, bool cullBack,
                                         float tmin, float tmax, TriHit& h) {
-  const float A[3] = {p0[0] - r.o[0], p0[1] - r.o[1], p0[2] - r.o[2]};
-  const float B[3] = {p1[0] - r.o[0], p1[1] - r.o[1], p1[2] - r.o[2]};
-  const float Cc[3] = {p2[0] - r.o[0], p2[1] - r.o[1], p2[2] - r.o[2]};
-  const float Ax = A[r.kx] - r.Sx * A[r.kz], Ay = A[r.ky] - r.Sy * A[r.kz];
-  const float Bx = B[r.kx] - r.Sx * B[r.kz], By = B[r.ky] - r.Sy * B[r.kz];
-  const float Cx = Cc[r.kx] - r.Sx * Cc[r.kz], Cy = Cc[r.ky] - r.Sy * Cc[r.kz];
+  const float A0 = p0[0] - r.o[0], A1 = p0[1] - r.o[1], A2 = p0[2] - r.o[2];
+  const float B0 = p1[0] - r.o[0], B1 = p1[1] - r.o[1], B2 = p1[2] - r.o[2];
+  const float C0 = p2[0] - r.o[0], C1 = p2[1] - r.o[1], C2 = p2[2] - r.o[2];
+  const float Akz = pick3(A0, A1, A2, r.kz), Bkz = pick3(B0, B1, B2, r.kz), Ckz = pick3(C0, C1, C2, r.kz);
+  const float Ax = pick3(A0, A1, A2, r.kx) - r.Sx * Akz, Ay = pick3(A0, A1, A2, r.ky) - r.Sy * Akz;
+  const float Bx = pick3(B0, B1, B2, r.kx) - r.Sx * Bkz, By = pick3(B0, B1, B2, r.ky) - r.Sy * Bkz;
+  const float Cx = pick3(C0, C1, C2, r.kx) - r.Sx * Ckz, Cy = pick3(C0, C1, C2, r.ky) - r.Sy * Ckz;
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -78,7 +88,7 @@ __device__ __forceinline__ bool tri_hit(const WoopRay& r, const float* p0, const
   const float det = (U + V) + W;
   if (det == 0.0f) return false;
   if (cullBack && det < 0.0f) return false;
-  const float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * Cc[r.kz];
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
   const float T = (U * Az + V * Bz) + W * Cz;
   const float t = T / det;
   if (!(t >= tmin && t <= tmax)) return false;
@@ -110,54 +120,103 @@ __device__ __forceinline__ const float* clas_vertices(const char* clas, uint32_t
   return reinterpret_cast<const float*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u));
 }
 
+// Slab test of a group box given as six floats; 1 if entered.
+__device__ __forceinline__ uint32_t group_hit(const WoopRay& r, float l0, float l1, float l2, float h0, float h1, float h2,
+                                              float tmin, float tmax) {
+  const float lo[3] = {l0, l1, l2}, hi[3] = {h0, h1, h2};
+  return box_hit(r, lo, hi, tmin, tmax) != INFINITY ? 1u : 0u;
+}
+
+constexpr uint32_t kGroupPairsPerStep = 4;
+
 // Intersects one CLAS block; updates `best` (closest) or returns true on
-// the first hit (any-hit).
+// the first hit (any-hit).  The loads of a step are issued together so a
+// lane waits for memory once per step rather than once per box or
+// triangle: group boxes eight at a time (four 48-byte pairs), then the four
+// triangles of each group entered (one 16-byte load of their words, then
+// their twelve vertices, issued with the next group's words).  Reads past
+// the last group or triangle stay inside the block (its sections are
+// rounded up to 16 bytes) or are clamped, and their results are masked off.
 __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uint32_t instanceID, bool cullBack, bool anyHit,
                                          float tmin, HitState& best) {
-  const tc_clas_header hdr = *reinterpret_cast<const tc_clas_header*>(clas);
-  const uint32_t nt = hdr.counts & 0xFF, ng = (hdr.counts >> 16) & 0xFF;
-  const tc_aabb* groups = reinterpret_cast<const tc_aabb*>(clas + sizeof(tc_clas_header));
+  const tc_clas_header* hp = reinterpret_cast<const tc_clas_header*>(clas);
+  const uint32_t counts = hp->counts, clusterID = hp->clusterID;
+  const uint32_t nt = counts & 0xFF, ng = (counts >> 16) & 0xFF;
+  const float4* gq = reinterpret_cast<const float4*>(clas + sizeof(tc_clas_header));
   const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
   const float* verts = clas_vertices(clas, nt, ng);
-  // group boxes first (ng <= 32), then the triangles of the groups hit, so
-  // the lanes of a wave run the triangle loop together
+  // group boxes (ng <= 32): pair p holds groups 2p and 2p + 1 in three float4
+  const uint32_t npairs = (ng + 1) >> 1;
   uint32_t mask = 0;
-  for (uint32_t g = 0; g < ng; g++) {
-    const tc_aabb gb = groups[g];
-    if (box_hit(r, gb.lo, gb.hi, tmin, best.h.t) != INFINITY) mask |= 1u << g;
+  for (uint32_t p0 = 0; p0 < npairs; p0 += kGroupPairsPerStep) {
+    float4 v[kGroupPairsPerStep][3];
+    uint32_t pp[kGroupPairsPerStep];
+#pragma unroll
+    for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
+      pp[j] = min(p0 + j, npairs - 1);
+      v[j][0] = gq[3 * pp[j]], v[j][1] = gq[3 * pp[j] + 1], v[j][2] = gq[3 * pp[j] + 2];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
+      const float4 a0 = v[j][0], a1 = v[j][1], a2 = v[j][2];
+      mask |= group_hit(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, tmin, best.h.t) << (2 * pp[j]);
+      mask |= group_hit(r, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w, tmin, best.h.t) << (2 * pp[j] + 1);
+    }
   }
-  while (mask) {
-    const uint32_t g = __builtin_ctz(mask);
-    mask &= mask - 1;
-    const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
-    for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
-      const uint32_t w = tris[t];
+  if (ng < 32) mask &= (1u << ng) - 1u;
+  // then the triangles of the groups entered, so the lanes of a wave run
+  // the triangle loop together
+  if (!mask) return false;
+  uint32_t g = __builtin_ctz(mask);
+  mask &= mask - 1;
+  uint4 w4 = *reinterpret_cast<const uint4*>(tris + g * TC_CLAS_GROUP_TRIANGLES);
+  while (true) {
+    const uint32_t t0 = g * TC_CLAS_GROUP_TRIANGLES;
+    uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    // the next group's words load while this group's vertices do
+    const uint32_t gn = mask ? __builtin_ctz(mask) : g;
+    w4 = *reinterpret_cast<const uint4*>(tris + gn * TC_CLAS_GROUP_TRIANGLES);
+    float v[4][9];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (t0 + j >= nt) w[j] = 0;  // vertex 0: an in-bounds read, result unused
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float* pk = verts + 3 * ((w[j] >> (8 * k)) & 0xFF);
+        v[j][3 * k] = pk[0], v[j][3 * k + 1] = pk[1], v[j][3 * k + 2] = pk[2];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (t0 + j >= nt) break;
       TriHit th;
-      if (!tri_hit(r, verts + 3 * (w & 0xFF), verts + 3 * ((w >> 8) & 0xFF), verts + 3 * ((w >> 16) & 0xFF), cullBack, tmin,
-                   best.h.t, th))
-        continue;
-      const uint32_t prim = w >> 24;
+      if (!tri_hit(r, v[j], v[j] + 3, v[j] + 6, cullBack, tmin, best.h.t, th)) continue;
+      const uint32_t prim = w[j] >> 24;
       if (anyHit) {
         best.h.t = th.t;
         return true;
       }
-      if (key_less(th.t, instanceID, hdr.clusterID, prim, best.h)) {
+      if (key_less(th.t, instanceID, clusterID, prim, best.h)) {
         best.h.t = th.t;
         best.h.u = th.u;
         best.h.v = th.v;
         best.h.instanceID = instanceID;
-        best.h.clusterID = hdr.clusterID;
+        best.h.clusterID = clusterID;
         best.h.primitiveID = prim;
         best.clas = clas;
-        best.word = w;
+        best.word = w[j];
       }
     }
+    if (!mask) return false;
+    g = gn;
+    mask &= mask - 1;
   }
-  return false;
 }
 
 // Traversal stack of one lane: the first kLdsStack entries in LDS (stride
-// T, the workgroup size), the rest in a private array.  Bound: every visited node
+// T, the workgroup size), the rest in a private array.  TLAS and BLAS
+// entries share it: the BLAS entries of the instance being walked sit above
+// the stack depth at which it was entered.  Bound: every visited node
 // pushes at most 3 of its 4 slots, and the four-wide view visits every
 // other level, so a ray holds at most 3 * (ceil(dT / 2) + ceil(dB / 2))
 // entries for binary depths dT (TLAS) and dB (BLAS).  Karras trees over
@@ -166,13 +225,19 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
 // 3 * (23 + 26) = 147 < kStackDepth entries.
 constexpr uint32_t kLdsStack = 24;
 constexpr uint32_t kStackDepth = 160;
+constexpr uint32_t kDone = 0xFFFFFFFFu;
 
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(5))) uint32_t private_u32;
+
+// Holds no array itself, so the stack pointer lives in a register; the
+// typed pointers make every access a ds_* or scratch_* instruction.
 template <uint32_t T>
 struct RayStack {
-  uint32_t* lds;
-  uint32_t ldsDepth;  // kLdsStack, or 0 with TC_TRACE_PRIVATE_STACK
+  lds_u32* lds;         // this lane's first entry, stride T
+  private_u32* spill;   // kStackDepth private entries
+  uint32_t ldsDepth;    // kLdsStack, or 0 with TC_TRACE_PRIVATE_STACK
   uint32_t sp;
-  uint32_t spill[kStackDepth];
   __device__ __forceinline__ void push(uint32_t e) {
     if (sp < ldsDepth)
       lds[sp * T] = e;
@@ -183,6 +248,25 @@ struct RayStack {
   __device__ __forceinline__ uint32_t pop() {
     --sp;
     return sp < ldsDepth ? lds[sp * T] : spill[sp - ldsDepth];
+  }
+  // next entry above stack depth `base`, or kDone
+  __device__ __forceinline__ uint32_t pop_above(uint32_t base) { return sp > base ? pop() : kDone; }
+  // pushes c[3], c[2], c[1] where t[k] < inf, with t ascending (the misses
+  // are a suffix).  With room in LDS every entry is written and only the
+  // pointer advance depends on the hit, so the wave runs no branch.
+  __device__ __forceinline__ void push3(const uint32_t c[4], const float t[4]) {
+    if (sp + 3 <= ldsDepth) {
+      lds[sp * T] = c[3];
+      sp += t[3] != INFINITY;
+      lds[sp * T] = c[2];
+      sp += t[2] != INFINITY;
+      lds[sp * T] = c[1];
+      sp += t[1] != INFINITY;
+    } else {
+      if (t[3] != INFINITY) push(c[3]);
+      if (t[2] != INFINITY) push(c[2]);
+      if (t[1] != INFINITY) push(c[1]);
+    }
   }
 };
 
@@ -208,8 +292,45 @@ __device__ __forceinline__ void cswap(float t[4], uint32_t c[4], int i, int j) {
   }
 }
 
+// Entry distances of the four slots of a node (+inf: missed or empty).
+// All four slab tests run unconditionally and the empty slots are selected
+// away afterwards, so the wave runs them as straight-line code.
+__device__ __forceinline__ void node_slots(const tc_bvh4_node* node, const WoopRay& r, float tmin, float tmax, float t[4],
+                                           uint32_t c[4]) {
+  float lo[3][4], hi[3][4];
+  load_node4(node, lo, hi, c);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
+    float bt = box_hit(r, bl, bh, tmin, tmax);
+    asm volatile("" : "+v"(bt));  // keep the test out of a branch on the slot
+    t[k] = c[k] == TC_BVH4_EMPTY ? INFINITY : bt;
+  }
+}
+
+// Visits one node: pushes the hit slots but the nearest (farthest first)
+// and returns the nearest one's child, or the next entry above `base`
+// when no slot is hit.
+template <uint32_t T>
+__device__ __forceinline__ uint32_t visit_node(const tc_bvh4_node* node, const WoopRay& r, float tmin, float tmax,
+                                               RayStack<T>& stack, uint32_t base) {
+  float t[4];
+  uint32_t c[4];
+  node_slots(node, r, tmin, tmax, t, c);
+  cswap(t, c, 0, 1);
+  cswap(t, c, 2, 3);
+  cswap(t, c, 0, 2);
+  cswap(t, c, 1, 3);
+  cswap(t, c, 1, 2);
+  if (t[0] == INFINITY) return stack.pop_above(base);
+  stack.push3(c, t);
+  return c[0];
+}
+
 // Full two-level traversal of one ray over the four-wide views.  Returns
-// true if anything was hit.
+// true if anything was hit.  The TLAS walk stops at each instance whose
+// BLAS has clusters; that BLAS is walked with the object-space ray until
+// the stack is back at the depth it was entered at.
 template <uint32_t T>
 __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3], float tmin, float tmax, bool cullBack,
                           bool anyHit, RayStack<T>& stack, HitState& best) {
@@ -218,75 +339,48 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
   best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
   best.clas = nullptr;
   best.word = 0;
-  WoopRay world, ray;
+  WoopRay world;
   setup_ray(world, o, d);
-  ray = world;
   stack.sp = 0;
   uint32_t e = *a.tlasRoot;
-  uint32_t inst = 0xFFFFFFFFu;
-  const tc_bvh4_node* blas4 = nullptr;
-  const uint64_t* blasLeaves = nullptr;
-  constexpr uint32_t kDone = 0xFFFFFFFFu;
-  // while-while: every lane walks nodes until it holds a CLAS leaf (or has
-  // finished), then the lanes with leaves intersect them together
-  while (e != kDone) {
-    while (true) {
-      if (!(e & kBlasEntry)) {
-        if (inst != 0xFFFFFFFFu) {
-          ray = world;
-          inst = 0xFFFFFFFFu;
+  while (true) {
+    // TLAS: find the next instance leaf with clusters
+    uint32_t inst = kDone;
+    while (e != kDone) {
+      if (e & TC_BVH_LEAF) {
+        const uint32_t i = e & 0x3FFFFFFFu;
+        if (a.tlasInstances[i].numClusters != 0) {
+          inst = i;
+          break;
         }
-        if (e & TC_BVH_LEAF) {
-          const uint32_t i = e & 0x3FFFFFFFu;
-          const tc_tlas_instance& ti = a.tlasInstances[i];
-          if (ti.numClusters != 0) {
-            float ol[3], dl[3];
-            xform34(ti.worldToObject, o, 1.0f, ol);
-            xform34(ti.worldToObject, d, 0.0f, dl);
-            setup_ray(ray, ol, dl);
-            inst = i;
-            blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
-            blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
-            e = kBlasEntry | ti.blasRoot;
-            continue;
-          }
-          e = stack.sp ? stack.pop() : kDone;
-          if (e == kDone) break;
-          continue;
-        }
-      } else if (e & TC_BVH_LEAF) {
-        break;  // CLAS leaf
+        e = stack.pop_above(0);
+      } else {
+        e = visit_node(a.tlasNodes4 + e, world, tmin, best.h.t, stack, 0);
       }
-      const tc_bvh4_node* node = (e & kBlasEntry) ? blas4 + (e & ~kBlasEntry) : a.tlasNodes4 + e;
-      const uint32_t tag = e & kBlasEntry;
-      float lo[3][4], hi[3][4], t[4];
-      uint32_t c[4];
-      load_node4(node, lo, hi, c);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
-        t[k] = c[k] == TC_BVH4_EMPTY ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
-        c[k] |= tag;
-      }
-      cswap(t, c, 0, 1);
-      cswap(t, c, 2, 3);
-      cswap(t, c, 0, 2);
-      cswap(t, c, 1, 3);
-      cswap(t, c, 1, 2);
-      if (t[0] == INFINITY) {
-        e = stack.sp ? stack.pop() : kDone;
-        if (e == kDone) break;
-        continue;
-      }
-      if (t[3] != INFINITY) stack.push(c[3]);
-      if (t[2] != INFINITY) stack.push(c[2]);
-      if (t[1] != INFINITY) stack.push(c[1]);
-      e = c[0];
     }
-    if (e == kDone) break;
-    const char* clas = reinterpret_cast<const char*>(blasLeaves[e & ~(kBlasEntry | TC_BVH_LEAF)]);
-    if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
-    e = stack.sp ? stack.pop() : kDone;
+    if (inst == kDone) break;
+    const tc_tlas_instance& ti = a.tlasInstances[inst];
+    WoopRay ray;
+    {
+      float ol[3], dl[3];
+      xform34(ti.worldToObject, o, 1.0f, ol);
+      xform34(ti.worldToObject, d, 0.0f, dl);
+      setup_ray(ray, ol, dl);
+    }
+    const tc_bvh4_node* blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
+    const uint64_t* blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+    const uint32_t base = stack.sp;
+    uint32_t b = ti.blasRoot;
+    // while-while: every lane walks nodes until it holds a CLAS leaf (or has
+    // finished), then the lanes with leaves intersect them together
+    while (b != kDone) {
+      while (b != kDone && !(b & TC_BVH_LEAF)) b = visit_node(blas4 + b, ray, tmin, best.h.t, stack, base);
+      if (b == kDone) break;
+      const char* clas = reinterpret_cast<const char*>(blasLeaves[b & ~TC_BVH_LEAF]);
+      if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
+      b = stack.pop_above(base);
+    }
+    e = stack.pop_above(0);
   }
   return best.h.instanceID != 0xFFFFFFFFu;
 }
@@ -299,6 +393,7 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
 // wave.  Suited to coherent rays: camera rays in 8 x 8 pixel tiles and their
 // shadow rays.
 constexpr uint32_t kPacketStack = 160;
+constexpr uint32_t kBlasEntry = 1u << 30;  // packet stack: marks BLAS entries
 
 __device__ __forceinline__ float wave_min_f(float v) {
 #pragma unroll
@@ -363,7 +458,9 @@ __device__ bool trace_packet(const TraceArgs& a, const float o[3], const float d
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const float bl[3] = {lo[0][k], lo[1][k], lo[2][k]}, bh[3] = {hi[0][k], hi[1][k], hi[2][k]};
-        t[k] = (c[k] == TC_BVH4_EMPTY || !live) ? INFINITY : box_hit(ray, bl, bh, tmin, best.h.t);
+        float bt = box_hit(ray, bl, bh, tmin, best.h.t);
+        asm volatile("" : "+v"(bt));
+        t[k] = (c[k] == TC_BVH4_EMPTY || !live) ? INFINITY : bt;
         tw[k] = wave_min_f(t[k]);  // nearest entry over the packet (uniform)
         c[k] |= tag;
       }
@@ -501,13 +598,17 @@ __device__ void decode_hit(const TraceArgs& a, const HitState& hs, tc_hit_decode
   out.depth = pp[2] / pp[3];
 }
 
-// Five waves per SIMD (at most 96 VGPRs): the traversal is latency-bound
-// and gains more from occupancy than it loses to the two spilled registers.
+// Four waves per SIMD (at most 128 VGPRs): the batched loads of the CLAS
+// test need the registers, and a frame's time is set by the longest waves
+// (rays with many leaves), which gain more from fewer round trips than from
+// a fifth wave.
 template <uint32_t T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_trace(TraceArgs a) {
   __shared__ uint32_t s_stack[kLdsStack * T];
+  uint32_t spill[kStackDepth];
   RayStack<T> stack;
-  stack.lds = s_stack + threadIdx.x;
+  stack.lds = (lds_u32*)(s_stack + threadIdx.x);
+  stack.spill = (private_u32*)spill;
   stack.ldsDepth = (a.flags & TC_TRACE_PRIVATE_STACK) ? 0u : kLdsStack;
   const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
   const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
