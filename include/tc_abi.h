@@ -733,6 +733,9 @@ typedef struct tc_accel {
   uint64_t memoryBytes;
   tc_bvh4_node* blasNodes4; /* four-wide view of blasNodes, same indices */
   tc_bvh4_node* tlasNodes4; /* four-wide view of tlasNodes */
+  uint64_t* blasLeafInfo;   /* per sorted reference: its CLAS header's counts
+                               | clusterID << 32 (read with blasLeaves, so the
+                               traversal need not wait for the header) */
 } tc_accel;
 
 /* Bytes of device memory for tc_accel_bind. */

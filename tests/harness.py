@@ -417,6 +417,7 @@ class GpuFrame:
         out["blasNodes4"] = self.r.copy(a.blasNodes4, 128 * total).view(abi.BVH4_NODE_DT)
         out["tlasNodes4"] = self.r.copy(a.tlasNodes4, 128 * n).view(abi.BVH4_NODE_DT)
         out["blasLeaves"] = self.r.copy(a.blasLeaves, 8 * total).view(np.uint64)
+        out["blasLeafInfo"] = self.r.copy(a.blasLeafInfo, 8 * total).view(np.uint64)
         out["refs"] = self.r.copy(sb.blasClusterAddresses, 8 * total).view(np.uint64)
         out["infos"] = self.r.copy(sb.blasBuildInfos, 16 * n).view(abi.BLAS_BUILD_INFO)
         return out

@@ -23,7 +23,7 @@ def clas_boxes(g, addrs):
     out = {}
     for a in np.unique(addrs):
         h = g.r.copy(int(a), 48).view(abi.CLAS_HEADER_DT)[0]
-        out[int(a)] = (h["lo"].copy(), h["hi"].copy())
+        out[int(a)] = (h["lo"].copy(), h["hi"].copy(), int(h["counts"]) | int(h["clusterID"]) << 32)
     return out
 
 
@@ -33,6 +33,8 @@ def check_blas(arr, boxes, inst):
     first = int(info["clusterReferences"] - arr["sb"].blasClusterAddresses) // 8
     leaves = arr["blasLeaves"][first:first + n]
     assert sorted(leaves.tolist()) == sorted(arr["refs"][first:first + n].tolist())
+    # the leaf info repeats each leaf's header counts and cluster ID
+    assert [int(x) for x in arr["blasLeafInfo"][first:first + n]] == [boxes[int(a)][2] for a in leaves]
     if n == 1:
         return
     nodes = arr["blasNodes"][first:first + n - 1]

@@ -179,7 +179,8 @@ class HitDecoded(C.Structure):
 class Accel(C.Structure):
     _fields_ = [("blasNodes", C.c_void_p), ("blasLeaves", C.c_void_p), ("tlasInstances", C.c_void_p),
                 ("tlasNodes", C.c_void_p), ("tlasRoot", C.c_void_p), ("numInstances", u32), ("capacity", u32),
-                ("memory", C.c_void_p), ("memoryBytes", u64), ("blasNodes4", C.c_void_p), ("tlasNodes4", C.c_void_p)]
+                ("memory", C.c_void_p), ("memoryBytes", u64), ("blasNodes4", C.c_void_p), ("tlasNodes4", C.c_void_p),
+                ("blasLeafInfo", C.c_void_p)]
 
 
 class TraceParams(C.Structure):

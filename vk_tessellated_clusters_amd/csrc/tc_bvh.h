@@ -16,6 +16,7 @@ struct BvhArgs {
   const uint64_t* sortedKeys;
   const uint32_t* sortedVals;
   uint64_t* leafAddr;  // CLAS address per sorted position (blasLeaves)
+  uint64_t* leafInfo;  // its header's counts | clusterID << 32 (blasLeafInfo)
   uint32_t* child;     // 2 per internal node, relative to the instance start
   uint32_t* parent;    // [0, capacity) internal nodes, [capacity, 2 capacity) leaves
   uint32_t* counter;   // refit arrival counters
@@ -43,7 +44,7 @@ struct BvhArgs {
 
 struct AccelLayout {
   uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter, rangeEnd, pendingCount, pendingList;
-  uint64_t blasNodes4, tlasNodes4;
+  uint64_t blasNodes4, tlasNodes4, blasLeafInfo;
   uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasRoot;
   uint64_t total;
 };

@@ -150,7 +150,7 @@ __device__ __forceinline__ void blas_segment(const BvhArgs& a, uint32_t inst, ui
 }
 
 // One thread per sorted position k: internal node k of its instance (when
-// k < last) and the sorted CLAS address / leaf box.
+// k < last), the sorted CLAS address and its header's counts and cluster ID.
 __global__ void k_blas_hierarchy(BvhArgs a) {
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->blasClusterCounter, a.capacity);
@@ -159,7 +159,10 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
     int64_t first, last;
     blas_segment(a, inst, n, first, last);
-    a.leafAddr[k] = refs[a.sortedVals[k]];
+    const uint64_t addr = refs[a.sortedVals[k]];
+    a.leafAddr[k] = addr;
+    const tc_clas_header* h = ptr<const tc_clas_header>(addr);
+    a.leafInfo[k] = (uint64_t)h->counts | (uint64_t)h->clusterID << 32;
     a.counter[k] = 0;
     if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity, a.rangeEnd);
   }
@@ -491,6 +494,7 @@ AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances) {
   put(L.blasNodes, c * sizeof(tc_bvh_node));
   put(L.blasNodes4, c * sizeof(tc_bvh4_node));
   put(L.blasLeaves, c * 8);
+  put(L.blasLeafInfo, c * 8);
   put(L.nodeBox, c * sizeof(tc_aabb));
   put(L.child, 2 * c * 4);
   put(L.parent, 2 * c * 4);
@@ -531,6 +535,7 @@ tc_status tc_accel_bind(void* memory, uint64_t bytes, uint32_t capacity, uint32_
   out->tlasRoot = reinterpret_cast<uint32_t*>(m + L.tlasRoot);
   out->blasNodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.blasNodes4);
   out->tlasNodes4 = reinterpret_cast<tc_bvh4_node*>(m + L.tlasNodes4);
+  out->blasLeafInfo = reinterpret_cast<uint64_t*>(m + L.blasLeafInfo);
   out->numInstances = numInstances;
   out->capacity = capacity;
   out->memory = memory;
@@ -571,6 +576,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   a.keys = sb.keys[0];
   a.vals = sb.vals[0];
   a.leafAddr = reinterpret_cast<uint64_t*>(m + L.blasLeaves);
+  a.leafInfo = reinterpret_cast<uint64_t*>(m + L.blasLeafInfo);
   a.child = reinterpret_cast<uint32_t*>(m + L.child);
   a.parent = reinterpret_cast<uint32_t*>(m + L.parent);
   a.counter = reinterpret_cast<uint32_t*>(m + L.counter);

@@ -14,10 +14,12 @@ struct TraceArgs {
   const tc_bvh4_node* tlasNodes4;
   const tc_bvh4_node* blasNodes4;
   uint64_t blasNodesBase;  // tc_tlas_instance.blasNodes - this = first node of the instance (x 64 B)
+  uint64_t blasLeavesBase;          // tc_tlas_instance.blasLeaves - this = first reference (x 8 B)
+  const uint64_t* blasLeafInfo;     // per reference: counts | clusterID << 32
   const uint32_t* tlasRoot;
   const tc_ray* rays;
   uint32_t width, height, numRays, flags;
-  uint32_t numItems;  // camera rays: whole 8 x 8 tiles
+  uint32_t numItems;  // camera rays: 64 per wave of the strip walk (some waves idle)
   tc_hit* hits;
   tc_hit_decoded* decoded;
   uint32_t* hitVertices;

@@ -120,6 +120,26 @@ __device__ __forceinline__ const float* clas_vertices(const char* clas, uint32_t
   return reinterpret_cast<const float*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u));
 }
 
+// Loads through the global address space: pointers rebuilt from 64-bit
+// addresses would otherwise become flat accesses, which also count against
+// the LDS counter the stack's ds_* accesses wait on.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+// (the HIP vector structs copy through a generic reference: load them as
+// native vectors)
+__device__ __forceinline__ float4 gload(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gload(const uint4* p) {
+  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Slab test of a group box given as six floats; 1 if entered.
 __device__ __forceinline__ uint32_t group_hit(const WoopRay& r, float l0, float l1, float l2, float h0, float h1, float h2,
                                               float tmin, float tmax) {
@@ -129,18 +149,18 @@ __device__ __forceinline__ uint32_t group_hit(const WoopRay& r, float l0, float 
 
 constexpr uint32_t kGroupPairsPerStep = 4;
 
-// Intersects one CLAS block; updates `best` (closest) or returns true on
-// the first hit (any-hit).  The loads of a step are issued together so a
+// Intersects one CLAS block, whose header's counts and cluster ID come in
+// `info` (tc_accel.blasLeafInfo); updates `best` (closest) or returns true
+// on the first hit (any-hit).  The loads of a step are issued together so a
 // lane waits for memory once per step rather than once per box or
 // triangle: group boxes eight at a time (four 48-byte pairs), then the four
 // triangles of each group entered (one 16-byte load of their words, then
 // their twelve vertices, issued with the next group's words).  Reads past
 // the last group or triangle stay inside the block (its sections are
 // rounded up to 16 bytes) or are clamped, and their results are masked off.
-__device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uint32_t instanceID, bool cullBack, bool anyHit,
-                                         float tmin, HitState& best) {
-  const tc_clas_header* hp = reinterpret_cast<const tc_clas_header*>(clas);
-  const uint32_t counts = hp->counts, clusterID = hp->clusterID;
+__device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uint64_t info, uint32_t instanceID, bool cullBack,
+                                         bool anyHit, float tmin, HitState& best) {
+  const uint32_t counts = (uint32_t)info, clusterID = (uint32_t)(info >> 32);
   const uint32_t nt = counts & 0xFF, ng = (counts >> 16) & 0xFF;
   const float4* gq = reinterpret_cast<const float4*>(clas + sizeof(tc_clas_header));
   const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
@@ -154,7 +174,7 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
 #pragma unroll
     for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
       pp[j] = min(p0 + j, npairs - 1);
-      v[j][0] = gq[3 * pp[j]], v[j][1] = gq[3 * pp[j] + 1], v[j][2] = gq[3 * pp[j] + 2];
+      v[j][0] = gload(gq + 3 * pp[j]), v[j][1] = gload(gq + 3 * pp[j] + 1), v[j][2] = gload(gq + 3 * pp[j] + 2);
     }
 #pragma unroll
     for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
@@ -169,13 +189,13 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
   if (!mask) return false;
   uint32_t g = __builtin_ctz(mask);
   mask &= mask - 1;
-  uint4 w4 = *reinterpret_cast<const uint4*>(tris + g * TC_CLAS_GROUP_TRIANGLES);
+  uint4 w4 = gload(reinterpret_cast<const uint4*>(tris + g * TC_CLAS_GROUP_TRIANGLES));
   while (true) {
     const uint32_t t0 = g * TC_CLAS_GROUP_TRIANGLES;
     uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
     // the next group's words load while this group's vertices do
     const uint32_t gn = mask ? __builtin_ctz(mask) : g;
-    w4 = *reinterpret_cast<const uint4*>(tris + gn * TC_CLAS_GROUP_TRIANGLES);
+    w4 = gload(reinterpret_cast<const uint4*>(tris + gn * TC_CLAS_GROUP_TRIANGLES));
     float v[4][9];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -183,7 +203,7 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const float* pk = verts + 3 * ((w[j] >> (8 * k)) & 0xFF);
-        v[j][3 * k] = pk[0], v[j][3 * k + 1] = pk[1], v[j][3 * k + 2] = pk[2];
+        v[j][3 * k] = gload(pk), v[j][3 * k + 1] = gload(pk + 1), v[j][3 * k + 2] = gload(pk + 2);
       }
     }
 #pragma unroll
@@ -369,6 +389,7 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
     }
     const tc_bvh4_node* blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
     const uint64_t* blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+    const uint64_t* leafInfo = a.blasLeafInfo + (ti.blasLeaves - a.blasLeavesBase) / 8;
     const uint32_t base = stack.sp;
     uint32_t b = ti.blasRoot;
     // while-while: every lane walks nodes until it holds a CLAS leaf (or has
@@ -376,8 +397,8 @@ __device__ bool trace_ray(const TraceArgs& a, const float o[3], const float d[3]
     while (b != kDone) {
       while (b != kDone && !(b & TC_BVH_LEAF)) b = visit_node(blas4 + b, ray, tmin, best.h.t, stack, base);
       if (b == kDone) break;
-      const char* clas = reinterpret_cast<const char*>(blasLeaves[b & ~TC_BVH_LEAF]);
-      if (clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) return true;
+      const char* clas = reinterpret_cast<const char*>(gload(blasLeaves + (b & ~TC_BVH_LEAF)));
+      if (clas_hit(ray, clas, gload(leafInfo + (b & ~TC_BVH_LEAF)), inst, cullBack, anyHit, tmin, best)) return true;
       b = stack.pop_above(base);
     }
     e = stack.pop_above(0);
@@ -416,6 +437,7 @@ __device__ bool trace_packet(const TraceArgs& a, const float o[3], const float d
   uint32_t inst = 0xFFFFFFFFu;
   const tc_bvh4_node* blas4 = nullptr;
   const uint64_t* blasLeaves = nullptr;
+  const uint64_t* leafInfo = nullptr;
   bool found = false;
   while (true) {
     if (!ballot(live)) break;
@@ -436,14 +458,16 @@ __device__ bool trace_packet(const TraceArgs& a, const float o[3], const float d
           inst = i;
           blas4 = a.blasNodes4 + (ti.blasNodes - a.blasNodesBase) / sizeof(tc_bvh_node);
           blasLeaves = reinterpret_cast<const uint64_t*>(ti.blasLeaves);
+          leafInfo = a.blasLeafInfo + (ti.blasLeaves - a.blasLeavesBase) / 8;
           e = __builtin_amdgcn_readfirstlane(kBlasEntry | ti.blasRoot);
           continue;
         }
         next = true;
       }
     } else if (e & TC_BVH_LEAF) {
-      const char* clas = reinterpret_cast<const char*>(blasLeaves[e & ~(kBlasEntry | TC_BVH_LEAF)]);
-      if (live && clas_hit(ray, clas, inst, cullBack, anyHit, tmin, best)) {
+      const uint32_t leaf = e & ~(kBlasEntry | TC_BVH_LEAF);
+      const char* clas = reinterpret_cast<const char*>(blasLeaves[leaf]);
+      if (live && clas_hit(ray, clas, leafInfo[leaf], inst, cullBack, anyHit, tmin, best)) {
         found = true;
         live = false;  // any-hit: this lane is done
       }
@@ -495,6 +519,30 @@ __device__ __forceinline__ void normalize4(float v[4]) {
   const float l = sqrtf(((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3]);
   const float s = 1.0f / l;
   for (int k = 0; k < 4; k++) v[k] = v[k] * s;
+}
+
+// Camera rays come in 8 x 8 pixel tiles, one per wave.  Consecutive
+// workgroups go to the eight XCDs in turn, so wave w of a launch runs on XCD
+// (w / wavesPerGroup) % 8.  The image is cut into eight vertical strips of
+// tiles and the waves of XCD x walk strip x in row order, so each XCD's L2
+// holds the geometry seen through one strip.  Strips differ in width by at
+// most one tile; a wave past the end of its strip has no tile (false).
+constexpr uint32_t kXcds = 8;
+
+__host__ __device__ __forceinline__ uint32_t strip_waves(uint32_t tilesX, uint32_t tilesY, uint32_t wavesPerGroup) {
+  const uint32_t perXcd = (tilesX + kXcds - 1) / kXcds * tilesY;
+  return kXcds * ((perXcd + wavesPerGroup - 1) / wavesPerGroup) * wavesPerGroup;
+}
+
+__device__ __forceinline__ bool strip_tile(uint32_t w, uint32_t wavesPerGroup, uint32_t tilesX, uint32_t tilesY, uint32_t& tx,
+                                           uint32_t& ty) {
+  const uint32_t b = w / wavesPerGroup, x = b % kXcds;
+  const uint32_t k = (b / kXcds) * wavesPerGroup + w % wavesPerGroup;
+  const uint32_t c0 = x * tilesX / kXcds, sw = (x + 1) * tilesX / kXcds - c0;
+  if (sw == 0 || k / sw >= tilesY) return false;
+  tx = c0 + k % sw;
+  ty = k / sw;
+  return true;
 }
 
 // render_raytrace.rgen.glsl:52-86
@@ -612,8 +660,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
   stack.ldsDepth = (a.flags & TC_TRACE_PRIVATE_STACK) ? 0u : kLdsStack;
   const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
   const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
-  // camera rays walk the image in 8 x 8 pixel tiles (one tile per wave)
-  const uint32_t tilesX = (a.width + 7) / 8;
+  const uint32_t tilesX = (a.width + 7) / 8, tilesY = (a.height + 7) / 8;
   for (uint32_t item = blockIdx.x * T + threadIdx.x; item < a.numItems; item += gridDim.x * T) {
     float o[3], d[3], tmin, tmax;
     uint32_t r = item;
@@ -623,8 +670,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
       tmin = ray.tMin;
       tmax = ray.tMax;
     } else {
-      const uint32_t tile = item >> 6, in = item & 63;
-      const uint32_t x = (tile % tilesX) * 8 + (in & 7), y = (tile / tilesX) * 8 + (in >> 3);
+      uint32_t tx, ty;
+      if (!strip_tile(item >> 6, T / kWave, tilesX, tilesY, tx, ty)) continue;
+      const uint32_t in = item & 63, x = tx * 8 + (in & 7), y = ty * 8 + (in >> 3);
       if (x >= a.width || y >= a.height) continue;
       r = y * a.width + x;
       primary_ray(a.view, x, y, a.width, a.height, o, d);
@@ -667,7 +715,7 @@ __global__ __launch_bounds__(T) void k_trace_packet(TraceArgs a) {
   uint32_t* stk = s_stk[wave];
   const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
   const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
-  const uint32_t tilesX = (a.width + 7) / 8;
+  const uint32_t tilesX = (a.width + 7) / 8, tilesY = (a.height + 7) / 8;
   const uint32_t wavesTotal = gridDim.x * (T / kWave);
   for (uint32_t base = (blockIdx.x * (T / kWave) + wave) * kWave; base < a.numItems; base += wavesTotal * kWave) {
     const uint32_t item = base + lane;
@@ -681,9 +729,10 @@ __global__ __launch_bounds__(T) void k_trace_packet(TraceArgs a) {
         tmin = ray.tMin;
         tmax = ray.tMax;
       } else {
-        const uint32_t tile = item >> 6, in = item & 63;
-        const uint32_t x = (tile % tilesX) * 8 + (in & 7), y = (tile / tilesX) * 8 + (in >> 3);
-        if (x < a.width && y < a.height) {
+        uint32_t tx = 0, ty = 0;
+        const bool tiled = strip_tile(item >> 6, T / kWave, tilesX, tilesY, tx, ty);
+        const uint32_t in = item & 63, x = tx * 8 + (in & 7), y = ty * 8 + (in >> 3);
+        if (tiled && x < a.width && y < a.height) {
           r = y * a.width + x;
           primary_ray(a.view, x, y, a.width, a.height, o, d);
           tmin = a.view->nearPlane;
@@ -732,7 +781,8 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   KArgs ka;
   tc_status s = make_args_public(b, config, ka);
   if (s != TC_OK) return s;
-  if (!accel || !p || !accel->tlasInstances || (!p->hits && !p->decoded && !p->hitVertices)) return TC_ERROR_INVALID_ARGUMENT;
+  if (!accel || !p || !accel->tlasInstances || !accel->blasLeafInfo || (!p->hits && !p->decoded && !p->hitVertices))
+    return TC_ERROR_INVALID_ARGUMENT;
   TraceArgs a;
   a.view = b->view;
   a.tessTable = b->tessTable;
@@ -744,10 +794,11 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   a.width = p->width;
   a.height = p->height;
   a.numRays = p->rays ? p->numRays : p->width * p->height;
-  a.numItems = p->rays ? p->numRays : ((p->width + 7) / 8) * ((p->height + 7) / 8) * 64;
   a.tlasNodes4 = accel->tlasNodes4;
   a.blasNodes4 = accel->blasNodes4;
   a.blasNodesBase = reinterpret_cast<uint64_t>(accel->blasNodes);
+  a.blasLeavesBase = reinterpret_cast<uint64_t>(accel->blasLeaves);
+  a.blasLeafInfo = accel->blasLeafInfo;
   a.hits = p->hits;
   a.decoded = p->decoded;
   a.hitVertices = p->hitVertices;
@@ -756,6 +807,7 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   if (a.numRays == 0) return TC_OK;
   const tc_tuning tn = tuning_snapshot();
   const uint32_t T = tn.traceThreads;
+  a.numItems = p->rays ? p->numRays : strip_waves((p->width + 7) / 8, (p->height + 7) / 8, T / kWave) * kWave;
   const uint32_t grid = std::min<uint32_t>((a.numItems + T - 1) / T, (uint32_t)num_cu_public() * tn.traceBlocksPerCU);
   if (a.flags & TC_TRACE_PACKETS) {
     if (T == 64)
