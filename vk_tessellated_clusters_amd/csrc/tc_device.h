@@ -67,12 +67,21 @@ __device__ __forceinline__ T* ptr(uint64_t a) {
   return reinterpret_cast<T*>(a);
 }
 
+// Load from device memory through the global address space.  A pointer
+// rebuilt from a 64-bit address is generic, and a generic (flat) load also
+// counts against the LDS counter, so LDS accesses after it wait for memory.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ v3 ld3(uint64_t base, uint32_t i) {
   const float* f = ptr<const float>(base) + 3ull * i;
-  return mk(f[0], f[1], f[2]);
+  return mk(gld(f), gld(f + 1), gld(f + 2));
 }
 __device__ __forceinline__ v2 ld2(uint64_t base, uint32_t i) {
-  const float2 f = ptr<const float2>(base)[i];
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 f = gld(ptr<const f2>(base) + i);
   return v2{f.x, f.y};
 }
 __device__ __forceinline__ void st3(uint64_t base, uint64_t i, v3 p) {
@@ -208,8 +217,8 @@ __device__ __forceinline__ v3 cfg_vertex_bary(const TableView& T, uint32_t cfg, 
 
 __device__ __forceinline__ float texel(const tc_displacement_texture& t, uint32_t x, uint32_t y) {
   uint64_t i = (uint64_t)y * t.width + x;
-  if (t.format == TC_TEXTURE_R16_UNORM) return float(ptr<const uint16_t>(t.texels)[i]) / 65535.0f;
-  return ptr<const float>(t.texels)[i];
+  if (t.format == TC_TEXTURE_R16_UNORM) return float(gld(ptr<const uint16_t>(t.texels) + i)) / 65535.0f;
+  return gld(ptr<const float>(t.texels) + i);
 }
 
 __device__ __forceinline__ int32_t wrapi(int32_t i, int32_t n) {
