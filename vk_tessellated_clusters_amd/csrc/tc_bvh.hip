@@ -325,11 +325,8 @@ __global__ void k_blas_collapse4(BvhArgs a) {
   }
 }
 
-// Per-instance TLAS records, world boxes and TLAS keys (one thread per
-// instance).
-__global__ void k_tlas_prep(BvhArgs a, const float* xforms /* 24 floats per instance */) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.numInstances) return;
+// TLAS record and world box of instance i.
+__device__ void tlas_prep_one(const BvhArgs& a, const float* xforms /* 24 floats per instance */, uint32_t i) {
   const tc_blas_build_info bi = ptr<const tc_blas_build_info>(a.build->blasBuildInfos)[i];
   const uint32_t first = (uint32_t)((bi.clusterReferences - a.build->blasClusterAddresses) / 8);
   const uint32_t cnt = bi.clusterReferencesCount;
@@ -372,6 +369,12 @@ __global__ void k_tlas_prep(BvhArgs a, const float* xforms /* 24 floats per inst
   a.tlasLeafBox[i] = wb;
 }
 
+// Per-instance TLAS records and world boxes (one thread per instance).
+__global__ void k_tlas_prep(BvhArgs a, const float* xforms) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.numInstances) tlas_prep_one(a, xforms, i);
+}
+
 // Scene bounds and TLAS keys over instances (one workgroup).
 __global__ __launch_bounds__(256) void k_tlas_keys(BvhArgs a) {
   __shared__ float s_lo[3][256], s_hi[3][256];
@@ -410,10 +413,11 @@ __global__ void k_tlas_hierarchy(BvhArgs a) {
   }
 }
 
-__global__ void k_tlas_refit(BvhArgs a) {
+// Bottom-up refit from sorted TLAS leaf k: the second arrival at a node
+// writes it.
+__device__ void tlas_refit_leaf(const BvhArgs& a, uint32_t k) {
   const uint32_t n = a.numInstances;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    if (n == 1) continue;
+  {
     uint32_t node = a.tlasParent[n + k];
     while (true) {
       __threadfence();
@@ -440,6 +444,12 @@ __global__ void k_tlas_refit(BvhArgs a) {
       node = a.tlasParent[node];
     }
   }
+}
+
+__global__ void k_tlas_refit(BvhArgs a) {
+  const uint32_t n = a.numInstances;
+  if (n == 1) return;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) tlas_refit_leaf(a, k);
 }
 
 __global__ void k_tlas_collapse4(BvhArgs a) {
@@ -473,6 +483,69 @@ __global__ void k_tlas_root(BvhArgs a) {
     a.tlasRoot[0] = a.numInstances == 1 ? (TC_BVH_LEAF | 0u) : 0u;
     a.tlasRoot[1] = a.numInstances;
   }
+}
+
+// The whole TLAS in one workgroup for up to kTlasRankSort instances, one
+// thread per instance: records and world boxes, scene bounds and keys, the
+// rank sort, hierarchy, refit and four-wide view, the steps the launches
+// above make, separated by workgroup barriers instead of kernel boundaries.
+__global__ __launch_bounds__(kTlasRankSort) void k_tlas_build_small(BvhArgs a, const float* xforms) {
+  __shared__ float s_b[6][kTlasRankSort / kWave];
+  __shared__ uint64_t s_key[kTlasRankSort], s_sorted[kTlasRankSort];
+  const uint32_t n = a.numInstances, i = threadIdx.x, lane = lane_id(), wave = threadIdx.x / kWave;
+  if (i < n) tlas_prep_one(a, xforms, i);
+  if (i == 0) {
+    a.tlasRoot[0] = n == 1 ? (TC_BVH_LEAF | 0u) : 0u;
+    a.tlasRoot[1] = n;
+  }
+  __syncthreads();
+  // scene bounds of the instance box centres (k_tlas_keys)
+  tc_aabb bx;
+  bool empty = true;
+  float c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (i < n) {
+    bx = a.tlasLeafBox[i];
+    empty = !(bx.lo[0] <= bx.hi[0]);
+    if (!empty)
+      for (int q = 0; q < 3; q++) c[q] = c[3 + q] = (bx.lo[q] + bx.hi[q]) * 0.5f;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    for (int d = 1; d < kWave; d <<= 1) {
+      c[q] = fminf(c[q], __shfl_xor(c[q], d, kWave));
+      c[3 + q] = fmaxf(c[3 + q], __shfl_xor(c[3 + q], d, kWave));
+    }
+  }
+  if (lane == 0)
+    for (int q = 0; q < 6; q++) s_b[q][wave] = c[q];
+  __syncthreads();
+  float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t w = 0; w < kTlasRankSort / kWave; w++)
+    for (int q = 0; q < 3; q++) slo[q] = fminf(slo[q], s_b[q][w]), shi[q] = fmaxf(shi[q], s_b[3 + q][w]);
+  // keys (empty instances sort last and are never entered), then ranks
+  if (i < n) s_key[i] = empty ? (0xFFFFFFFFull << 32) | i : ((uint64_t)morton_of(bx.lo, bx.hi, slo, shi) << 32) | i;
+  __syncthreads();
+  if (i < n) {
+    const uint64_t k = s_key[i];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; j++) rank += s_key[j] < k ? 1u : 0u;
+    a.tlasKeys[rank] = k;
+    a.tlasVals[rank] = i;
+    s_sorted[rank] = k;
+  }
+  __syncthreads();
+  // hierarchy over the sorted keys (k_tlas_hierarchy)
+  if (i < n) {
+    a.tlasCounter[i] = 0;
+    if (i + 1 < n) karras_node(s_sorted, 0, (int64_t)n - 1, i, a.tlasChild, a.tlasParent, n);
+  }
+  __syncthreads();
+  BvhArgs r = a;
+  r.tlasSortedKeys = a.tlasKeys;
+  r.tlasSortedVals = a.tlasVals;
+  if (i < n && n > 1) tlas_refit_leaf(r, i);
+  __syncthreads();
+  if (i + 1 < n) collapse4(a.tlasNodes, i, a.tlasNodes4);
 }
 
 }  // namespace tc
@@ -614,6 +687,10 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   // TLAS over instances; its sort reuses the other ping-pong slot
   a.tlasKeys = sb.keys[res ^ 1];
   a.tlasVals = sb.vals[res ^ 1];
+  if (a.numInstances <= kTlasRankSort) {
+    hipLaunchKernelGGL(k_tlas_build_small, dim3(1), dim3(kTlasRankSort), 0, st, a, instanceXforms);
+    return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+  }
   hipLaunchKernelGGL(k_tlas_prep, dim3((a.numInstances + 63) / 64), dim3(64), 0, st, a, instanceXforms);
   hipLaunchKernelGGL(k_tlas_keys, dim3(1), dim3(256), 0, st, a);
   // the TLAS sort runs on the second ping-pong slot of the workspace; its
