@@ -147,7 +147,29 @@ __device__ __forceinline__ uint32_t group_hit(const WoopRay& r, float l0, float 
   return box_hit(r, lo, hi, tmin, tmax) != INFINITY ? 1u : 0u;
 }
 
-constexpr uint32_t kGroupPairsPerStep = 4;
+// Group-box hit mask of the pairs [0, npairs) of a block, P pairs (2P
+// boxes) per step with the step's loads issued together; pair p holds
+// groups 2p and 2p + 1 in three float4.
+template <uint32_t P>
+__device__ __forceinline__ uint32_t group_mask(const WoopRay& r, const float4* gq, uint32_t npairs, float tmin, float tmax) {
+  uint32_t mask = 0;
+  for (uint32_t p0 = 0; p0 < npairs; p0 += P) {
+    float4 v[P][3];
+    uint32_t pp[P];
+#pragma unroll
+    for (uint32_t j = 0; j < P; j++) {
+      pp[j] = min(p0 + j, npairs - 1);
+      v[j][0] = gload(gq + 3 * pp[j]), v[j][1] = gload(gq + 3 * pp[j] + 1), v[j][2] = gload(gq + 3 * pp[j] + 2);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < P; j++) {
+      const float4 a0 = v[j][0], a1 = v[j][1], a2 = v[j][2];
+      mask |= group_hit(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, tmin, tmax) << (2 * pp[j]);
+      mask |= group_hit(r, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w, tmin, tmax) << (2 * pp[j] + 1);
+    }
+  }
+  return mask;
+}
 
 // Intersects one CLAS block, whose header's counts and cluster ID come in
 // `info` (tc_accel.blasLeafInfo); updates `best` (closest) or returns true
@@ -165,24 +187,12 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
   const float4* gq = reinterpret_cast<const float4*>(clas + sizeof(tc_clas_header));
   const uint32_t* tris = reinterpret_cast<const uint32_t*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u));
   const float* verts = clas_vertices(clas, nt, ng);
-  // group boxes (ng <= 32): pair p holds groups 2p and 2p + 1 in three float4
+  // group boxes (ng <= 32): eight per step, or two per step when every
+  // lane of the wave holds a block of at most two groups (most tessellated
+  // parts), which would otherwise test six clamped repeats
   const uint32_t npairs = (ng + 1) >> 1;
-  uint32_t mask = 0;
-  for (uint32_t p0 = 0; p0 < npairs; p0 += kGroupPairsPerStep) {
-    float4 v[kGroupPairsPerStep][3];
-    uint32_t pp[kGroupPairsPerStep];
-#pragma unroll
-    for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
-      pp[j] = min(p0 + j, npairs - 1);
-      v[j][0] = gload(gq + 3 * pp[j]), v[j][1] = gload(gq + 3 * pp[j] + 1), v[j][2] = gload(gq + 3 * pp[j] + 2);
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < kGroupPairsPerStep; j++) {
-      const float4 a0 = v[j][0], a1 = v[j][1], a2 = v[j][2];
-      mask |= group_hit(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, tmin, best.h.t) << (2 * pp[j]);
-      mask |= group_hit(r, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w, tmin, best.h.t) << (2 * pp[j] + 1);
-    }
-  }
+  uint32_t mask = __all(npairs <= 1) ? group_mask<1>(r, gq, npairs, tmin, best.h.t)
+                                     : group_mask<4>(r, gq, npairs, tmin, best.h.t);
   if (ng < 32) mask &= (1u << ng) - 1u;
   // then the triangles of the groups entered, so the lanes of a wave run
   // the triangle loop together
