@@ -26,7 +26,7 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
   uint32_t tilesPart = (cfg.maxPartTriangles + kWave - 1) / kWave;
   L.maxTiles = std::max(std::max(cfg.maxVisibleClusters, tilesSplit), std::max(tilesPart, 1u));
   L.sortCapacity = std::max(cfg.maxGeneratedClusters, 1u);
-  L.sortMaxTiles = (L.sortCapacity + kSortTile - 1) / kSortTile;
+  L.sortMaxTiles = (L.sortCapacity + kSortTileSmall - 1) / kSortTileSmall;
   L.instanceCapacity = kMaxInstances;
   uint64_t o = kWsHeaderBytes;
   L.status = o;

@@ -702,7 +702,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   tb.keys[1] = reinterpret_cast<uint64_t*>(m + L.tlasChild);
   tb.vals[1] = reinterpret_cast<uint32_t*>(m + L.tlasParent);
   tb.capacity = a.numInstances;
-  tb.maxTiles = (a.numInstances + kSortTile - 1) / kSortTile;
+  tb.maxTiles = (a.numInstances + kSortTileSmall - 1) / kSortTileSmall;
   hipLaunchKernelGGL(k_tlas_root, dim3(1), dim3(64), 0, st, a);
   if (a.numInstances <= kTlasRankSort) {
     hipLaunchKernelGGL(k_tlas_rank_sort, dim3(1), dim3(kTlasRankSort), 0, st, a);

@@ -58,7 +58,9 @@ struct WsLayout {
   uint32_t instanceCapacity;
 };
 
-constexpr uint32_t kSortTile = 1024;  // keys per radix-sort workgroup
+constexpr uint32_t kSortTile = 1024;      // keys per radix-sort workgroup (large sorts)
+constexpr uint32_t kSortTileSmall = 256;  // ... for sorts of at most kSortSmallMax keys
+constexpr uint32_t kSortSmallMax = 1u << 19;
 constexpr uint32_t kSortRadix = 256;
 
 WsLayout ws_layout(const tc_config& cfg, uint32_t numInstances);

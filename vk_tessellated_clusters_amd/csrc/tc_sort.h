@@ -11,7 +11,7 @@ struct SortBuffers {
   uint32_t* hist;         // kSortRadix x maxTiles
   uint32_t* digitTotals;  // kSortRadix
   uint32_t capacity;      // max element count
-  uint32_t maxTiles;      // ceil(capacity / kSortTile)
+  uint32_t maxTiles;      // ceil(capacity / kSortTileSmall)
 };
 
 // Sorts keys[0]/vals[0] by key bits [bit0, bit0 + bits) (whole passes of 8

@@ -2,10 +2,14 @@
 // deterministic BLAS insertion built on it.
 //
 // One pass per 8-bit digit: (1) per-tile digit histograms, (2) per-digit
-// exclusive scan over tiles, (3) stable scatter.  A tile is kSortTile keys
-// handled by one wave in 16 rounds of 64; inside a round, lanes holding the
-// same digit find each other with 8 ballots, so ranks follow key order and
-// the sort is stable.  The element count is read on the device.
+// exclusive scan over tiles, (3) stable scatter.  A tile is handled by one
+// wave in rounds of 64 keys; inside a round, lanes holding the same digit
+// find each other with 8 ballots, so ranks follow key order and the sort is
+// stable.  The element count is read on the device, and so is the tile
+// size: kSortTileSmall keys (4 rounds) for sorts of up to kSortSmallMax
+// keys, so that a frame's few hundred thousand keys spread over four times
+// as many waves, else kSortTile keys (16 rounds), which keeps the per-digit
+// scan over tiles short for large sorts.
 #include "tc_sort.h"
 
 namespace tc {
@@ -13,6 +17,7 @@ namespace tc {
 __device__ __forceinline__ uint32_t sort_count(const uint32_t* count, uint32_t capacity) {
   return min(*count, capacity);
 }
+__device__ __forceinline__ uint32_t sort_tile(uint32_t n) { return n <= kSortSmallMax ? kSortTileSmall : kSortTile; }
 
 // lanes of this wave whose digit equals mine
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
@@ -28,14 +33,14 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 __global__ __launch_bounds__(64) void k_sort_hist(const uint64_t* keys, const uint32_t* count, uint32_t capacity,
                                                   uint32_t* hist, uint32_t maxTiles, uint32_t shift) {
   __shared__ uint32_t s_cnt[kSortRadix];
-  const uint32_t n = sort_count(count, capacity);
+  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
   const uint32_t tile = blockIdx.x;
-  const uint32_t begin = tile * kSortTile;
+  const uint32_t begin = tile * ts;
   if (begin >= n) return;
   const uint32_t lane = lane_id();
   for (uint32_t d = lane; d < kSortRadix; d += kWave) s_cnt[d] = 0;
   __syncthreads();
-  for (uint32_t r = 0; r < kSortTile; r += kWave) {
+  for (uint32_t r = 0; r < ts; r += kWave) {
     const uint32_t i = begin + r + lane;
     if (i < n) atomicAdd(&s_cnt[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
   }
@@ -48,8 +53,8 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* dig
                                                    uint32_t capacity, uint32_t maxTiles) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_carry;
-  const uint32_t n = sort_count(count, capacity);
-  const uint32_t numTiles = (n + kSortTile - 1) / kSortTile;
+  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
+  const uint32_t numTiles = (n + ts - 1) / ts;
   const uint32_t d = blockIdx.x;
   uint32_t* h = hist + (uint64_t)d * maxTiles;
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
@@ -79,9 +84,9 @@ __global__ __launch_bounds__(64) void k_sort_scatter(const uint64_t* keysIn, con
                                                      const uint32_t* count, uint32_t capacity, uint32_t maxTiles,
                                                      uint32_t shift) {
   __shared__ uint32_t s_base[kSortRadix];
-  const uint32_t n = sort_count(count, capacity);
+  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
   const uint32_t tile = blockIdx.x;
-  const uint32_t begin = tile * kSortTile;
+  const uint32_t begin = tile * ts;
   if (begin >= n) return;
   const uint32_t lane = lane_id();
   // digit bases: exclusive scan of digit totals (4 digits per lane)
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(64) void k_sort_scatter(const uint64_t* keysIn, con
     }
   }
   __syncthreads();
-  for (uint32_t r = 0; r < kSortTile; r += kWave) {
+  for (uint32_t r = 0; r < ts; r += kWave) {
     const uint32_t i = begin + r + lane;
     const bool valid = i < n;
     uint64_t key = valid ? keysIn[i] : 0ull;
