@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
   __shared__ uint64_t s_wave[kInstWaves][3];
   __shared__ uint64_t s_excl[3];
   __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_okCount, s_okTris;  // the tile's accepted parts and triangles
 
   WsHeader* h = a.ws;
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
@@ -49,7 +50,10 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
   uint32_t* starts = s_start[wave];
 
   while (true) {
-    if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+      s_tile = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_okCount = s_okTris = 0;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= numTiles) break;
@@ -124,8 +128,8 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
     add_blas_reference_counts(ptr<tc_blas_build_info>(b->blasBuildInfos), ok ? ti.cluster.instanceID : 0u, 1u, ok);
     const uint32_t triSum = wave_sum(ok ? nt : 0u);
     if (lane == 0 && okMask) {
-      atomicAdd(&b->tempInstantiateCounter, (uint32_t)__popcll(okMask));
-      atomicAdd(&a.readback->numTotalTriangles, triSum);
+      atomicAdd(&s_okCount, (uint32_t)__popcll(okMask));
+      atomicAdd(&s_okTris, triSum);
     }
 
     // :295-337 stage the base triangle of every accepted part
@@ -187,6 +191,11 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
       st3(b->genVertices, (uint64_t)pb.vertexOffset + vert, pos);
     }
     __syncthreads();
+    // one device-scope atomic per counter and tile, not per wave
+    if (threadIdx.x == 0 && s_okCount) {
+      atomicAdd(&b->tempInstantiateCounter, s_okCount);
+      atomicAdd(&a.readback->numTotalTriangles, s_okTris);
+    }
   }
 }
 

@@ -201,6 +201,9 @@ __global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segS
 // the CLAS of its instance (CLAS order), in that instance's reference list.
 __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
                                  uint32_t capacity, int transient) {
+  __shared__ unsigned long long s_sizeSum;
+  if (threadIdx.x == 0) s_sizeSum = 0;
+  __syncthreads();
   tc_scene_building* b = a.build;
   const InsertList l = insert_list(b, transient, capacity);
   tc_blas_build_info* infos = ptr<tc_blas_build_info>(b->blasBuildInfos);
@@ -216,8 +219,12 @@ __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* 
       sizeSum += l.sizes[idx];
     }
   }
+  // one device-scope atomic per workgroup: every wave adding to the one
+  // counter serialised thousands of atomics at one L2 address
   sizeSum = wave_sum64(sizeSum);
-  if (lane_id() == 0 && sizeSum) atomicAdd((unsigned long long*)&a.readback->numGenActualDatas, (unsigned long long)sizeSum);
+  if (lane_id() == 0 && sizeSum) atomicAdd(&s_sizeSum, (unsigned long long)sizeSum);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sizeSum) atomicAdd((unsigned long long*)&a.readback->numGenActualDatas, s_sizeSum);
 }
 
 }  // namespace tc
