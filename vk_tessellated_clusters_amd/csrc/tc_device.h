@@ -532,6 +532,17 @@ __device__ __forceinline__ uint32_t take_ticket(WsHeader* h) {
   return uniform(t);
 }
 
+// The next ticket of a launch with numTiles tiles, or numTiles once they
+// are all taken.  Tickets only grow, so a read at or past numTiles means no
+// work is left, and the workgroups of an empty launch (a split pass with
+// nothing to split: thousands of them) leave without an atomic on the one
+// ticket word.  The read costs a round trip per tile, so launches that
+// always have work take tickets directly.  One thread.
+__device__ __forceinline__ uint32_t take_ticket_below(WsHeader* h, uint32_t numTiles) {
+  if (__hip_atomic_load(&h->ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= numTiles) return numTiles;
+  return __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Decoupled look-back over N counters for `tile`, executed by one full
 // wave.  agg[] is this tile's wave-uniform aggregate; returns the exclusive
 // prefix of every counter over tiles 0..tile-1 of this launch.  Tiles are

@@ -311,7 +311,9 @@ __global__ __launch_bounds__(64) void k_triangle_split(KArgs a) {
   const ViewParams vp = view_params(a);
   const TableView T = load_table(a.tessTable);
   while (true) {
-    const uint32_t tile = take_ticket(h);
+    uint32_t tile = 0;
+    if (lane_id() == 0) tile = take_ticket_below(h, numTiles);
+    tile = uniform(tile);
     if (tile >= numTiles) break;
     split_tile(a, T, vp, s_items, s_start, tile, numTiles, epoch, itemBegin, itemEnd, B0, B1, transY);
   }
