@@ -12,7 +12,7 @@ import ctypes as C
 from . import abi
 
 CANDIDATES = {
-    "classifyBlocksPerCU": [2, 3, 5, 8, 12],
+    "classifyBlocksPerCU": [2, 3, 4, 5, 8],
     "splitBlocksPerCU": [2, 4, 8, 16],
     "instantiateBlocksPerCU": [2, 3, 4, 6, 8],
     "clasWavesPerCU": [16, 32, 64, 128],

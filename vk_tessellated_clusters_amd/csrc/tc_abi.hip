@@ -133,7 +133,7 @@ int num_cu_public() { return num_cu(); }
 // defaults from the autotuner on the 64-instance grid frame: traversal with
 // about one ray per thread (256-thread workgroups, 32 per CU) hides its
 // memory latency best
-static constexpr tc_tuning kDefaultTuning = {5, 8, 3, 64, 512, 64, 8, 0};
+static constexpr tc_tuning kDefaultTuning = {5, 8, 2, 64, 512, 64, 8, 0};
 static std::mutex g_tuningMutex;
 static tc_tuning g_tuning = kDefaultTuning;
 
