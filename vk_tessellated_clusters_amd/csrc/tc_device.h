@@ -215,9 +215,21 @@ __device__ __forceinline__ v3 cfg_vertex_bary(const TableView& T, uint32_t cfg, 
 
 // --------------------------------------------------------- displacement
 
+// x / 65535 for a 16-bit x, correctly rounded: x times the rounded
+// reciprocal, then one residual correction with fused multiply-adds.  Equal
+// to the IEEE quotient for every one of the 65536 inputs (an exhaustive
+// check runs in the CPU tests); the division takes about ten instructions.
+__device__ __forceinline__ float unorm16(uint32_t x) {
+  constexpr float R = 1.0f / 65535.0f;
+  const float xf = (float)x;
+  const float q0 = xf * R;
+  const float r = __builtin_fmaf(-q0, 65535.0f, xf);
+  return __builtin_fmaf(r, R, q0);
+}
+
 __device__ __forceinline__ float texel(const tc_displacement_texture& t, uint32_t x, uint32_t y) {
   uint64_t i = (uint64_t)y * t.width + x;
-  if (t.format == TC_TEXTURE_R16_UNORM) return float(gld(ptr<const uint16_t>(t.texels) + i)) / 65535.0f;
+  if (t.format == TC_TEXTURE_R16_UNORM) return unorm16(gld(ptr<const uint16_t>(t.texels) + i));
   return gld(ptr<const float>(t.texels) + i);
 }
 
