@@ -96,13 +96,15 @@ __device__ __forceinline__ uint32_t vertex_base(const tc_scene_building* b, uint
   return vbuf >= b->genVertices ? (uint32_t)((vbuf - b->genVertices) / 12u) : 0xFFFFFFFFu;
 }
 
-// Small templates (at most kSubTris triangles and kSubVerts vertices: the
-// tessellated parts, a few triangles each) are built by 16-lane subgroups,
-// four per wave; larger ones (whole clusters) by the whole wave.  Both
-// paths write the same bytes (box min / max are exact in any order).
+// Templates of at most kSubTris triangles and kSubVerts vertices (the
+// tessellated parts and the whole clusters of up to 128 triangles) are
+// built by 16-lane subgroups, four per wave, each staging its block in a
+// quarter of the wave's LDS; larger ones by the whole wave.  Both paths
+// write the same bytes (box min / max are exact in any order).
 constexpr uint32_t kSub = 16;
-constexpr uint32_t kSubTris = 16;
-constexpr uint32_t kSubVerts = 16;
+constexpr uint32_t kSubTris = 128;
+constexpr uint32_t kSubVerts = 128;
+constexpr uint32_t kSubsPerWave = kWave / kSub;
 constexpr uint32_t kClasWaves = 4;
 
 __device__ __forceinline__ float sub_min(float v) {
@@ -119,8 +121,11 @@ __device__ __forceinline__ float sub_max(float v) {
 // Each wave takes 4 consecutive records per step (one per subgroup),
 // grid-stride over tempInstantiateCounter.
 __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a) {
-  __shared__ float s_vall[kClasWaves][kMaxClusterVerts * 3];
-  __shared__ uint32_t s_tall[kClasWaves][kMaxClusterTris];
+  // per wave: four subgroup quarters, or one whole-wave block (at most
+  // kMaxClusterVerts vertices and kMaxClusterTris triangles)
+  static_assert(kSubsPerWave * kSubVerts >= kMaxClusterVerts && kSubsPerWave * kSubTris >= kMaxClusterTris, "LDS per wave");
+  __shared__ float s_vall[kClasWaves][kSubsPerWave * kSubVerts * 3];
+  __shared__ uint32_t s_tall[kClasWaves][kSubsPerWave * kSubTris];
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->tempInstantiateCounter, a.cfg.maxGeneratedClusters);
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
@@ -168,10 +173,10 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (small && sl < ng) {
+    for (uint32_t g = sl; small && g < ng; g += kSub) {
       float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      const uint32_t t1 = min(nt, (sl + 1) * TC_CLAS_GROUP_TRIANGLES);
-      for (uint32_t t = sl * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
+      const uint32_t t1 = min(nt, (g + 1) * TC_CLAS_GROUP_TRIANGLES);
+      for (uint32_t t = g * TC_CLAS_GROUP_TRIANGLES; t < t1; t++) {
         const uint32_t w = s_st[t];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -182,8 +187,8 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
       }
       tc_aabb box;
 #pragma unroll
-      for (int c = 0; c < 3; c++) box.lo[c] = glo[c], box.hi[c] = ghi[c], lo[c] = glo[c], hi[c] = ghi[c];
-      reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header))[sl] = box;
+      for (int c = 0; c < 3; c++) box.lo[c] = glo[c], box.hi[c] = ghi[c], lo[c] = fminf(lo[c], glo[c]), hi[c] = fmaxf(hi[c], ghi[c]);
+      reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header))[g] = box;
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) lo[c] = sub_min(lo[c]), hi[c] = sub_max(hi[c]);
