@@ -150,7 +150,7 @@ __device__ __forceinline__ void blas_segment(const BvhArgs& a, uint32_t inst, ui
 }
 
 // One thread per sorted position k: internal node k of its instance (when
-// k < last), the sorted CLAS address and its header's counts and cluster ID.
+// k < last) and the sorted CLAS address.
 __global__ void k_blas_hierarchy(BvhArgs a) {
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->blasClusterCounter, a.capacity);
@@ -159,10 +159,7 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
     int64_t first, last;
     blas_segment(a, inst, n, first, last);
-    const uint64_t addr = refs[a.sortedVals[k]];
-    a.leafAddr[k] = addr;
-    const tc_clas_header* h = ptr<const tc_clas_header>(addr);
-    a.leafInfo[k] = (uint64_t)h->counts | (uint64_t)h->clusterID << 32;
+    a.leafAddr[k] = refs[a.sortedVals[k]];
     a.counter[k] = 0;
     if ((int64_t)k < last) karras_node(a.sortedKeys, first, last, k, a.child, a.parent, a.capacity, a.rangeEnd);
   }
@@ -170,9 +167,13 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     a.pendingCount[i] = 0;
 }
 
+// Box of sorted leaf k from its CLAS header.  Every leaf's box is read once,
+// by the refit of its parent, which also writes the leaf's info word
+// (counts | clusterID << 32) from the same header.
 __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[3], float hi[3]) {
   const tc_clas_header h = *ptr<const tc_clas_header>(a.leafAddr[k]);
   for (int c = 0; c < 3; c++) lo[c] = h.lo[c], hi[c] = h.hi[c];
+  a.leafInfo[k] = (uint64_t)h.counts | (uint64_t)h.clusterID << 32;
 }
 
 // Bottom-up refit in two launches, with no device-scope fence.
@@ -340,8 +341,10 @@ __device__ void tlas_prep_one(const BvhArgs& a, const float* xforms /* 24 floats
   t._pad = 0;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   if (cnt == 1) {
+    // a single CLAS is the root and has no parent to refit: its box and info here
     const tc_clas_header h = *ptr<const tc_clas_header>(a.leafAddr[first]);
     for (int q = 0; q < 3; q++) lo[q] = h.lo[q], hi[q] = h.hi[q];
+    a.leafInfo[first] = (uint64_t)h.counts | (uint64_t)h.clusterID << 32;
   } else if (cnt > 1) {
     const tc_aabb bx = a.nodeBox[first];
     for (int q = 0; q < 3; q++) lo[q] = bx.lo[q], hi[q] = bx.hi[q];
