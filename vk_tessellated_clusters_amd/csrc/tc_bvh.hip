@@ -417,15 +417,24 @@ __global__ void k_tlas_hierarchy(BvhArgs a) {
 }
 
 // Bottom-up refit from sorted TLAS leaf k: the second arrival at a node
-// writes it.
+// writes it.  With WORKGROUP every climber is in one workgroup, so
+// workgroup-scope ordering suffices (a device-scope release writes back the
+// XCD's L2 at every level).
+template <bool WORKGROUP>
 __device__ void tlas_refit_leaf(const BvhArgs& a, uint32_t k) {
   const uint32_t n = a.numInstances;
   {
     uint32_t node = a.tlasParent[n + k];
     while (true) {
-      __threadfence();
+      if (WORKGROUP)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      else
+        __threadfence();
       if (atomicAdd(&a.tlasCounter[node], 1u) == 0) break;
-      __threadfence();
+      if (WORKGROUP)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      else
+        __threadfence();
       tc_bvh_node out;
       float lo[2][3], hi[2][3];
       for (int c = 0; c < 2; c++) {
@@ -452,7 +461,7 @@ __device__ void tlas_refit_leaf(const BvhArgs& a, uint32_t k) {
 __global__ void k_tlas_refit(BvhArgs a) {
   const uint32_t n = a.numInstances;
   if (n == 1) return;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) tlas_refit_leaf(a, k);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) tlas_refit_leaf<false>(a, k);
 }
 
 __global__ void k_tlas_collapse4(BvhArgs a) {
@@ -546,7 +555,7 @@ __global__ __launch_bounds__(kTlasRankSort) void k_tlas_build_small(BvhArgs a, c
   BvhArgs r = a;
   r.tlasSortedKeys = a.tlasKeys;
   r.tlasSortedVals = a.tlasVals;
-  if (i < n && n > 1) tlas_refit_leaf(r, i);
+  if (i < n && n > 1) tlas_refit_leaf<true>(r, i);
   __syncthreads();
   if (i + 1 < n) collapse4(a.tlasNodes, i, a.tlasNodes4);
 }
