@@ -299,6 +299,18 @@ def lib():
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError("libtc_hip.so is missing at %s: run __graft_entry__.build() first" % LIB_PATH)
+    # PyTorch-ROCm ships its own copy of the HIP runtime (same soname as
+    # the system one this library links; the first one loaded serves the
+    # process).  When this library is loaded first and torch initialises
+    # the runtime afterwards, every later allocation made through the
+    # library returns hipErrorNoDevice; with torch's runtime loaded and
+    # started first, both share the device.  So start it first whenever
+    # torch is present.  (Without torch the system runtime works alone.)
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     P = C.c_void_p
     sig = {
