@@ -138,18 +138,28 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
   const tc_template_instantiate_info* infos = ptr<const tc_template_instantiate_info>(b->tempInstantiations);
   const uint64_t* dsts = ptr<const uint64_t>(b->tempClusterAddresses);
   uint32_t* sizes = ptr<uint32_t>(b->tempClusterSizes);
-  const uint32_t perStep = kWave / kSub;
-  for (uint32_t base = (blockIdx.x * kClasWaves + wave) * perStep; base < n; base += gridDim.x * kClasWaves * perStep) {
+  const uint32_t perStep = kWave / kSub, stride = gridDim.x * kClasWaves * perStep;
+  // two-stage prefetch along the grid-stride loop: while a record is built,
+  // the next record's template (its info is already here) and the info of
+  // the record after it load, so a step waits on its vertex loads alone
+  // instead of the info -> template -> vertices chain
+  uint32_t base = (blockIdx.x * kClasWaves + wave) * perStep;
+  tc_template_instantiate_info info1{}, info2{};
+  tc_clas_template T1{};
+  if (base + sub < n) {
+    info1 = infos[base + sub];
+    T1 = *ptr<const tc_clas_template>(info1.clusterTemplateAddress);
+  }
+  if (base + stride + sub < n) info2 = infos[base + stride + sub];
+  for (; base < n; base += stride) {
     const uint32_t i = base + sub;
     const bool valid = i < n;
-    tc_template_instantiate_info info;
-    tc_clas_template T;
-    bool small = false;
-    if (valid) {
-      info = infos[i];
-      T = *ptr<const tc_clas_template>(info.clusterTemplateAddress);
-      small = T.numTriangles <= kSubTris && T.numVertices <= kSubVerts;
-    }
+    const tc_template_instantiate_info info = info1;
+    const tc_clas_template T = T1;
+    info1 = info2;
+    if (i + stride < n) T1 = *ptr<const tc_clas_template>(info1.clusterTemplateAddress);
+    if (i + 2 * stride < n) info2 = infos[i + 2 * stride];
+    const bool small = valid && T.numTriangles <= kSubTris && T.numVertices <= kSubVerts;
     const uint32_t nt = small ? T.numTriangles : 0u, nv = small ? T.numVertices : 0u;
     const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
     char* dst = small ? ptr<char>(dsts[i]) : nullptr;
