@@ -639,7 +639,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   AccelLayout L = accel_layout(capacity, accel->numInstances);
   if (accel->memoryBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   WsLayout W = ws_layout(config, b->numRenderInstances);
-  if (W.sortCapacity < capacity) return TC_ERROR_INVALID_ARGUMENT;
+  // the instance sort of a large TLAS runs in the same key buffers
+  if (W.sortCapacity < capacity || W.sortCapacity < accel->numInstances) return TC_ERROR_INVALID_ARGUMENT;
   char* m = static_cast<char*>(accel->memory);
   char* w = static_cast<char*>(b->workspace);
   hipStream_t st = (hipStream_t)stream;
