@@ -233,3 +233,72 @@ def test_packet_traversal_matches_per_ray():
         assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
     finally:
         g.close()
+
+
+def lattice_scene(n=1100, width=96, height=54):
+    """n one-cluster patches (2 x 2 quads) on a square lattice with seeded
+    yaw, one displacement map, and a camera looking down across the field:
+    more instances than the one-workgroup TLAS build takes, so the TLAS is
+    built by separate launches with a radix sort over instances."""
+    geo = scene.grid_patch(2, 2, 1.4, 1.4)
+    side = int(np.ceil(np.sqrt(n)))
+    rng = np.random.default_rng(5)
+    insts = []
+    for k in range(n):
+        x, z = 1.0 * (k % side - side / 2), 1.0 * (k // side - side / 2)
+        insts.append(scene.Instance(0, scene.rigid_transform(float(rng.uniform(0, 6.28)), [x, 0.0, z]),
+                                    displacement_index=0, displacement_scale=0.1))
+    h = scene.to_r16(scene.fbm_height(64, 3, 3))
+    cam = scene.Camera(eye=(0.0, 14.0, -22.0), center=(0.0, 0.0, 0.0), width=width, height=height, near=0.05,
+                       far=500.0)
+    return scene.HostScene([geo], insts, [(h, abi.TEXTURE_R16_UNORM)], cam, 8.0, "lattice")
+
+
+def test_large_tlas_structure_and_trace():
+    host = lattice_scene()
+    n = len(host.instances)
+    cfg = harness.small_config()
+    W, H = host.camera.width, host.camera.height
+    of = harness.OracleFrame(host, cfg)
+    of.run(host.frame())
+    oh, od = of.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(host.frame(), stages="accel")
+        arr = g.accel_arrays()
+        gh, gd = g.trace(W, H, abi.TRACE_CULL_BACK | abi.TRACE_SHADOW)
+    finally:
+        g.close()
+    seen = []
+
+    def tvisit(word, depth=0):
+        assert depth < 64
+        if word & abi.BVH_LEAF:
+            seen.append(word & ~abi.BVH_LEAF)
+            return
+        nd = arr["tlasNodes"][word]
+        tvisit(int(nd["child0"]), depth + 1)
+        tvisit(int(nd["child1"]), depth + 1)
+
+    tvisit(arr["tlasRoot"])
+    assert sorted(seen) == list(range(n))
+    check_wide(arr["tlasNodes"][:n - 1], arr["tlasNodes4"][:n - 1])
+    assert (oh["t"] > 0).mean() > 0.3, "camera must see geometry"
+    assert len(np.unique(oh["instanceID"][oh["t"] > 0])) > 200
+    compare_hits(gh, oh, gd, od)
+
+
+def test_large_tlas_needs_sort_capacity():
+    """The instance sort runs in the cluster sort's key buffers, so a scene
+    with more instances than maxGeneratedClusters is refused before any
+    launch."""
+    host = lattice_scene()
+    cfg = harness.small_config()
+    cfg.maxGeneratedClusters = 1024
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(host.frame(), stages="tess")
+        st = abi.lib().tc_renderer_build_accel(g.r.handle, None)
+        assert st == abi.TC_ERROR_INVALID_ARGUMENT
+    finally:
+        g.close()
