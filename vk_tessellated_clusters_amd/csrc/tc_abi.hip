@@ -457,12 +457,21 @@ tc_status tc_clusters_cull(const tc_bindings* b, tc_config config, tc_stream str
   return launched();
 }
 
-tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream) {
+// The stage's prologue, fused with the build_setup of `setupMode` before it
+// when setupMode >= 0 (the frame sequence), else alone (the public entries).
+static void launch_prologue(const KArgs& a, int setupMode, uint32_t stage, hipStream_t st) {
+  if (setupMode >= 0)
+    hipLaunchKernelGGL(k_setup_prologue, dim3(1), dim3(256), 0, st, a, setupMode, stage);
+  else
+    hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, stage);
+}
+
+static tc_status launch_classify(const tc_bindings* b, tc_config config, int setupMode, tc_stream stream) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)0);
+  launch_prologue(a, setupMode, 0, st);
   uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().classifyBlocksPerCU);
   if (config.flags & TC_FLAG_2X_TRANSIENT) {
@@ -476,18 +485,23 @@ tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream 
   return launched();
 }
 
+tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream) {
+  return launch_classify(b, config, -1, stream);
+}
+
 // One split pass, or with TC_FLAG_PERSISTENT_SPLIT all `passes` passes in
 // one persistent launch (renderer_raytrace_clusters_tess.cpp:510-513).
-static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t passes, tc_stream stream) {
+static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t passes, tc_stream stream,
+                              int setupMode = -1) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
+  if ((config.flags & TC_FLAG_PERSISTENT_SPLIT) && passes > kMaxSplitPasses) return TC_ERROR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)1);
+  launch_prologue(a, setupMode, 1, st);
   uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
   uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU));
   if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
-    if (passes > kMaxSplitPasses) return TC_ERROR_UNSUPPORTED;
     hipLaunchKernelGGL(k_split_queue_init, dim3(1), dim3(256), 0, st, a, passes);
     hipLaunchKernelGGL(k_triangle_split_persistent, dim3(grid), dim3(64), 0, st, a);
   } else {
@@ -500,12 +514,12 @@ tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream st
   return launch_split(b, config, tc_num_split_passes(config.maxSplitFactor), stream);
 }
 
-tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
+static tc_status launch_instantiate(const tc_bindings* b, tc_config config, int setupMode, tc_stream stream) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, (uint32_t)2);
+  launch_prologue(a, setupMode, 2, st);
   uint32_t tiles = (config.maxPartTriangles + kInstTile - 1) / kInstTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().instantiateBlocksPerCU);
   if (config.flags & TC_FLAG_PN)
@@ -513,6 +527,10 @@ tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config 
   else
     hipLaunchKernelGGL(k_instantiate<false>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
   return launched();
+}
+
+tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
+  return launch_instantiate(b, config, -1, stream);
 }
 
 tc_status tc_blas_setup_insertion(const tc_bindings* b, tc_config config, tc_stream stream) {
@@ -551,21 +569,19 @@ tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t n
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
   if (numSplitPasses == 0) numSplitPasses = tc_num_split_passes(config.maxSplitFactor);
-  // renderer_raytrace_clusters_tess.cpp:467-582
-  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_CLASSIFY, stream)) != TC_OK) return s;
-  if ((s = tc_cluster_classify(b, config, stream)) != TC_OK) return s;
-  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT, stream)) != TC_OK) return s;
+  // renderer_raytrace_clusters_tess.cpp:467-582; each build_setup that
+  // precedes a stage runs in that stage's prologue launch
+  if ((s = launch_classify(b, config, TC_BUILD_SETUP_CLASSIFY, stream)) != TC_OK) return s;
   if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
     // every pass, with its SPLIT_PASS setup, in one launch
-    if ((s = launch_split(b, config, numSplitPasses, stream)) != TC_OK) return s;
+    if ((s = launch_split(b, config, numSplitPasses, stream, TC_BUILD_SETUP_SPLIT)) != TC_OK) return s;
   } else {
-    for (uint32_t p = 0; p < numSplitPasses; p++) {
-      if ((s = tc_triangle_split(b, config, stream)) != TC_OK) return s;
-      if (p + 1 < numSplitPasses && (s = tc_build_setup(b, config, TC_BUILD_SETUP_SPLIT_PASS, stream)) != TC_OK) return s;
-    }
+    for (uint32_t p = 0; p < numSplitPasses; p++)
+      if ((s = launch_split(b, config, numSplitPasses, stream, p == 0 ? TC_BUILD_SETUP_SPLIT : TC_BUILD_SETUP_SPLIT_PASS)) !=
+          TC_OK)
+        return s;
   }
-  if ((s = tc_build_setup(b, config, TC_BUILD_SETUP_INSTANTIATE_TESS, stream)) != TC_OK) return s;
-  if ((s = tc_triangle_tess_template_instantiate(b, config, stream)) != TC_OK) return s;
+  if ((s = launch_instantiate(b, config, TC_BUILD_SETUP_INSTANTIATE_TESS, stream)) != TC_OK) return s;
   return tc_build_setup(b, config, TC_BUILD_SETUP_BUILD_BLAS, stream);
 }
 

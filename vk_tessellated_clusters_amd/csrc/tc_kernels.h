@@ -71,6 +71,7 @@ tc_tuning tuning_snapshot();
 
 __global__ void k_prologue(KArgs a, uint32_t stage);
 __global__ void k_build_setup(KArgs a, int mode);
+__global__ void k_setup_prologue(KArgs a, int mode, uint32_t stage);
 template <bool TWO_X>
 __global__ void k_cluster_classify(KArgs a);
 __global__ void k_mini_vertices(KArgs a);

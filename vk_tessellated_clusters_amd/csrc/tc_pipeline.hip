@@ -16,7 +16,7 @@ namespace tc {
 // Opens a new look-back launch: bumps the epoch (clearing stale status
 // words when it wraps), resets the ticket and snapshots the counters the
 // stage advances.  One workgroup.
-__global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
+__device__ void prologue_body(const KArgs& a, uint32_t stage) {
   __shared__ uint32_t s_epoch;
   WsHeader* h = a.ws;
   tc_scene_building* b = a.build;
@@ -70,12 +70,13 @@ __global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_prologue(KArgs a, uint32_t stage) { prologue_body(a, stage); }
+
 // ---------------------------------------------------------- build_setup
 
 // build_setup.comp.glsl:102-264 (transient builds with TC_FLAG_1X_TRANSIENT
-// or TC_FLAG_2X_TRANSIENT)
-__global__ void k_build_setup(KArgs a, int mode) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// or TC_FLAG_2X_TRANSIENT).  One thread.
+__device__ void build_setup_one(const KArgs& a, int mode) {
   tc_scene_building* b = a.build;
   tc_readback* rb = a.readback;
   const tc_config& cfg = a.cfg;
@@ -147,6 +148,19 @@ __global__ void k_build_setup(KArgs a, int mode) {
     b->partTriangleCounter = part;
     b->dispatchTriangleInstantiate = tc_dispatch_indirect{(part + 31) / 32, 1, 1};
   }
+}
+
+__global__ void k_build_setup(KArgs a, int mode) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  build_setup_one(a, mode);
+}
+
+// A build_setup followed by the next stage's prologue in one launch: the
+// prologue's snapshot is taken by the same thread after the setup's writes.
+__global__ __launch_bounds__(256) void k_setup_prologue(KArgs a, int mode, uint32_t stage) {
+  if (threadIdx.x == 0) build_setup_one(a, mode);
+  __syncthreads();
+  prologue_body(a, stage);
 }
 
 // -------------------------------------------------------- triangle_split
