@@ -452,7 +452,7 @@ tc_status tc_clusters_cull(const tc_bindings* b, tc_config config, tc_stream str
   tc_status s = make_args(b, config, a, &L);
   if (s != TC_OK) return s;
   uint32_t* offsets = reinterpret_cast<uint32_t*>(static_cast<char*>(b->workspace) + L.cullOffsets);
-  hipLaunchKernelGGL(k_clusters_cull_scan, dim3(1), dim3(1), 0, (hipStream_t)stream, a, offsets);
+  hipLaunchKernelGGL(k_clusters_cull_scan, dim3(1), dim3(kWave), 0, (hipStream_t)stream, a, offsets);
   hipLaunchKernelGGL(k_clusters_cull_fill, dim3(a.numInstances), dim3(256), 0, (hipStream_t)stream, a, (const uint32_t*)offsets);
   return launched();
 }

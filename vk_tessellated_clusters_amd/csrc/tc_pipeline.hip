@@ -586,14 +586,20 @@ __global__ void k_instances_classify(KArgs a) {
 // clusters_cull.comp.glsl:112-161 in ray-tracing mode (every cluster of
 // every instance, instance by instance).  Workgroup i appends instance i at
 // the offset of the exclusive scan held in `offsets`.
-__global__ void k_clusters_cull_scan(KArgs a, uint32_t* offsets) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: exclusive scan of the instances' cluster counts, 64 instances
+// at a time, after the clusters already appended.
+__global__ __launch_bounds__(64) void k_clusters_cull_scan(KArgs a, uint32_t* offsets) {
+  if (blockIdx.x != 0) return;
+  const uint32_t lane = lane_id();
   uint32_t run = a.build->visibleClusterCounter;
-  for (uint32_t i = 0; i < a.numInstances; i++) {
-    offsets[i] = run;
-    run += a.instances[i].numClusters;
+  for (uint32_t i0 = 0; i0 < a.numInstances; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    const uint32_t n = i < a.numInstances ? a.instances[i].numClusters : 0u;
+    const uint32_t incl = wave_incl_scan(n);
+    if (i < a.numInstances) offsets[i] = run + incl - n;
+    run += shfl(incl, kWave - 1);
   }
-  a.build->visibleClusterCounter = run;
+  if (lane == 0) a.build->visibleClusterCounter = run;
 }
 
 __global__ void k_clusters_cull_fill(KArgs a, const uint32_t* offsets) {
