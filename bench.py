@@ -50,14 +50,8 @@ def profiled_traffic(kernel, workload):
 
 
 def bench_config(abi, transient_1x=False, large=False, animation=False, transient_2x=False):
-    if large:
-        c = abi.default_config(maxVisibleClusters=1 << 18, maxPartTriangles=1 << 23, maxSplitTriangles=1 << 20,
-                               maxGeneratedClusters=1 << 23, maxGeneratedClusterMegs=12288,
-                               maxGeneratedVertices=1 << 27)
-    else:
-        c = abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
-                               maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
-                               maxGeneratedVertices=1 << 25)
+    from vk_tessellated_clusters_amd import scene
+    c = scene.workload_config(large)
     if transient_1x:
         c.flags |= abi.FLAG_1X_TRANSIENT
     if transient_2x:

@@ -291,7 +291,17 @@ typedef struct tc_readback {
   uint32_t debugA[64];
   uint32_t debugB[64];
   uint32_t debugC[64];
+  /* Not in the reference's Readback: it takes the record's tail padding,
+   * so the size and every field above are unchanged.  TC_READBACK_ERROR_*
+   * bits of this frame, cleared with the rest of the readback at frame
+   * start. */
+  uint32_t errorFlags;
 } tc_readback;
+
+/* tc_readback.errorFlags: a persistent split (TC_FLAG_PERSISTENT_SPLIT)
+ * wait reached its bound, so that launch gave up and its split and part
+ * lists and counters are invalid for this frame. */
+#define TC_READBACK_ERROR_SPLIT_WAIT 1u
 
 /* Sun direction of the reference's SkySimpleParameters (an nvpro_core2 type
  * that is not part of the reference tree); the path reads only this member
@@ -593,7 +603,11 @@ typedef struct tc_tuning {
   uint32_t traceBlocksPerCU;       /* traversal workgroups per CU */
   uint32_t traceThreads;           /* traversal workgroup size */
   uint32_t bvhBlocksPerCU;         /* BVH keys / hierarchy / collapse */
-  uint32_t _reserved;
+  /* Sleeps before a wait of the persistent split gives up and flags the
+   * frame (TC_READBACK_ERROR_SPLIT_WAIT); default 2^24, and 0 in
+   * tc_set_tuning restores it.  Not a performance knob: only failure-path
+   * tests lower it. */
+  uint32_t splitWaitLimit;
 } tc_tuning;
 
 tc_status tc_get_tuning(tc_tuning* out);

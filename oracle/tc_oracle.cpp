@@ -223,7 +223,11 @@ float sample_height(const tc_displacement_texture& t, float u, float v) {
   return top * (1.0f - fy) + bot * fy;
 }
 
-/* displacement.glsl:30-104: PN triangle; pow(u,3) written as u*u*u */
+/* displacement.glsl:30-104: PN triangle (Vlachos et al., Curved PN
+ * Triangles, I3D 2001) with verts[0] at u = 1, verts[1] at v = 1 and
+ * verts[2] at w = 1; pow(u,3) is written as u*u*u, and each term's scalar
+ * weight is formed before it scales the control point (the GLSL scales the
+ * point by 3.0, then by each factor: the same value up to rounding) */
 struct PN {
   v3 b030, b003, b300, b021, b012, b102, b201, b210, b120, b111;
 };
@@ -1074,6 +1078,16 @@ int oracle_intersect_hiz(const tc_hiz* hiz, const tc_frame_constants* viewLast, 
 }
 float oracle_ripple_cos(float x) { return ripple_sincos(x, true); }
 
+void oracle_pn(const float verts[9], const float normals[9], const float bary[3], float out[3]) {
+  v3 vt[3], n[3];
+  for (int k = 0; k < 3; k++)
+    vt[k] = mk(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]), n[k] = mk(normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]);
+  PN b;
+  setup_pn(b, vt, n);
+  v3 r = get_pn(b, mk(bary[0], bary[1], bary[2]));
+  out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+
 /* instances_classify.comp.glsl:100-127 with culling.glsl:30-90 (frustum and
  * size test; no HiZ pyramid exists, so the occlusion test passes). */
 void oracle_instances_classify(const oracle_ctx* c, tc_config cfg) {
@@ -1439,9 +1453,29 @@ const float* clas_verts(const char* clas, uint32_t nt, uint32_t ng) {
   return reinterpret_cast<const float*>(clas + sizeof(tc_clas_header) + ((ng * 24u + 15u) & ~15u) + ((nt * 4u + 15u) & ~15u));
 }
 
+/* Object-space union of each instance's CLAS boxes.  A ray that enters a
+ * CLAS box enters this union too (the slab bounds move monotonically with
+ * the box bounds under rounding), so skipping instances whose union is
+ * missed changes no result; it only saves the per-CLAS tests. */
+std::vector<tc_aabb> instance_boxes(const oracle_ctx* c) {
+  std::vector<tc_aabb> out(c->numRenderInstances);
+  const tc_blas_build_info* infos = P<const tc_blas_build_info>(c->build->blasBuildInfos);
+  for (uint32_t i = 0; i < c->numRenderInstances; i++) {
+    tc_aabb& u = out[i];
+    for (int k = 0; k < 3; k++) u.lo[k] = INFINITY, u.hi[k] = -INFINITY;
+    const uint64_t* refs = P<const uint64_t>(infos[i].clusterReferences);
+    for (uint32_t r = 0; r < infos[i].clusterReferencesCount; r++) {
+      tc_clas_header hdr;
+      memcpy(&hdr, P<const char>(refs[r]), sizeof(hdr));
+      for (int k = 0; k < 3; k++) u.lo[k] = fminf(u.lo[k], hdr.lo[k]), u.hi[k] = fmaxf(u.hi[k], hdr.hi[k]);
+    }
+  }
+  return out;
+}
+
 /* every CLAS of every instance, tested in CLAS-list order */
 bool trace_one(const oracle_ctx* c, const float* xf, const float o[3], const float d[3], float tmin, float tmax, bool cull,
-               bool anyHit, Best& best) {
+               bool anyHit, Best& best, const tc_aabb* instBoxes) {
   best.h.t = tmax;
   best.h.u = best.h.v = 0.0f;
   best.h.instanceID = best.h.clusterID = best.h.primitiveID = 0xFFFFFFFFu;
@@ -1456,6 +1490,7 @@ bool trace_one(const oracle_ctx* c, const float* xf, const float o[3], const flo
     xform34(xf + 24 * i + 12, d, 0.0f, dl);
     Ray r;
     setup_ray(r, ol, dl);
+    if (box_hit(r, instBoxes[i].lo, instBoxes[i].hi, tmin, best.h.t) == INFINITY) continue;
     const uint64_t* refs = P<const uint64_t>(bi.clusterReferences);
     for (uint32_t k = 0; k < bi.clusterReferencesCount; k++) {
       const char* clas = P<const char>(refs[k]);
@@ -1592,12 +1627,34 @@ void decode(const oracle_ctx* c, const float* xf, const Best& b, tc_hit_decoded&
   out.depth = pp[2] / pp[3];
 }
 
+/* one ray: closest hit, its decode and (flags & 2) the shadow ray from the
+ * hit point toward the sun (render_shading.glsl:263-274) */
+void trace_and_decode(const oracle_ctx* c, const float* xforms, const tc_aabb* instBoxes, const float o[3], const float d[3],
+                      float tmin, float tmax, uint32_t flags, tc_hit* hitOut, tc_hit_decoded* decOut) {
+  bool cull = (flags & 1u) != 0, shadows = (flags & 2u) != 0;
+  Best b;
+  trace_one(c, xforms, o, d, tmin, tmax, cull, false, b, instBoxes);
+  tc_hit hit = b.h;
+  if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
+  if (hitOut) *hitOut = hit;
+  if (!decOut) return;
+  tc_hit_decoded dec;
+  decode(c, xforms, b, dec);
+  if (shadows && hit.instanceID != 0xFFFFFFFFu) {
+    const float* sun = c->view->skyParams.sunDirection;
+    float sd[3] = {sun[0], sun[1], sun[2]};
+    Best sb;
+    dec.shadowed = trace_one(c, xforms, dec.wPos, sd, 0.001f, 10000000.0f, false, true, sb, instBoxes) ? 1u : 0u;
+  }
+  *decOut = dec;
+}
+
 }  // namespace
 
 extern "C" void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_ray* rays, uint32_t numRays, uint32_t width,
                              uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded, uint32_t nthreads) {
   uint32_t n = rays ? numRays : width * height;
-  bool cull = (flags & 1u) != 0, shadows = (flags & 2u) != 0;
+  const std::vector<tc_aabb> boxes = instance_boxes(c);
   parallel_for(n, nthreads, [&](uint32_t r) {
     float o[3], d[3], tmin, tmax;
     if (rays) {
@@ -1607,22 +1664,26 @@ extern "C" void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_
       primary_ray(c->view, r % width, r / width, width, height, o, d);
       tmin = c->view->nearPlane, tmax = c->view->farPlane;
     }
-    Best b;
-    trace_one(c, xforms, o, d, tmin, tmax, cull, false, b);
-    tc_hit hit = b.h;
-    if (hit.instanceID == 0xFFFFFFFFu) hit.t = -1.0f;
-    if (hits) hits[r] = hit;
-    if (!decoded) return;
-    tc_hit_decoded dec;
-    decode(c, xforms, b, dec);
-    if (shadows && hit.instanceID != 0xFFFFFFFFu) {
-      const float* sun = c->view->skyParams.sunDirection;
-      float sd[3] = {sun[0], sun[1], sun[2]};
-      Best sb;
-      dec.shadowed = trace_one(c, xforms, dec.wPos, sd, 0.001f, 10000000.0f, false, true, sb) ? 1u : 0u;
-    }
-    decoded[r] = dec;
+    trace_and_decode(c, xforms, boxes.data(), o, d, tmin, tmax, flags, hits ? hits + r : nullptr,
+                     decoded ? decoded + r : nullptr);
   });
+}
+
+extern "C" void oracle_trace_pixels(const oracle_ctx* c, const float* xforms, const uint32_t* pixels, uint32_t numPixels,
+                                    uint32_t width, uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded,
+                                    uint32_t nthreads) {
+  const std::vector<tc_aabb> boxes = instance_boxes(c);
+  parallel_for(numPixels, nthreads, [&](uint32_t r) {
+    float o[3], d[3];
+    primary_ray(c->view, pixels[r] % width, pixels[r] / width, width, height, o, d);
+    trace_and_decode(c, xforms, boxes.data(), o, d, c->view->nearPlane, c->view->farPlane, flags, hits ? hits + r : nullptr,
+                     decoded ? decoded + r : nullptr);
+  });
+}
+
+extern "C" void oracle_primary_ray(const tc_frame_constants* view, uint32_t x, uint32_t y, uint32_t width, uint32_t height,
+                                   float origin[3], float direction[3]) {
+  primary_ray(view, x, y, width, height, origin, direction);
 }
 
 uint32_t oracle_num_split_passes(uint32_t splitFactor) {

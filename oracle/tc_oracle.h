@@ -53,6 +53,9 @@ void oracle_config_factors(uint32_t cfg, uint32_t out[3]);
 void oracle_config_triangle_vertices(uint32_t cfg, uint32_t tri, uint32_t out[3]);
 void oracle_config_vertex_barycentrics(uint32_t cfg, uint32_t vert, float wuv[3]);
 float oracle_sample_height(const tc_displacement_texture* tex, float u, float v);
+/* displacement.glsl:47-104: PN-triangle position of barycentrics (u, v, w)
+ * over verts[0..2] / normals[0..2] (3 floats each) */
+void oracle_pn(const float verts[9], const float normals[9], const float bary[3], float out[3]);
 /* the ripple animation's sin / cos (FP64 reduction, FP32 Taylor series) */
 float oracle_ripple_sin(float x);
 float oracle_ripple_cos(float x);
@@ -103,6 +106,16 @@ uint32_t oracle_num_split_passes(uint32_t splitFactor);
  * worldToObject, 3x4 row-major). */
 void oracle_trace(const oracle_ctx* c, const float* xforms, const tc_ray* rays, uint32_t numRays, uint32_t width,
                   uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded, uint32_t nthreads);
+
+/* oracle_trace's camera rays for a subset of pixels (index y * width + x):
+ * pixel-sampled parity at full viewport sizes. */
+void oracle_trace_pixels(const oracle_ctx* c, const float* xforms, const uint32_t* pixels, uint32_t numPixels,
+                         uint32_t width, uint32_t height, uint32_t flags, tc_hit* hits, tc_hit_decoded* decoded,
+                         uint32_t nthreads);
+
+/* render_raytrace.rgen.glsl:52-86: origin and direction of pixel (x, y). */
+void oracle_primary_ray(const tc_frame_constants* view, uint32_t x, uint32_t y, uint32_t width, uint32_t height,
+                        float origin[3], float direction[3]);
 
 /* The chain of renderer_raytrace_clusters_tess.cpp:467-582 from
  * build_setup CLASSIFY through build_setup BUILD_BLAS (visible clusters and

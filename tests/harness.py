@@ -68,6 +68,12 @@ def oracle():
         L.oracle_table_lookup.argtypes = [P]
         L.oracle_trace.argtypes = [ctx, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
         L.oracle_trace.restype = None
+        L.oracle_trace_pixels.argtypes = [ctx, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
+        L.oracle_trace_pixels.restype = None
+        L.oracle_primary_ray.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P]
+        L.oracle_primary_ray.restype = None
+        L.oracle_pn.argtypes = [P, P, P, P]
+        L.oracle_pn.restype = None
         for n in ("oracle_table_num_vertices", "oracle_table_num_triangles", "oracle_table_num_configs"):
             getattr(L, n).restype = C.c_uint32
         for n, t in (("oracle_table_vertices", C.c_uint32), ("oracle_table_triangles", C.c_uint32),
@@ -125,6 +131,23 @@ class Results:
         self.arrays = {}
 
 
+def gather_ranges(fetch, base, offsets, sizes) -> np.ndarray:
+    """Bytes [base + offsets[i], + sizes[i]) of every range, concatenated
+    in list order, from one bulk fetch of the span they cover (frames of a
+    few million CLAS would otherwise take one copy per block)."""
+    offs = np.asarray(offsets, np.int64)
+    lens = np.asarray(sizes, np.int64)
+    if offs.size == 0:
+        return np.zeros(0, np.uint8)
+    lo = int(offs.min())
+    hi = int((offs + lens).max())
+    pool = fetch(int(base) + lo, hi - lo)
+    rel = offs - lo
+    if rel[0] == 0 and np.array_equal(rel[1:], (rel + lens)[:-1]):
+        return pool[:int(rel[-1] + lens[-1])].copy()  # contiguous in list order
+    return np.concatenate([pool[o:o + n] for o, n in zip(rel.tolist(), lens.tolist())])
+
+
 def _finish_results(sb, rb, fetch, cfg, template_id):
     """fetch(addr, nbytes) -> np.uint8 array.  template_id maps a template
     address to a comparable identity."""
@@ -153,11 +176,23 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
     R.arrays["tempClusterOffsets"] = ca - np.uint64(base["genClusterData"])
     sizes = fetch(base["tempClusterSizes"], 4 * ntemp).view(np.uint32)
     R.arrays["tempClusterSizes"] = sizes
-    R.arrays["genVertices"] = fetch(base["genVertices"], 12 * b.genVertexCounter).view(np.float32).reshape(-1, 3) \
-        if b.genVertexCounter <= cfg.maxGeneratedVertices else None
+    nvtx = b.genVertexCounter
+    R.genVerticesPrefix = False
+    if nvtx > cfg.maxGeneratedVertices:
+        # overflow: the accepted vertex allocations form a prefix of the
+        # list, so every vertex below the last accepted record's start is
+        # written (by records of either kind)
+        starts = [int(vb.max()) if ntemp else 0]
+        if cfg.flags & abi.TRANSIENT_FLAGS:
+            ntr = min(b.transBuildCounter, cfg.maxGeneratedClusters)
+            tvb = fetch(base["transBuilds"], 64 * ntr).view(abi.CLAS_BUILD_INFO_DT)["vertexBuffer"]
+            if ntr:
+                starts.append(int(tvb.max()) - int(base["genVertices"]))
+        nvtx = max(starts) // 12
+        R.genVerticesPrefix = True
+    R.arrays["genVertices"] = fetch(base["genVertices"], 12 * nvtx).view(np.float32).reshape(-1, 3)
     # CLAS blocks, concatenated in CLAS order
-    blocks = [fetch(int(base["genClusterData"] + int(o)), int(s)) for o, s in zip(R.arrays["tempClusterOffsets"], sizes)]
-    R.arrays["clasBlocks"] = np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
+    R.arrays["clasBlocks"] = gather_ranges(fetch, base["genClusterData"], R.arrays["tempClusterOffsets"], sizes)
     bi = fetch(base["blasBuildInfos"], 16 * n_inst).view(abi.BLAS_BUILD_INFO).copy()
     bi["clusterReferences"] = (bi["clusterReferences"].astype(np.int64) - np.int64(base["blasClusterAddresses"])
                                ).astype(np.uint64)
@@ -176,16 +211,16 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
         R.arrays["transClusterOffsets"] = ta - np.uint64(base["genClusterData"])
         tsz = fetch(base["transClusterSizes"], 4 * ntr).view(np.uint32)
         R.arrays["transClusterSizes"] = tsz
-        blocks = [fetch(int(base["genClusterData"] + int(o)), int(sz)) for o, sz in zip(R.arrays["transClusterOffsets"], tsz)]
-        R.arrays["transClasBlocks"] = np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
+        R.arrays["transClasBlocks"] = gather_ranges(fetch, base["genClusterData"], R.arrays["transClusterOffsets"], tsz)
         # written bytes of every transient part slot: ClusterInfo + mappings
         # (u8 per triangle for 1X subsets, u16 for 2X batches)
         def slot_bytes(r):
             width = 2 if int(r["clusterID"]) >> 30 == abi.RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED else 1
             return 8 + width * (int(r["packed"]) & 511)
 
-        slots = [fetch(int(base["partTriangles"] + 24 * (int(r["clusterID"]) & 0x3FFFFFFF)), slot_bytes(r)) for r in tb]
-        R.arrays["transSlots"] = np.concatenate(slots) if slots else np.zeros(0, np.uint8)
+        slot_offs = 24 * (tb["clusterID"].astype(np.int64) & 0x3FFFFFFF)
+        slot_lens = np.array([slot_bytes(r) for r in tb], np.int64)
+        R.arrays["transSlots"] = gather_ranges(fetch, base["partTriangles"], slot_offs, slot_lens)
     return R
 
 
@@ -195,7 +230,9 @@ def _finish_results(sb, rb, fetch, cfg, template_id):
 class OracleFrame:
     """Host buffers of one frame and the oracle chain over them."""
 
-    def __init__(self, host_scene, cfg: abi.Config):
+    def __init__(self, host_scene, cfg: abi.Config, textures=None):
+        """textures: (array, format) pairs that replace the scene's height
+        maps (e.g. the texels a GpuFrame generated on the device)."""
         L = abi.lib()
         self.cfg = cfg
         self.keep = []
@@ -240,7 +277,7 @@ class OracleFrame:
                                    "clusterTemplateInstantiatonSizes": s.ctypes.data})
         from vk_tessellated_clusters_amd.scene import FbmTexture
         self.textures = (abi.DisplacementTexture * max(1, len(host_scene.textures)))()
-        for i, tex in enumerate(host_scene.textures):
+        for i, tex in enumerate(textures if textures is not None else host_scene.textures):
             arr, fmt = (tex.host(), tex.format) if isinstance(tex, FbmTexture) else tex
             a = keep(arr)
             self.textures[i].texels = a.ctypes.data
@@ -280,7 +317,9 @@ class OracleFrame:
             sb.transClusterSizes = alloc("transClusterSizes", 4 * cfg.maxGeneratedClusters)
             sb.transTriIndices = sb.genVertices
             sb.transTriMappings = sb.partTriangles
-        self.clas_bytes = min(cfg.maxGeneratedClusterMegs << 20, 1 << 30)
+        # the oracle fills up to maxGeneratedClusterMegs (its overflow test);
+        # np.zeros pages are only backed once written
+        self.clas_bytes = cfg.maxGeneratedClusterMegs << 20
         sb.genClusterData = alloc("genClusterData", self.clas_bytes)
         self.sb = sb
         self.rb = abi.Readback()
@@ -355,6 +394,17 @@ class OracleFrame:
                               hits.ctypes.data, dec.ctypes.data, nthreads)
         return hits, dec
 
+    def trace_pixels(self, width, height, flags, pixels, nthreads=8):
+        """Camera rays of the listed pixels (y * width + x) only."""
+        xf = np.zeros(24 * len(self.instances), np.float32)
+        abi.check(abi.lib().tc_instance_xforms_host(C.addressof(self.instances), len(self.instances), xf.ctypes.data))
+        px = np.ascontiguousarray(pixels, np.uint32)
+        hits = np.zeros(px.size, abi.HIT_DT)
+        dec = np.zeros(px.size, abi.HIT_DECODED_DT)
+        oracle().oracle_trace_pixels(C.byref(self.ctx), xf.ctypes.data, px.ctypes.data, px.size, width, height, flags,
+                                     hits.ctypes.data, dec.ctypes.data, nthreads)
+        return hits, dec
+
     def results(self) -> Results:
         def fetch(addr, nbytes):
             return np.frombuffer(C.string_at(addr, nbytes), np.uint8).copy() if nbytes else np.zeros(0, np.uint8)
@@ -403,6 +453,17 @@ class GpuFrame:
         torch.cuda.synchronize()
         return hits.cpu().numpy().view(abi.HIT_DT), dec.cpu().numpy().view(abi.HIT_DECODED_DT)
 
+    def device_textures(self):
+        """Host copies (array, format) of the height maps as the device
+        holds them."""
+        out = []
+        for i in range(self.ds.num_textures):
+            t = self.ds.textures[i]
+            dt = np.uint16 if t.format == abi.TEXTURE_R16_UNORM else np.float32
+            n = t.width * t.height * np.dtype(dt).itemsize
+            out.append((self.r.copy(t.texels, n).view(dt).reshape(t.height, t.width), t.format))
+        return out
+
     def accel_arrays(self):
         """Host copies of the built acceleration structure."""
         a, _ = self.r.accel()
@@ -431,6 +492,24 @@ class GpuFrame:
         self.r.close()
 
 
+def compare_hits(gh, oh, gd, od, min_agree=0.9999):
+    """Closest-hit identities (instance, cluster, primitive) agree on at
+    least min_agree of the rays, t within 1e-5 relative where they agree,
+    and the decoded base triangle, sub-triangle, cluster and shadow flag on
+    at least min_agree of those rays.  Returns the agreement."""
+    gid = np.stack([gh["instanceID"], gh["clusterID"], gh["primitiveID"]], 1)
+    oid = np.stack([oh["instanceID"], oh["clusterID"], oh["primitiveID"]], 1)
+    same = np.all(gid == oid, axis=1)
+    agree = same.mean()
+    assert agree >= min_agree, "hit agreement %.6f" % agree
+    hit = same & (oh["t"] > 0)
+    rel = np.abs(gh["t"][hit] - oh["t"][hit]) / np.maximum(np.abs(oh["t"][hit]), 1e-30)
+    assert rel.max(initial=0) <= 1e-5, "hit t differs by %g relative" % rel.max(initial=0)
+    for k in ("triangleID", "subTriangleID", "clusterID", "shadowed"):
+        assert (gd[k][same] == od[k][same]).mean() >= min_agree, k
+    return agree
+
+
 def ulp_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """|ULP distance| between float32 arrays of equal shape."""
     ia = a.astype(np.float32).view(np.int32).astype(np.int64)
@@ -442,6 +521,7 @@ def ulp_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 def compare(gpu: Results, ref: Results, max_ulp: int = 2):
     """Raises AssertionError naming the first differing output."""
+    assert gpu.readback["errorFlags"] == 0, "frame flagged invalid: errorFlags 0x%x" % gpu.readback["errorFlags"]
     assert gpu.counters == ref.counters, _diff_dict("SceneBuilding", gpu.counters, ref.counters)
     assert gpu.readback == ref.readback, _diff_dict("Readback", gpu.readback, ref.readback)
     keys = ["instanceStates", "visibleClusters", "splitTriangles", "partTriangles", "tempInstantiations",
@@ -457,8 +537,8 @@ def compare(gpu: Results, ref: Results, max_ulp: int = 2):
                 k, bad.size, bad[0], a[bad[0]], b[bad[0]]))
     assert gpu.arrays["templateIdentity"] == ref.arrays["templateIdentity"], "template identities differ"
     gv, rv = gpu.arrays["genVertices"], ref.arrays["genVertices"]
-    if gv is not None and rv is not None:
-        assert gv.shape == rv.shape, "genVertices shape"
+    assert gv.shape == rv.shape, "genVertices shape %s vs %s" % (gv.shape, rv.shape)
+    if not np.array_equal(gv.view(np.uint32), rv.view(np.uint32)):
         d = ulp_diff(gv, rv)
         assert d.max(initial=0) <= max_ulp, "genVertices differ by %d ULP" % d.max()
     cb, rbk = gpu.arrays["clasBlocks"], ref.arrays["clasBlocks"]
@@ -469,3 +549,68 @@ def compare(gpu: Results, ref: Results, max_ulp: int = 2):
 def _diff_dict(name, a, b):
     d = {k: (a.get(k), b.get(k)) for k in set(a) | set(b) if a.get(k) != b.get(k)}
     return "%s differs: %s" % (name, dict(list(d.items())[:12]))
+
+
+# ------------------------------------------------------- BVH key order
+
+
+def _spread8(q: np.ndarray) -> np.ndarray:
+    """Bits 0..7 of q moved to bits 0, 3, 6, ... 21."""
+    q = q.astype(np.uint32)
+    q = (q | (q << np.uint32(8))) & np.uint32(0x0300F00F)
+    q = (q | (q << np.uint32(4))) & np.uint32(0x030C30C3)
+    q = (q | (q << np.uint32(2))) & np.uint32(0x09249249)
+    return q
+
+
+def blas_keys_host(lo: np.ndarray, hi: np.ndarray, geo_lo: np.ndarray, geo_hi: np.ndarray,
+                   inst: np.ndarray) -> np.ndarray:
+    """64-bit BLAS sort keys recomputed on the host in FP32: the instance
+    index above a 24-bit Morton code (8 bits per axis) of each CLAS box
+    centre normalised to its instance's undisplaced geometry bounds, with
+    the device's operation order ((lo + hi) * 0.5, (c - lo) / extent,
+    clamp, truncation of n * 255 + 0.5)."""
+    f = np.float32
+    c = (lo.astype(f) + hi.astype(f)) * f(0.5)
+    ext = (geo_hi - geo_lo).astype(f)
+    safe = np.where(ext > 0, ext, f(1))
+    n = np.where(ext > 0, (c - geo_lo.astype(f)) / safe, f(0.5)).astype(f)
+    n = np.minimum(np.maximum(n, f(0)), f(1))
+    q = (n * f(255) + f(0.5)).astype(np.uint32)
+    code = _spread8(q[:, 0]) | (_spread8(q[:, 1]) << np.uint32(1)) | (_spread8(q[:, 2]) << np.uint32(2))
+    return (inst.astype(np.uint64) << np.uint64(32)) | code.astype(np.uint64)
+
+
+def check_blas_key_order(g: "GpuFrame", arr: dict) -> int:
+    """Asserts that the BVH leaves of every instance are the stable sort of
+    the inserted CLAS references by their recomputed 64-bit keys (one sort
+    over all instances, as the device runs it); returns the reference
+    count."""
+    refs = arr["refs"].astype(np.uint64)
+    total = refs.size
+    if total == 0:
+        return 0
+    infos = arr["infos"]
+    base = int(arr["sb"].blasClusterAddresses)
+    counts = infos["clusterReferencesCount"].astype(np.int64)
+    starts = (infos["clusterReferences"].astype(np.int64) - base) // 8
+    inst = np.zeros(total, np.int64)
+    for i in np.nonzero(counts)[0]:
+        inst[starts[i]:starts[i] + counts[i]] = i
+    # CLAS headers of every reference from one bulk copy of the span
+    lo_a = int(refs.min())
+    span = g.r.copy(lo_a, int(refs.max()) - lo_a + 48).view(np.uint32)  # blocks are 16-byte aligned
+    rel = (refs - np.uint64(lo_a)).astype(np.int64) // 4
+    hdr = span[rel[:, None] + np.arange(12)].reshape(-1).view(abi.CLAS_HEADER_DT)
+    ri = g.ds.instances
+    glo = np.array([list(ri[i].geoLo)[:3] for i in range(len(ri))], np.float32)
+    ghi = np.array([list(ri[i].geoHi)[:3] for i in range(len(ri))], np.float32)
+    keys = blas_keys_host(hdr["lo"], hdr["hi"], glo[inst], ghi[inst], inst)
+    order = np.argsort(keys, kind="stable")
+    want = refs[order]
+    got = arr["blasLeaves"][:total]
+    if not np.array_equal(got, want):
+        bad = np.nonzero(got != want)[0]
+        raise AssertionError("BVH leaf order differs from the key sort at %d of %d positions (first %d)"
+                             % (bad.size, total, bad[0]))
+    return total

@@ -3,6 +3,8 @@ library and the CPU oracle run the same seeded frames and must agree bit
 for bit on every counter, list, record, CLAS block and BLAS reference list,
 and within 2 ULP on generated positions (the kernels reproduce the
 oracle's float evaluation order, so the observed difference is 0 ULP)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -130,6 +132,54 @@ def split_queue_error(g):
     """The persistent split's error word (a wait hit its bound)."""
     ws = g.r.bindings.workspace
     return int(g.r.copy(ws + 256 + 12, 4).view(np.uint32)[0])
+
+
+def test_persistent_split_wait_bound_is_reported():
+    """A persistent-split wait that reaches its bound ends every wait of the
+    launch and flags the frame in tc_readback.errorFlags.  The bound is
+    lowered from the host (tc_tuning.splitWaitLimit = 1 sleep), so every
+    wait of the launch still ends at once.  Only classify and the split
+    stage run on the flagged frame (the split list is reset to all-ones at
+    frame start, so every item the launch reads is either valid or
+    rejected); the next frame, with the default bound, is clean and matches
+    the oracle."""
+    host = small_grid(tess_rate_pixels=1.0)
+    cfg = harness.small_config(flags=abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_PERSISTENT_SPLIT,
+                               maxSplitFactor=2)
+    of = harness.OracleFrame(host, cfg)
+    of.run(host.frame())
+    ref = of.results()
+    L = abi.lib()
+    fc = host.frame()
+    g = harness.GpuFrame(host, cfg)
+    try:
+        g.run(fc)
+        t = abi.Tuning()
+        abi.check(L.tc_get_tuning(C.byref(t)))
+        t.splitWaitLimit = 1
+        abi.check(L.tc_set_tuning(C.byref(t)))
+        r = g.r
+        b = C.byref(r.bindings)
+        print("forced wait bound: classify + persistent split only", flush=True)
+        abi.check(L.tc_renderer_begin_frame(r.handle, C.byref(fc), None))
+        abi.check(L.tc_instances_classify(b, cfg, None))
+        abi.check(L.tc_clusters_cull(b, cfg, None))
+        abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_CLASSIFY, None))
+        abi.check(L.tc_cluster_classify(b, cfg, None))
+        abi.check(L.tc_build_setup(b, cfg, abi.BUILD_SETUP_SPLIT, None))
+        abi.check(L.tc_triangle_split(b, cfg, None))
+        abi.check(L.tc_device_synchronize())
+        flagged = r.readback().errorFlags
+        assert split_queue_error(g) == 1
+        assert flagged == abi.READBACK_ERROR_SPLIT_WAIT
+        abi.check(L.tc_set_tuning(None))
+        g.run(fc)
+        res = g.results()
+    finally:
+        abi.check(L.tc_set_tuning(None))
+        g.close()
+    assert res.readback["errorFlags"] == 0
+    harness.compare(res, ref)
 
 
 def test_persistent_split_parity():

@@ -37,6 +37,8 @@ RT_CLUSTER_MODE_SINGLE_TESSELLATED = 1
 RT_CLUSTER_MODE_1X_SUBSET_CLUSTER = 2
 RT_CLUSTER_MODE_2X_BATCHED_TESSELLATED = 3
 
+READBACK_ERROR_SPLIT_WAIT = 1
+
 TEXTURE_R16_UNORM = 0
 TEXTURE_R32_FLOAT = 1
 
@@ -125,7 +127,7 @@ class Readback(C.Structure):
         ("numBlasReservedSizes", u32), ("numBlasActualSizes", u32), ("debugU64", u64),
         ("clusterTriangleId", u32), ("_packedDepth0", u32), ("instanceId", u32), ("_packedDepth1", u32),
         ("debugI", i32), ("debugUI", u32), ("debugF", u32),
-        ("debugA", u32 * 64), ("debugB", u32 * 64), ("debugC", u32 * 64),
+        ("debugA", u32 * 64), ("debugB", u32 * 64), ("debugC", u32 * 64), ("errorFlags", u32),
     ]
 
 
@@ -238,7 +240,7 @@ class Bindings(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("classifyBlocksPerCU", u32), ("splitBlocksPerCU", u32), ("instantiateBlocksPerCU", u32),
                 ("clasWavesPerCU", u32), ("traceBlocksPerCU", u32), ("traceThreads", u32), ("bvhBlocksPerCU", u32),
-                ("_reserved", u32)]
+                ("splitWaitLimit", u32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if not k.startswith("_")}

@@ -26,7 +26,8 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
   uint32_t tilesPart = (cfg.maxPartTriangles + kWave - 1) / kWave;
   L.maxTiles = std::max(std::max(cfg.maxVisibleClusters, tilesSplit), std::max(tilesPart, 1u));
   L.sortCapacity = std::max(cfg.maxGeneratedClusters, 1u);
-  L.sortMaxTiles = (L.sortCapacity + kSortTileSmall - 1) / kSortTileSmall;
+  // tiles of the larger of the CLAS sort and the TLAS instance sort
+  L.sortMaxTiles = (std::max(L.sortCapacity, kMaxInstances) + kSortTileSmall - 1) / kSortTileSmall;
   L.instanceCapacity = kMaxInstances;
   uint64_t o = kWsHeaderBytes;
   L.status = o;
@@ -133,7 +134,10 @@ int num_cu_public() { return num_cu(); }
 // defaults from the autotuner on the 64-instance grid frame: traversal with
 // about one ray per thread (256-thread workgroups, 32 per CU) hides its
 // memory latency best
-static constexpr tc_tuning kDefaultTuning = {5, 8, 2, 64, 512, 64, 8, 0};
+// The last field is the sleeps (s_sleep 2, about 128 clocks each) before a
+// persistent-split wait gives up: seconds, far beyond any frame's split time.
+static constexpr uint32_t kSplitWaitLimitDefault = 1u << 24;
+static constexpr tc_tuning kDefaultTuning = {5, 8, 2, 64, 512, 64, 8, kSplitWaitLimitDefault};
 static std::mutex g_tuningMutex;
 static tc_tuning g_tuning = kDefaultTuning;
 
@@ -158,7 +162,7 @@ extern "C" tc_status tc_set_tuning(const tc_tuning* t) {
   }
   std::lock_guard<std::mutex> lock(g_tuningMutex);
   g_tuning = t ? *t : kDefaultTuning;
-  g_tuning._reserved = 0;
+  if (g_tuning.splitWaitLimit == 0) g_tuning.splitWaitLimit = kSplitWaitLimitDefault;
   return TC_OK;
 }
 
@@ -502,7 +506,7 @@ static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t p
   uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
   uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU));
   if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
-    hipLaunchKernelGGL(k_split_queue_init, dim3(1), dim3(256), 0, st, a, passes);
+    hipLaunchKernelGGL(k_split_queue_init, dim3(1), dim3(256), 0, st, a, passes, tuning_snapshot().splitWaitLimit);
     hipLaunchKernelGGL(k_triangle_split_persistent, dim3(grid), dim3(64), 0, st, a);
   } else {
     hipLaunchKernelGGL(k_triangle_split, dim3(grid), dim3(64), 0, st, a);

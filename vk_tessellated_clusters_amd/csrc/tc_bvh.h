@@ -45,7 +45,8 @@ struct BvhArgs {
 struct AccelLayout {
   uint64_t blasNodes, blasLeaves, nodeBox, child, parent, counter, rangeEnd, pendingCount, pendingList;
   uint64_t blasNodes4, tlasNodes4, blasLeafInfo;
-  uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasRoot;
+  uint64_t tlasInstances, tlasNodes, tlasNodeBox, tlasLeafBox, tlasChild, tlasParent, tlasCounter, tlasKeys, tlasVals,
+      tlasRoot;
   uint64_t total;
 };
 

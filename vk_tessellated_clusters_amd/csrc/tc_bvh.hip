@@ -2,15 +2,19 @@
 //
 // Replaces the NV cluster BLAS build and the TLAS update
 // (renderer_raytrace_clusters_tess.cpp:695-745, renderer.cpp:318-337):
-//   keys    30-bit Morton code of every CLAS centre inside its instance's
-//           geometry bounds, above it the instance index (one 64-bit key);
-//   sort    the LDS radix sort of tc_sort.hip (stable, CLAS order breaks ties);
+//   keys    24-bit Morton code (8 bits per axis) of every CLAS centre inside
+//           its instance's geometry bounds, the instance index above it:
+//           (instance << 32) | morton24, one 64-bit key;
+//   sort    the LDS radix sort of tc_sort.hip over the three Morton digits
+//           and the one or two instance digits (stable, CLAS order breaks
+//           ties);
 //   build   Karras' binary radix tree over each instance's key segment
 //           (internal nodes of instance i live at [s_i, s_i + n_i - 1));
 //   refit   bottom-up with one arrival counter per internal node, in LDS
 //           for nodes whose key range lies inside one workgroup's chunk of
 //           sorted positions, in global memory for the few that straddle;
-//   TLAS    the same builder over the instances' world-space boxes.
+//   TLAS    the same builder over the instances' world-space boxes, with a
+//           30-bit Morton code (morton_of) per instance.
 #include <algorithm>
 
 #include "tc_bvh.h"
@@ -469,24 +473,8 @@ __global__ void k_tlas_collapse4(BvhArgs a) {
   if (i + 1 < a.numInstances) collapse4(a.tlasNodes, i, a.tlasNodes4);
 }
 
-// TLAS keys sorted in one workgroup (up to kTlasRankSort instances): the
-// keys are distinct (the instance index is their low word), so every key's
-// rank is the count of smaller keys; in place
+// instances the one-workgroup TLAS build (k_tlas_build_small) takes
 constexpr uint32_t kTlasRankSort = 1024;
-__global__ __launch_bounds__(1024) void k_tlas_rank_sort(BvhArgs a) {
-  __shared__ uint64_t s_key[kTlasRankSort];
-  __shared__ uint32_t s_val[kTlasRankSort];
-  const uint32_t n = min(a.numInstances, kTlasRankSort);
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_key[i] = a.tlasKeys[i], s_val[i] = a.tlasVals[i];
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const uint64_t k = s_key[i];
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < n; j++) rank += s_key[j] < k ? 1u : 0u;
-    a.tlasKeys[rank] = k;
-    a.tlasVals[rank] = s_val[i];
-  }
-}
 
 // TLAS root word (internal node 0, or the single instance) and the TLAS
 // element count read by the sort
@@ -595,6 +583,8 @@ AccelLayout accel_layout(uint32_t capacity, uint32_t numInstances) {
   put(L.tlasChild, 2 * n * 4);
   put(L.tlasParent, 2 * n * 4);
   put(L.tlasCounter, n * 4);
+  put(L.tlasKeys, n * 8);
+  put(L.tlasVals, n * 4);
   put(L.tlasRoot, 256);
   L.total = o;
   return L;
@@ -639,8 +629,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   AccelLayout L = accel_layout(capacity, accel->numInstances);
   if (accel->memoryBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   WsLayout W = ws_layout(config, b->numRenderInstances);
-  // the instance sort of a large TLAS runs in the same key buffers
-  if (W.sortCapacity < capacity || W.sortCapacity < accel->numInstances) return TC_ERROR_INVALID_ARGUMENT;
+  if (W.sortCapacity < capacity) return TC_ERROR_INVALID_ARGUMENT;
   char* m = static_cast<char*>(accel->memory);
   char* w = static_cast<char*>(b->workspace);
   hipStream_t st = (hipStream_t)stream;
@@ -697,36 +686,30 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
                      dim3(kRefitThreads), 0, st, a);
   hipLaunchKernelGGL(k_blas_refit_top, dim3(a.numInstances), dim3(kRefitThreads), 0, st, a);
   hipLaunchKernelGGL(k_blas_collapse4, dim3(grid), dim3(256), 0, st, a);
-  // TLAS over instances; its sort reuses the other ping-pong slot
-  a.tlasKeys = sb.keys[res ^ 1];
-  a.tlasVals = sb.vals[res ^ 1];
+  // TLAS over instances, with key and value arrays of its own
+  a.tlasKeys = reinterpret_cast<uint64_t*>(m + L.tlasKeys);
+  a.tlasVals = reinterpret_cast<uint32_t*>(m + L.tlasVals);
   if (a.numInstances <= kTlasRankSort) {
     hipLaunchKernelGGL(k_tlas_build_small, dim3(1), dim3(kTlasRankSort), 0, st, a, instanceXforms);
     return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
   }
   hipLaunchKernelGGL(k_tlas_prep, dim3((a.numInstances + 63) / 64), dim3(64), 0, st, a, instanceXforms);
   hipLaunchKernelGGL(k_tlas_keys, dim3(1), dim3(256), 0, st, a);
-  // the TLAS sort runs on the second ping-pong slot of the workspace; its
-  // scratch slot is the TLAS child / parent arrays, written only afterwards
-  // (32 key bits = 4 passes, so the result lands back in slot 0)
+  // the TLAS sort's scratch slot is the TLAS child / parent arrays, written
+  // only afterwards (32 key bits = 4 passes, so the result lands back in
+  // slot 0); the workspace histogram holds kMaxInstances keys' tiles
   SortBuffers tb = sb;
-  tb.keys[0] = sb.keys[res ^ 1];
-  tb.vals[0] = sb.vals[res ^ 1];
+  tb.keys[0] = a.tlasKeys;
+  tb.vals[0] = a.tlasVals;
   tb.keys[1] = reinterpret_cast<uint64_t*>(m + L.tlasChild);
   tb.vals[1] = reinterpret_cast<uint32_t*>(m + L.tlasParent);
   tb.capacity = a.numInstances;
   tb.maxTiles = (a.numInstances + kSortTileSmall - 1) / kSortTileSmall;
   hipLaunchKernelGGL(k_tlas_root, dim3(1), dim3(64), 0, st, a);
-  if (a.numInstances <= kTlasRankSort) {
-    hipLaunchKernelGGL(k_tlas_rank_sort, dim3(1), dim3(kTlasRankSort), 0, st, a);
-    a.tlasSortedKeys = a.tlasKeys;
-    a.tlasSortedVals = a.tlasVals;
-  } else {
-    uint32_t tres = 0;
-    radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);
-    a.tlasSortedKeys = tb.keys[tres];
-    a.tlasSortedVals = tb.vals[tres];
-  }
+  uint32_t tres = 0;
+  radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);
+  a.tlasSortedKeys = tb.keys[tres];
+  a.tlasSortedVals = tb.vals[tres];
   const uint32_t tgrid = (a.numInstances + 255) / 256;
   hipLaunchKernelGGL(k_tlas_hierarchy, dim3(tgrid), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_tlas_refit, dim3(tgrid), dim3(256), 0, st, a);

@@ -521,8 +521,9 @@ struct SplitQueue {
   uint32_t passReady;
   uint32_t finished;
   uint32_t numPasses;
-  uint32_t error;  // a wait exceeded its spin limit
-  uint32_t _pad[4];
+  uint32_t error;      // a wait exceeded its spin limit
+  uint32_t waitLimit;  // sleeps before a wait gives up (tc_tuning.splitWaitLimit)
+  uint32_t _pad[3];
   SplitPass pass[kMaxSplitPasses];
 };
 static_assert(256 + sizeof(SplitQueue) <= kWsHeaderBytes, "split queue exceeds the workspace header");

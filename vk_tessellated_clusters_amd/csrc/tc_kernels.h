@@ -76,7 +76,7 @@ template <bool TWO_X>
 __global__ void k_cluster_classify(KArgs a);
 __global__ void k_mini_vertices(KArgs a);
 __global__ void k_triangle_split(KArgs a);
-__global__ void k_split_queue_init(KArgs a, uint32_t numPasses);
+__global__ void k_split_queue_init(KArgs a, uint32_t numPasses, uint32_t waitLimit);
 __global__ void k_triangle_split_persistent(KArgs a);
 template <bool PN>
 __global__ void k_instantiate(KArgs a);

@@ -382,7 +382,7 @@ __device__ void publish_split_passes(KArgs a, SplitQueue* q, WsHeader* h, uint32
 
 // Pass 0 of the persistent split from the prologue's snapshot (workspace
 // header), with look-back epochs for every pass.
-__global__ __launch_bounds__(256) void k_split_queue_init(KArgs a, uint32_t numPasses) {
+__global__ __launch_bounds__(256) void k_split_queue_init(KArgs a, uint32_t numPasses, uint32_t waitLimit) {
   __shared__ uint32_t s_wrap;
   WsHeader* h = a.ws;
   SplitQueue* q = split_queue(h);
@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void k_split_queue_init(KArgs a, uint32_t numP
   if (s_wrap) h->epoch = 1;
   q->numPasses = min(numPasses, kMaxSplitPasses);
   q->error = 0;
+  q->waitLimit = waitLimit;
   q->finished = 0;
   SplitPass& p0 = q->pass[0];
   p0.tileBegin = 0;
@@ -437,6 +438,7 @@ __global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
   const uint64_t transY = h->base[2];
   const ViewParams vp = view_params(a);
   const TableView T = load_table(a.tessTable);
+  const uint32_t waitLimit = ld_agent(&q->waitLimit);
   uint32_t p = 0;
   while (true) {
     const uint32_t g = take_ticket(h);
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
         if (__hip_atomic_load(&q->finished, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
         // a wait far beyond any frame's split time: flag the launch and end
         // every wait (look-backs included) so the grid drains
-        if (spins >= (1u << 24) || ld_agent(&q->error)) {
+        if (spins >= waitLimit || ld_agent(&q->error)) {
           __hip_atomic_store(&q->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
@@ -460,7 +462,13 @@ __global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
       }
     }
     found = uniform(found);
-    if (found == 0xFFFFFFFFu) break;
+    if (found == 0xFFFFFFFFu) {
+      // a launch that gave up reports it with the frame's readback
+      if (lane == 0 && ld_agent(&q->error))
+        __hip_atomic_fetch_or(&a.readback->errorFlags, TC_READBACK_ERROR_SPLIT_WAIT, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
     p = found;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const SplitPass& Q = q->pass[p];

@@ -249,10 +249,12 @@ __device__ __forceinline__ bool clas_hit(const WoopRay& r, const char* clas, uin
 // the stack depth at which it was entered.  Bound: every visited node
 // pushes at most 3 of its 4 slots, and the four-wide view visits every
 // other level, so a ray holds at most 3 * (ceil(dT / 2) + ceil(dB / 2))
-// entries for binary depths dT (TLAS) and dB (BLAS).  Karras trees over
-// 30-bit Morton codes with index tie-breaks have depth <= 30 + log2(size):
-// dT <= 46 (2^16 instances) and dB <= 52 (2^22 references), i.e. at most
-// 3 * (23 + 26) = 147 < kStackDepth entries.
+// entries for binary depths dT (TLAS) and dB (BLAS).  A Karras tree's depth
+// is at most its distinct key bits plus the index bits that break ties.
+// TLAS keys are a 30-bit Morton code above the instance index, so dT <=
+// 30 + 16 = 46 (2^16 instances); a BLAS covers one instance's segment,
+// whose keys differ only in the 24-bit Morton code, so dB <= 24 + 23 = 47
+// (2^23 references).  That is at most 3 * (23 + 24) = 141 < kStackDepth.
 constexpr uint32_t kLdsStack = 24;
 constexpr uint32_t kStackDepth = 160;
 constexpr uint32_t kDone = 0xFFFFFFFFu;

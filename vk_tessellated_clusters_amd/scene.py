@@ -379,6 +379,28 @@ class HostScene:
 
 # ------------------------------------------------------------ workloads
 
+# the reference renderer's default flags (renderer.hpp:43-48): displacement,
+# culling, PN base surface and both transient build forms
+REFERENCE_DEFAULT_FLAGS = (abi.FLAG_DISPLACEMENT | abi.FLAG_CULLING | abi.FLAG_PN | abi.FLAG_1X_TRANSIENT |
+                           abi.FLAG_2X_TRANSIENT)
+
+
+def workload_config(large: bool = False, flags: int = None) -> abi.Config:
+    """Capacities of the benchmark frames: 1920 x 1080 workloads, or
+    (large) the 3840 x 2160 frame of about 70M displaced triangles."""
+    if large:
+        c = abi.default_config(maxVisibleClusters=1 << 18, maxPartTriangles=1 << 23, maxSplitTriangles=1 << 20,
+                               maxGeneratedClusters=1 << 23, maxGeneratedClusterMegs=12288,
+                               maxGeneratedVertices=1 << 27)
+    else:
+        c = abi.default_config(maxVisibleClusters=1 << 20, maxPartTriangles=1 << 22, maxSplitTriangles=1 << 16,
+                               maxGeneratedClusters=1 << 21, maxGeneratedClusterMegs=4096,
+                               maxGeneratedVertices=1 << 25)
+    if flags is not None:
+        c.flags = flags
+    return c
+
+
 
 def scene_icosphere(segments=8, tex_size=512, amplitude=0.05, seed=0, octaves=4, width=1280, height=720,
                     tess_rate_pixels=8.0, distance=3.0):
