@@ -717,6 +717,18 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
  * passes (0: derive from maxSplitFactor as the reference does). */
 tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t numSplitPasses, tc_stream stream);
 
+/* No counterpart in the reference: tc_frame_tessellate for callers that
+ * cannot allocate device memory or bring a workspace (code under graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * look-back state lives in a library-owned device array of 1 MiB + 1 KiB
+ * per device (b->workspace is ignored and may be null), so it calls no
+ * allocator.  Limits: ceil(maxVisibleClusters / 64), ceil(maxSplitTriangles
+ * / 64) and ceil(maxPartTriangles / 256) must each be at most 16384, else
+ * TC_ERROR_CAPACITY.  Calls that overlap on one device race on that array;
+ * calls in stream order do not. */
+tc_status tc_frame_tessellate_noalloc(const tc_bindings* b, tc_config config, uint32_t numSplitPasses,
+                                      tc_stream stream);
+
 /* Our CLAS build (replaces the NV template instantiation,
  * renderer_raytrace_clusters_tess.cpp:595-623): one block per
  * tempInstantiations entry at tempClusterAddresses[i]; writes

@@ -126,3 +126,20 @@ def test_tuning_roundtrip_and_validation():
         abi.check(L.tc_set_tuning(None))
     abi.check(L.tc_get_tuning(C.byref(t)))
     assert t.as_dict() == default
+
+
+def test_noalloc_tessellate_rejects_oversized_configs():
+    """tc_frame_tessellate_noalloc refuses configs whose tile counts exceed
+    its fixed device array, before any device call."""
+    import ctypes as C
+    L = abi.lib()
+    b = abi.Bindings()
+    for f in ("view", "tessTable", "readback", "instances", "build"):
+        setattr(b, f, 0x1000)
+    b.numRenderInstances = 1
+    for field, value in (("maxPartTriangles", (1 << 22) + 256), ("maxVisibleClusters", (1 << 20) + 64),
+                         ("maxSplitTriangles", (1 << 20) + 64)):
+        cfg = abi.default_config(**{field: value})
+        assert L.tc_frame_tessellate_noalloc(C.byref(b), cfg, 0, None) == abi.TC_ERROR_CAPACITY, field
+    bad = abi.default_config(clusterTriangleCount=0)
+    assert L.tc_frame_tessellate_noalloc(C.byref(b), bad, 0, None) == abi.TC_ERROR_INVALID_ARGUMENT

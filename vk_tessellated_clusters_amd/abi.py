@@ -17,6 +17,7 @@ u8, u16, u32, i32, u64, f32 = C.c_uint8, C.c_uint16, C.c_uint32, C.c_int32, C.c_
 
 TC_OK = 0
 TC_ERROR_INVALID_ARGUMENT = 1
+TC_ERROR_CAPACITY = 6
 TC_STATUS = {0: "TC_OK", 1: "TC_ERROR_INVALID_ARGUMENT", 2: "TC_ERROR_LAUNCH", 3: "TC_ERROR_OUT_OF_MEMORY",
              4: "TC_ERROR_NO_DEVICE", 5: "TC_ERROR_UNSUPPORTED", 6: "TC_ERROR_CAPACITY"}
 
@@ -283,7 +284,8 @@ EXPORTS = [
     "tc_device_check", "tc_workspace_size", "tc_workspace_init", "tc_table_upload", "tc_table_free",
     "tc_table_lookup_host", "tc_clas_size", "tc_cluster_template_order", "tc_instances_classify", "tc_clusters_cull",
     "tc_build_setup", "tc_cluster_classify", "tc_triangle_split", "tc_triangle_tess_template_instantiate",
-    "tc_blas_setup_insertion", "tc_blas_clusters_insert", "tc_frame_tessellate", "tc_renderer_create",
+    "tc_blas_setup_insertion", "tc_blas_clusters_insert", "tc_frame_tessellate", "tc_frame_tessellate_noalloc",
+    "tc_renderer_create",
     "tc_renderer_destroy", "tc_renderer_get_bindings", "tc_renderer_get_building", "tc_renderer_begin_frame",
     "tc_renderer_tessellate", "tc_num_split_passes", "tc_memcpy", "tc_device_synchronize", "tc_clas_instantiate",
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
@@ -337,6 +339,7 @@ def lib():
         "tc_blas_setup_insertion": (i32, [C.POINTER(Bindings), Config, P]),
         "tc_blas_clusters_insert": (i32, [C.POINTER(Bindings), Config, C.c_int, P]),
         "tc_frame_tessellate": (i32, [C.POINTER(Bindings), Config, u32, P]),
+        "tc_frame_tessellate_noalloc": (i32, [C.POINTER(Bindings), Config, u32, P]),
         "tc_renderer_create": (i32, [P, u32, P, u32, C.POINTER(Config), C.POINTER(P)]),
         "tc_renderer_destroy": (i32, [P]),
         "tc_renderer_get_bindings": (i32, [P, C.POINTER(Bindings)]),

@@ -76,8 +76,12 @@ static constexpr uint32_t kSupportedFlags =
     TC_FLAG_DISPLACEMENT | TC_FLAG_CULLING | TC_FLAG_PN | TC_FLAG_VERTEX_TAPE | TC_FLAG_1X_TRANSIENT |
     TC_FLAG_2X_TRANSIENT | TC_FLAG_ANIMATION | TC_FLAG_PERSISTENT_SPLIT | TC_FLAG_HIZ;
 
-static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr) {
-  if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build || !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
+// useWorkspace false: the caller's workspace is not used (nor required);
+// the argument block's ws / status / maxTiles are left for the caller.
+static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a, WsLayout* layout = nullptr,
+                           bool useWorkspace = true) {
+  if (!b || !b->view || !b->tessTable || !b->readback || !b->instances || !b->build) return TC_ERROR_INVALID_ARGUMENT;
+  if (useWorkspace && !b->workspace) return TC_ERROR_INVALID_ARGUMENT;
   if (!config_ok(cfg)) return TC_ERROR_INVALID_ARGUMENT;
   if (cfg.flags & ~kSupportedFlags) return TC_ERROR_UNSUPPORTED;
   if (b->numRenderInstances == 0 || b->numRenderInstances > kMaxInstances) return TC_ERROR_INVALID_ARGUMENT;
@@ -87,7 +91,7 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   // cluster and part vertices (not on 2X batch vertices)
   if ((cfg.flags & TC_FLAG_VERTEX_TAPE) && (cfg.flags & (TC_FLAG_ANIMATION | TC_FLAG_2X_TRANSIENT))) return TC_ERROR_UNSUPPORTED;
   WsLayout L = ws_layout(cfg, b->numRenderInstances);
-  if (b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
+  if (useWorkspace && b->workspaceBytes < L.total) return TC_ERROR_INVALID_ARGUMENT;
   a.view = b->view;
   a.tessTable = b->tessTable;
   a.readback = b->readback;
@@ -96,9 +100,9 @@ static tc_status make_args(const tc_bindings* b, const tc_config& cfg, KArgs& a,
   a.textures = b->displacementTextures;
   a.numInstances = b->numRenderInstances;
   a.numTextures = (cfg.flags & TC_FLAG_DISPLACEMENT) ? b->numDisplacementTextures : 0;
-  a.ws = reinterpret_cast<WsHeader*>(b->workspace);
-  a.status = reinterpret_cast<uint64_t*>(static_cast<char*>(b->workspace) + L.status);
-  a.maxTiles = L.maxTiles;
+  a.ws = useWorkspace ? reinterpret_cast<WsHeader*>(b->workspace) : nullptr;
+  a.status = useWorkspace ? reinterpret_cast<uint64_t*>(static_cast<char*>(b->workspace) + L.status) : nullptr;
+  a.maxTiles = useWorkspace ? L.maxTiles : 0;
   a._pad = 0;
   a.tape = (cfg.flags & TC_FLAG_VERTEX_TAPE) ? b->vertexTape : nullptr;
   a.cfg = cfg;
@@ -470,11 +474,7 @@ static void launch_prologue(const KArgs& a, int setupMode, uint32_t stage, hipSt
     hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, a, stage);
 }
 
-static tc_status launch_classify(const tc_bindings* b, tc_config config, int setupMode, tc_stream stream) {
-  KArgs a;
-  tc_status s = make_args(b, config, a);
-  if (s != TC_OK) return s;
-  hipStream_t st = (hipStream_t)stream;
+static tc_status launch_classify(const KArgs& a, tc_config config, int setupMode, hipStream_t st) {
   launch_prologue(a, setupMode, 0, st);
   uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().classifyBlocksPerCU);
@@ -490,18 +490,16 @@ static tc_status launch_classify(const tc_bindings* b, tc_config config, int set
 }
 
 tc_status tc_cluster_classify(const tc_bindings* b, tc_config config, tc_stream stream) {
-  return launch_classify(b, config, -1, stream);
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  return launch_classify(a, config, -1, (hipStream_t)stream);
 }
 
 // One split pass, or with TC_FLAG_PERSISTENT_SPLIT all `passes` passes in
 // one persistent launch (renderer_raytrace_clusters_tess.cpp:510-513).
-static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t passes, tc_stream stream,
-                              int setupMode = -1) {
-  KArgs a;
-  tc_status s = make_args(b, config, a);
-  if (s != TC_OK) return s;
+static tc_status launch_split(const KArgs& a, tc_config config, uint32_t passes, hipStream_t st, int setupMode = -1) {
   if ((config.flags & TC_FLAG_PERSISTENT_SPLIT) && passes > kMaxSplitPasses) return TC_ERROR_UNSUPPORTED;
-  hipStream_t st = (hipStream_t)stream;
   launch_prologue(a, setupMode, 1, st);
   uint32_t tiles = (config.maxSplitTriangles + kWave - 1) / kWave;
   uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().splitBlocksPerCU));
@@ -515,14 +513,13 @@ static tc_status launch_split(const tc_bindings* b, tc_config config, uint32_t p
 }
 
 tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream stream) {
-  return launch_split(b, config, tc_num_split_passes(config.maxSplitFactor), stream);
-}
-
-static tc_status launch_instantiate(const tc_bindings* b, tc_config config, int setupMode, tc_stream stream) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
-  hipStream_t st = (hipStream_t)stream;
+  return launch_split(a, config, tc_num_split_passes(config.maxSplitFactor), (hipStream_t)stream);
+}
+
+static tc_status launch_instantiate(const KArgs& a, tc_config config, int setupMode, hipStream_t st) {
   launch_prologue(a, setupMode, 2, st);
   uint32_t tiles = (config.maxPartTriangles + kInstTile - 1) / kInstTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().instantiateBlocksPerCU);
@@ -534,7 +531,10 @@ static tc_status launch_instantiate(const tc_bindings* b, tc_config config, int 
 }
 
 tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream) {
-  return launch_instantiate(b, config, -1, stream);
+  KArgs a;
+  tc_status s = make_args(b, config, a);
+  if (s != TC_OK) return s;
+  return launch_instantiate(a, config, -1, (hipStream_t)stream);
 }
 
 tc_status tc_blas_setup_insertion(const tc_bindings* b, tc_config config, tc_stream stream) {
@@ -568,25 +568,83 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
   return launched();
 }
 
+// renderer_raytrace_clusters_tess.cpp:467-582 on a prepared argument block;
+// each build_setup that precedes a stage runs in that stage's prologue
+// launch
+static tc_status frame_tessellate(const KArgs& a, tc_config config, uint32_t numSplitPasses, hipStream_t st) {
+  tc_status s;
+  if (numSplitPasses == 0) numSplitPasses = tc_num_split_passes(config.maxSplitFactor);
+  if ((s = launch_classify(a, config, TC_BUILD_SETUP_CLASSIFY, st)) != TC_OK) return s;
+  if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
+    // every pass, with its SPLIT_PASS setup, in one launch
+    if ((s = launch_split(a, config, numSplitPasses, st, TC_BUILD_SETUP_SPLIT)) != TC_OK) return s;
+  } else {
+    for (uint32_t p = 0; p < numSplitPasses; p++)
+      if ((s = launch_split(a, config, numSplitPasses, st, p == 0 ? TC_BUILD_SETUP_SPLIT : TC_BUILD_SETUP_SPLIT_PASS)) != TC_OK)
+        return s;
+  }
+  if ((s = launch_instantiate(a, config, TC_BUILD_SETUP_INSTANTIATE_TESS, st)) != TC_OK) return s;
+  hipLaunchKernelGGL(k_build_setup, dim3(1), dim3(1), 0, st, a, (int)TC_BUILD_SETUP_BUILD_BLAS);
+  return launched();
+}
+
 tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t numSplitPasses, tc_stream stream) {
   KArgs a;
   tc_status s = make_args(b, config, a);
   if (s != TC_OK) return s;
-  if (numSplitPasses == 0) numSplitPasses = tc_num_split_passes(config.maxSplitFactor);
-  // renderer_raytrace_clusters_tess.cpp:467-582; each build_setup that
-  // precedes a stage runs in that stage's prologue launch
-  if ((s = launch_classify(b, config, TC_BUILD_SETUP_CLASSIFY, stream)) != TC_OK) return s;
-  if (config.flags & TC_FLAG_PERSISTENT_SPLIT) {
-    // every pass, with its SPLIT_PASS setup, in one launch
-    if ((s = launch_split(b, config, numSplitPasses, stream, TC_BUILD_SETUP_SPLIT)) != TC_OK) return s;
-  } else {
-    for (uint32_t p = 0; p < numSplitPasses; p++)
-      if ((s = launch_split(b, config, numSplitPasses, stream, p == 0 ? TC_BUILD_SETUP_SPLIT : TC_BUILD_SETUP_SPLIT_PASS)) !=
-          TC_OK)
-        return s;
+  return frame_tessellate(a, config, numSplitPasses, (hipStream_t)stream);
+}
+
+// ---------------------------------------------- allocation-free variant
+//
+// tc_frame_tessellate_noalloc runs the same chain with its look-back state
+// (the workspace header with the tile ticket, epochs, counter snapshots and
+// the persistent split's pass table, then the tiles' status words) in this
+// file-scope array instead of the caller's workspace.  Each device holds
+// its own copy from module load to process end.  It starts zeroed, as
+// tc_workspace_init leaves a workspace; after that nothing needs clearing
+// between calls: every stage's prologue resets the ticket and opens a new
+// epoch, and status words of earlier epochs read as unpublished.  Calls
+// that overlap on one device race on it.
+constexpr uint32_t kScratchTiles = 16384;
+constexpr uint64_t kScratchBytes = kWsHeaderBytes + (uint64_t)kMaxCounters * kScratchTiles * 8;
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / 8];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratchMutex;
+static void* g_scratchPtr[kMaxScratchDevices];
+
+// Device address of `scratch` on the current device, looked up once per
+// device.  Host code uses only this address, never the symbol itself.
+static tc_status scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return TC_ERROR_NO_DEVICE;
+  if (dev < 0 || dev >= kMaxScratchDevices) return TC_ERROR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TC_ERROR_LAUNCH;
+    g_scratchPtr[dev] = p;
   }
-  if ((s = launch_instantiate(b, config, TC_BUILD_SETUP_INSTANTIATE_TESS, stream)) != TC_OK) return s;
-  return tc_build_setup(b, config, TC_BUILD_SETUP_BUILD_BLAS, stream);
+  *out = g_scratchPtr[dev];
+  return TC_OK;
+}
+
+tc_status tc_frame_tessellate_noalloc(const tc_bindings* b, tc_config config, uint32_t numSplitPasses, tc_stream stream) {
+  KArgs a;
+  tc_status s = make_args(b, config, a, nullptr, false);
+  if (s != TC_OK) return s;
+  // every stage's tile count must fit the status words of the array
+  if ((config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile > kScratchTiles ||
+      (config.maxSplitTriangles + kWave - 1) / kWave > kScratchTiles ||
+      (config.maxPartTriangles + kInstTile - 1) / kInstTile > kScratchTiles)
+    return TC_ERROR_CAPACITY;
+  void* base = nullptr;
+  if ((s = scratch_address(&base)) != TC_OK) return s;
+  a.ws = reinterpret_cast<WsHeader*>(base);
+  a.status = reinterpret_cast<uint64_t*>(static_cast<char*>(base) + kWsHeaderBytes);
+  a.maxTiles = kScratchTiles;
+  return frame_tessellate(a, config, numSplitPasses, (hipStream_t)stream);
 }
 
 }  // extern "C"
