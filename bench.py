@@ -94,9 +94,13 @@ def frame_sequence(host, n=100, animation=False):
 
 
 def per_mesh_ms(torch, ds, cfg, fc, stream, reps):
-    """Frame time of the per-mesh form: one renderer sized for one mesh,
-    re-pointed at each mesh in turn, the whole tessellation + CLAS + BLAS
-    chain launched once per mesh (eager launches, no graph)."""
+    """Frame time of the per-mesh form: a renderer for one mesh runs the
+    tessellation + CLAS + BLAS + TLAS chain once per mesh, all 500 chains in
+    one graph (tc_renderer_graph_capture_meshwise), so no host launch cost
+    is timed.  Its capacities, which set its grid sizes, are the largest
+    single mesh's needs, measured by running every mesh once in a renderer
+    with the batched frame's capacities.  Returns (ms, displaced triangles
+    summed over the meshes, capacities used)."""
     from vk_tessellated_clusters_amd import abi
     from vk_tessellated_clusters_amd.device import Renderer
     L = abi.lib()
@@ -106,29 +110,54 @@ def per_mesh_ms(torch, ds, cfg, fc, stream, reps):
     class One:
         instances, textures, num_textures = one, ds.textures, ds.num_textures
 
-    r1 = Renderer(One, cfg)
     xf = np.zeros(24 * n, np.float32)
     abi.check(L.tc_instance_xforms_host(C.addressof(ds.instances), n, xf.ctypes.data))
     inst_dev = torch.from_numpy(np.frombuffer(bytes(ds.instances), np.uint8).copy()).cuda()
     xf_dev = torch.from_numpy(xf.view(np.uint8)).cuda()
     s = C.c_void_p(stream.cuda_stream)
+    # each mesh's needs, in a renderer with the batched capacities
+    need = {k: 0 for k in ("vis", "part", "split", "vert", "clas", "data")}
+    tris = 0
+    big = Renderer(One, cfg)
+    for i in range(n):
+        abi.check(L.tc_renderer_rebind_instances(big.handle, C.c_void_p(inst_dev.data_ptr() + 192 * i),
+                                                 C.c_void_p(xf_dev.data_ptr() + 96 * i), 1))
+        abi.check(L.tc_renderer_frame(big.handle, C.byref(fc), None, s))
+        torch.cuda.synchronize()
+        rb = big.readback()
+        tris += rb.numTotalTriangles
+        for k, v in (("vis", rb.numVisibleClusters), ("part", rb.numPartTriangles), ("split", rb.numSplitTriangles),
+                     ("vert", rb.numGenVertices), ("clas", rb.numBlasClusters), ("data", rb.numGenDatas)):
+            need[k] = max(need[k], int(v))
+    big.close()
 
-    def frame():
-        for i in range(n):
-            abi.check(L.tc_renderer_rebind_instances(r1.handle, C.c_void_p(inst_dev.data_ptr() + 192 * i),
-                                                     C.c_void_p(xf_dev.data_ptr() + 96 * i), 1))
-            abi.check(L.tc_renderer_frame(r1.handle, C.byref(fc), None, s))
+    def up(v, q):
+        return max(q, (int(v * 1.25) + q - 1) // q * q)
 
-    frame()
+    c1 = abi.Config.from_buffer_copy(bytes(cfg))
+    c1.maxVisibleClusters = up(need["vis"], 64)
+    c1.maxPartTriangles = up(need["part"], 256)
+    c1.maxSplitTriangles = up(need["split"], 64)
+    c1.maxGeneratedVertices = up(need["vert"], 1024)
+    c1.maxGeneratedClusters = up(need["clas"], 256)
+    c1.maxGeneratedClusterMegs = up(need["data"] / (1 << 20), 1)
+    r1 = Renderer(One, c1)
+    abi.check(L.tc_renderer_graph_capture_meshwise(r1.handle, C.c_void_p(inst_dev.data_ptr()),
+                                                   C.c_void_p(xf_dev.data_ptr()), n, C.byref(fc), s), "meshwise capture")
+    for _ in range(2):
+        abi.check(L.tc_renderer_graph_launch(r1.handle, None, s))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        frame()
+        abi.check(L.tc_renderer_graph_launch(r1.handle, None, s))
     e1.record(stream)
     torch.cuda.synchronize()
     r1.close()
-    return e0.elapsed_time(e1) / reps
+    caps = {"maxVisibleClusters": c1.maxVisibleClusters, "maxPartTriangles": c1.maxPartTriangles,
+            "maxSplitTriangles": c1.maxSplitTriangles, "maxGeneratedVertices": c1.maxGeneratedVertices,
+            "maxGeneratedClusters": c1.maxGeneratedClusters, "maxGeneratedClusterMegs": c1.maxGeneratedClusterMegs}
+    return e0.elapsed_time(e1) / reps, tris, caps
 
 
 class StageTimer:
@@ -231,28 +260,37 @@ def algorithmic_bytes(r, host, cfg):
 
 def cpu_baseline(host, cfg, sample_instances, budget_s):
     """The oracle (C++ port of the reference stages) on a bounded sample of
-    the workload: the first `sample_instances` instances, all host threads."""
+    the workload: the first `sample_instances` instances, whole frames from
+    instance culling through BLAS insertion, timed on every host thread and
+    on one thread."""
     sys.path.insert(0, str(ROOT / "tests"))
     import harness
     from vk_tessellated_clusters_amd.scene import HostScene
     sub = HostScene(host.geometries, host.instances[:sample_instances], host.textures, host.camera,
                     host.tess_rate_pixels, host.name)
-    threads = min(16, os.cpu_count() or 1)
+    cores = os.cpu_count() or 1
     of = harness.OracleFrame(sub, cfg)
-    of.run(sub.frame(), nthreads=threads, stages="tess")  # warm-up
-    times, tris = [], 0
-    t_end = time.time() + budget_s
-    while time.time() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        of.run(sub.frame(), nthreads=threads, stages="all")
-        times.append(time.perf_counter() - t0)
-        tris = of.rb.numTotalTriangles
-        if len(times) >= 20:
-            break
-    med = float(np.median(times))
-    return {"value": tris / med, "unit": "displaced tris/s", "cores": threads, "kind": "port",
-            "sample": "first %d of %d instances, whole frame (classify..insert), median of %d frames, %d tris/frame"
-                      % (sample_instances, len(host.instances), len(times), tris)}
+
+    def rate(threads, budget, max_frames):
+        of.run(sub.frame(), nthreads=threads, stages="tess")  # warm-up
+        times = []
+        t_end = time.time() + budget
+        while time.time() < t_end or len(times) < 3:
+            t0 = time.perf_counter()
+            of.run(sub.frame(), nthreads=threads, stages="all")
+            times.append(time.perf_counter() - t0)
+            if len(times) >= max_frames:
+                break
+        return of.rb.numTotalTriangles / float(np.median(times)), len(times)
+
+    multi, nm = rate(cores, budget_s * 0.4, 20)
+    single, ns = rate(1, budget_s * 0.6, 7)
+    tris = of.rb.numTotalTriangles
+    return {"value": multi, "unit": "displaced tris/s", "cores": cores, "kind": "port",
+            "value_1thread": single, "threads": {"all": cores, "one": 1},
+            "sample": "first %d of %d instances, whole frame (cull..insert), %d tris/frame; median of %d frames on "
+                      "%d threads (os.cpu_count()) and of %d frames on 1 thread"
+                      % (sample_instances, len(host.instances), tris, nm, cores, ns)}
 
 
 def bench_diff(args):
@@ -317,43 +355,32 @@ def bench_diff(args):
     return 0
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="grid64")
-    ap.add_argument("--cpu-sample-instances", type=int, default=8)
-    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", dest="graph", action="store_false")
-    ap.add_argument("--per-mesh-reps", type=int, default=2)
-    ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
-                    help="pick launch shapes by measurement first (default for batched500)")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
-    ap.add_argument("--transient-1x", action="store_true",
-                    help="1X transient cluster builds (the reference renderer's default, renderer.hpp:47)")
-    ap.add_argument("--transient-2x", action="store_true",
-                    help="2X transient mini batches (the reference renderer's default, renderer.hpp:48)")
-    ap.add_argument("--animation", action="store_true",
-                    help="ripple animation of the generated vertices, clock advancing every frame")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="after timing, copy a buffer of this size once (scale reference for PMC byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
-              "see the sharding notes); run with --gpus 1", file=sys.stderr)
-        return 2
+def profiled_fields(kernel, workload):
+    """key=value lines of profiles/<kernel>_<workload>_traffic.txt."""
+    path = ROOT / "profiles" / ("%s_%s_traffic.txt" % (kernel, workload))
+    out = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                try:
+                    out[k.strip()] = float(v)
+                except ValueError:
+                    pass
+    return out
 
-    if args.workload == "diff":
-        return bench_diff(args)
 
-    import torch
-    from vk_tessellated_clusters_amd import abi
+def measure(args, torch, workload, flags=0, steps=None, warmup=None, cpu=False, per_mesh=False, autotuned=None):
+    """Times `steps` whole frames of `workload` (graph replays), then stage
+    by stage; returns the result dict of the bench line."""
+    from vk_tessellated_clusters_amd import abi, autotune
     from vk_tessellated_clusters_amd.device import DeviceScene, Renderer
 
-    host, desc = make_scene(args.workload)
-    cfg = bench_config(abi, args.transient_1x, args.workload == "large", args.animation, args.transient_2x)
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    host, desc = make_scene(workload)
+    cfg = bench_config(abi, args.transient_1x, workload == "large", args.animation, args.transient_2x)
+    cfg.flags |= flags
     ds = DeviceScene(host)
     r = Renderer(ds, cfg)
     frames = frame_sequence(host, animation=args.animation)
@@ -375,11 +402,11 @@ def main():
         r.build_accel(stream)
         r.trace(tp, stream)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         frame()
     torch.cuda.synchronize()
-    from vk_tessellated_clusters_amd import autotune
-    autotuned = args.autotune if args.autotune is not None else args.workload == "batched500"
+    if autotuned is None:
+        autotuned = args.autotune if args.autotune is not None else workload == "batched500"
     if autotuned:
         autotune.autotune(r, fc, tp, stream, reps=3, log=lambda m: print(m, file=sys.stderr, flush=True))
     if args.graph:
@@ -394,16 +421,16 @@ def main():
         torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for k in range(args.steps):
+    for k in range(steps):
         frame(k)
     e1.record(stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = e0.elapsed_time(e1) / steps
     # displaced triangles of every timed frame (the frames are deterministic;
     # counted in an untimed pass when the camera moves)
     if len(frames) > 1:
         frame_tris = []
-        for k in range(args.steps):
+        for k in range(steps):
             r.tessellate(frames[k % len(frames)], stream)
             frame_tris.append(int(r.readback().numTotalTriangles))
         mean_tris = sum(frame_tris) / len(frame_tris)
@@ -412,7 +439,7 @@ def main():
 
     # stage-by-stage frames for the stage times and the roofline kernel
     stage_ms = {}
-    nstage = max(3, min(args.steps, 10))
+    nstage = max(3, min(steps, 10))
     for _ in range(nstage):
         t = StageTimer(torch, stream)
         staged_frame(r, fc, stream, t, tp)
@@ -432,37 +459,39 @@ def main():
     dom_ms = stage_ms[dom]
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
     frame_bytes = sum(alg.values())
-    traffic = profiled_traffic(ROOFLINE_KERNELS[dom], args.workload)
-
+    traffic = profiled_traffic(ROOFLINE_KERNELS[dom], workload)
+    transient = bool(cfg.flags & abi.TRANSIENT_FLAGS)
+    flag_names = [n for n, f in (("PN", abi.FLAG_PN), ("1X transient builds", abi.FLAG_1X_TRANSIENT),
+                                 ("2X transient batches", abi.FLAG_2X_TRANSIENT), ("ripple animation", abi.FLAG_ANIMATION))
+                  if cfg.flags & f]
+    trace_prof = profiled_fields("k_trace", workload)
     out = {
         "metric": "displaced tris/s, cluster-BVH build ms/frame, Mrays/s, HBM GB/s (1 GPU)",
         "value": tris / (ms * 1e-3),
         "unit": "displaced tris/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc + (", 1X transient builds" if args.transient_1x else "") +
-                   (", 2X transient batches" if args.transient_2x else "") +
-                   (", ripple animation" if args.animation else ""),
+        "config": {"workload": desc + "".join(", " + f for f in flag_names),
                    "displaced_tris_per_frame": int(tris), "base_tris": host.num_base_triangles,
                    "frames_on_path": len(frames),
                    "clusters": host.num_clusters, "full_clusters": int(rb.numFullClusters),
                    "part_triangles": int(rb.numPartTriangles), "split_triangles": int(rb.numSplitTriangles),
-                   "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if (args.transient_1x or args.transient_2x) else 0)),
-                   "transient_clas": int(sb.transBuildCounter if (args.transient_1x or args.transient_2x) else 0),
+                   "clas_per_frame": int(sb.tempInstantiateCounter + (sb.transBuildCounter if transient else 0)),
+                   "transient_clas": int(sb.transBuildCounter if transient else 0),
                    "gen_vertices": int(sb.genVertexCounter)},
-        "capacity_ok": bool(rb.numVisibleClusters <= cfg.maxVisibleClusters and
-                            rb.numPartTriangles <= cfg.maxPartTriangles and
-                            rb.numSplitTriangles <= cfg.maxSplitTriangles and
-                            rb.numGenVertices <= cfg.maxGeneratedVertices and
-                            rb.numBlasClusters <= cfg.maxGeneratedClusters and
-                            rb.numGenDatas <= cfg.maxGeneratedClusterMegs * (1 << 20)),
+        "frame_ok": bool(rb.numVisibleClusters <= cfg.maxVisibleClusters and
+                         rb.numPartTriangles <= cfg.maxPartTriangles and
+                         rb.numSplitTriangles <= cfg.maxSplitTriangles and
+                         rb.numGenVertices <= cfg.maxGeneratedVertices and
+                         rb.numBlasClusters <= cfg.maxGeneratedClusters and
+                         rb.numGenDatas <= cfg.maxGeneratedClusterMegs * (1 << 20) and rb.errorFlags == 0),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "tuning": dict(autotune.get_tuning(), autotuned=bool(autotuned)),
         "cluster_bvh_build_ms_per_frame": bvh_ms,
@@ -473,22 +502,33 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": alg[dom],
                      "launch_ms": dom_ms},
     }
-    if args.workload == "batched500":
+    if trace_prof:
+        # traversal: HBM bytes per launch from the committed PMC summary over
+        # this run's trace time, with the occupancy that summary measured
+        tb = trace_prof.get("traffic_bytes_per_launch")
+        out["trace"] = {"ms": trace_ms, "hbm_gbs": tb / (trace_ms * 1e-3) / 1e9 if tb and trace_ms else None,
+                        "traffic_bytes_per_launch": tb, "l2_hit_pct": trace_prof.get("l2_hit_pct"),
+                        "waves_per_simd": trace_prof.get("mean_waves_per_simd"),
+                        "occupancy_pct": trace_prof.get("occupancy_pct")}
+    if per_mesh:
         # tessellation + CLAS + BLAS + TLAS of all meshes: one graph replay vs
-        # per-mesh launches (rays excluded from both)
+        # the per-mesh form (rays excluded from both)
         abi.check(r.L.tc_renderer_graph_capture(r.handle, C.byref(fc), None, C.c_void_p(stream.cuda_stream)))
         for _ in range(2):
             abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)))
         torch.cuda.synchronize()
         b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         b0.record(stream)
-        for _ in range(args.steps):
+        for _ in range(steps):
             abi.check(r.L.tc_renderer_graph_launch(r.handle, None, C.c_void_p(stream.cuda_stream)))
         b1.record(stream)
         torch.cuda.synchronize()
-        batched = b0.elapsed_time(b1) / args.steps
-        pm = per_mesh_ms(torch, ds, cfg, fc, stream, args.per_mesh_reps)
-        out["batched_vs_per_mesh"] = {"batched_graph_ms": batched, "per_mesh_ms": pm, "speedup": pm / batched}
+        batched = b0.elapsed_time(b1) / steps
+        batched_tris = int(r.readback().numTotalTriangles)
+        pm, pm_tris, caps = per_mesh_ms(torch, ds, cfg, fc, stream, args.per_mesh_reps)
+        out["batched_vs_per_mesh"] = {"batched_graph_ms": batched, "per_mesh_graph_ms": pm, "speedup": pm / batched,
+                                      "same_triangles": pm_tris == batched_tris, "displaced_tris": batched_tris,
+                                      "per_mesh_capacities": caps}
     if args.calibrate_copy_mib:
         src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -496,12 +536,79 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = src.numel() * 4
-    if not args.no_cpu:
+    if cpu:
         out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_sample_instances, args.cpu_budget_s)
-    print(json.dumps(out))
     r.close()
-    return 0
+    del ds
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
+
+def secondary(res):
+    """The few figures of a secondary workload carried on the bench line."""
+    keep = {"ms_per_step", "value", "config", "frame_ok", "stage_ms", "cluster_bvh_build_ms_per_frame",
+            "mrays_per_s", "rays_per_frame", "batched_vs_per_mesh", "steps"}
+    out = {k: v for k, v in res.items() if k in keep}
+    out["displaced_tris_per_s"] = out.pop("value")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="grid64")
+    ap.add_argument("--cpu-sample-instances", type=int, default=8)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", dest="graph", action="store_false")
+    ap.add_argument("--no-secondary", dest="secondary", action="store_false",
+                    help="skip the secondary workloads the default grid64 line carries")
+    ap.add_argument("--per-mesh-reps", type=int, default=3)
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
+                    help="pick launch shapes by measurement first (default for batched500)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--transient-1x", action="store_true",
+                    help="1X transient cluster builds (the reference renderer's default, renderer.hpp:47)")
+    ap.add_argument("--transient-2x", action="store_true",
+                    help="2X transient mini batches (the reference renderer's default, renderer.hpp:48)")
+    ap.add_argument("--reference-defaults", action="store_true",
+                    help="the reference renderer's default flags: PN, 1X and 2X transient builds (renderer.hpp:43-48)")
+    ap.add_argument("--animation", action="store_true",
+                    help="ripple animation of the generated vertices, clock advancing every frame")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after timing, copy a buffer of this size once (scale reference for PMC byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the frame shards by instance; "
+              "see the sharding notes); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    if args.workload == "diff":
+        return bench_diff(args)
+
+    import torch
+    from vk_tessellated_clusters_amd import abi, scene
+
+    flags = scene.REFERENCE_DEFAULT_FLAGS if args.reference_defaults else 0
+    out = measure(args, torch, args.workload, flags=flags, cpu=not args.no_cpu,
+                  per_mesh=args.workload == "batched500")
+    if args.workload == "grid64" and args.secondary and not (args.transient_1x or args.transient_2x or args.animation
+                                                             or args.reference_defaults):
+        # the same frame with the reference renderer's default flags, the
+        # 500-mesh batched frame against its per-mesh form, and the 4K
+        # spline sequence, each as a few timed frames
+        sec = {}
+        for name, wl, fl, kw in (("reference_default_flags", "grid64", scene.REFERENCE_DEFAULT_FLAGS, {}),
+                                 ("batched500", "batched500", 0, {"per_mesh": True, "autotuned": False}),
+                                 ("large", "large", 0, {})):
+            print("secondary workload:", name, file=sys.stderr, flush=True)
+            sec[name] = secondary(measure(args, torch, wl, flags=fl, steps=min(args.steps, 10), warmup=2, **kw))
+        out["secondary"] = sec
+    print(json.dumps(out))
+    return 0
 
 if __name__ == "__main__":
     sys.exit(main())

@@ -884,6 +884,18 @@ tc_status tc_renderer_graph_capture(tc_renderer* renderer, const tc_frame_consta
                                     const tc_trace_params* params, tc_stream stream);
 tc_status tc_renderer_graph_launch(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
 
+/* No counterpart in the reference (its frame processes every instance in
+ * one dispatch per stage).  The per-mesh form of a multi-mesh frame, for
+ * comparison with the batched frame: one graph that runs the chain
+ * tessellation .. CLAS .. BLAS insertion .. BLAS / TLAS build once per
+ * instance record of devInstances (count records in device memory, with 24
+ * floats per record in devXforms as tc_instance_xforms_host writes them),
+ * on a renderer created for one instance.  Replay with
+ * tc_renderer_graph_launch. */
+tc_status tc_renderer_graph_capture_meshwise(tc_renderer* renderer, const tc_render_instance* devInstances,
+                                             const float* devXforms, uint32_t count, const tc_frame_constants* frameHost,
+                                             tc_stream stream);
+
 /* objectToWorld and worldToObject (3x4 row-major, 24 floats) per instance,
  * the transforms the renderer uploads; worldToObject is inverted in FP64. */
 tc_status tc_instance_xforms_host(const tc_render_instance* instances, uint32_t n, float* out);

@@ -290,7 +290,8 @@ EXPORTS = [
     "tc_renderer_tessellate", "tc_num_split_passes", "tc_memcpy", "tc_device_synchronize", "tc_clas_instantiate",
     "tc_table_templates_host", "tc_accel_size", "tc_accel_bind", "tc_accel_build", "tc_trace_rays",
     "tc_renderer_get_accel", "tc_renderer_build_accel", "tc_renderer_trace", "tc_renderer_frame",
-    "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_fbm_height",
+    "tc_instance_xforms_host", "tc_renderer_graph_capture", "tc_renderer_graph_launch", "tc_renderer_graph_capture_meshwise",
+    "tc_fbm_height",
     "tc_renderer_rebind_instances", "tc_depth_adjoint", "tc_get_tuning", "tc_set_tuning", "tc_clas_build_transient",
     "tc_clusterize_mesh", "tc_hiz_setup", "tc_hiz_clear", "tc_hiz_build",
 ]
@@ -362,6 +363,7 @@ def lib():
         "tc_instance_xforms_host": (i32, [P, u32, P]),
         "tc_renderer_graph_capture": (i32, [P, C.POINTER(FrameConstants), C.POINTER(TraceParams), P]),
         "tc_renderer_graph_launch": (i32, [P, C.POINTER(FrameConstants), P]),
+        "tc_renderer_graph_capture_meshwise": (i32, [P, P, P, u32, C.POINTER(FrameConstants), P]),
         "tc_fbm_height": (i32, [P, u32, u32, u32, u32, u32, P]),
         "tc_depth_adjoint": (i32, [C.POINTER(AdjointParams), P]),
         "tc_clas_build_transient": (i32, [C.POINTER(Bindings), Config, P]),

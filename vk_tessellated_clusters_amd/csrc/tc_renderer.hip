@@ -479,24 +479,83 @@ tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {
   return TC_OK;
 }
 
-// Frame start: frame constants upload (from pinned host memory) and
+}  // extern "C"
+
+// Frame start: frame constants upload (from pinned host memory, unless
+// fetchView is false: the frame keeps the constants already fetched) and
 // per-frame state reset.
-tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
-  if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
-  Renderer* r = R(rr);
-  hipStream_t st = (hipStream_t)stream;
-  // during graph capture only the fetch is recorded; each replay fills its
-  // slot in tc_renderer_graph_launch
-  const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
-  if (!r->capturing && !ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
-  if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
-  hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter, r->view,
-                     r->viewLast);
-  if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
+static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, hipStream_t st, bool fetchView) {
+  if (fetchView) {
+    // during graph capture only the fetch is recorded; each replay fills
+    // its slot in tc_renderer_graph_launch
+    const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
+    if (!r->capturing && !ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
+    if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
+    hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter,
+                       r->view, r->viewLast);
+    if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
+  }
   hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
                      (const tc_frame_constants*)r->view, r->readback, reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles),
                      r->splitBytes / 4);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+static tc_status tessellate(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream, bool fetchView) {
+  Renderer* r = R(rr);
+  tc_status s;
+  if ((s = begin_frame(r, frameHost, (hipStream_t)stream, fetchView)) != TC_OK) return s;
+  tc_bindings b;
+  tc_renderer_get_bindings(rr, &b);
+  if ((s = tc_instances_classify(&b, r->cfg, stream)) != TC_OK) return s;
+  if ((s = tc_clusters_cull(&b, r->cfg, stream)) != TC_OK) return s;
+  return tc_frame_tessellate(&b, r->cfg, 0, stream);
+}
+
+extern "C" {
+
+tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
+  if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
+  return begin_frame(R(rr), frameHost, (hipStream_t)stream, true);
+}
+
+// Captures the per-mesh form of a multi-mesh frame into the renderer's
+// graph: the chain tessellation .. CLAS .. BLAS insertion .. BLAS / TLAS
+// build run once for each of `count` instance records (device memory, 192
+// bytes each, with 24 transform floats each in devXforms) by a renderer
+// created for one instance.  The first chain fetches the frame constants,
+// the others reuse them.  tc_renderer_graph_launch replays it.
+tc_status tc_renderer_graph_capture_meshwise(tc_renderer* rr, const tc_render_instance* devInstances, const float* devXforms,
+                                             uint32_t count, const tc_frame_constants* frameHost, tc_stream stream) {
+  if (!rr || !devInstances || !devXforms || !count || !frameHost || !stream) return TC_ERROR_INVALID_ARGUMENT;
+  Renderer* r = R(rr);
+  if (r->numInstances != 1) return TC_ERROR_INVALID_ARGUMENT;
+  hipStream_t st = (hipStream_t)stream;
+  if (r->graphExec) (void)hipGraphExecDestroy(r->graphExec), r->graphExec = nullptr;
+  if (r->graph) (void)hipGraphDestroy(r->graph), r->graph = nullptr;
+  if (!ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return TC_ERROR_LAUNCH;
+  r->capturing = true;
+  r->lastFrame = *frameHost;
+  tc_status s = TC_OK;
+  for (uint32_t i = 0; i < count && s == TC_OK; i++) {
+    r->instances = const_cast<tc_render_instance*>(devInstances + i);
+    r->xforms = const_cast<float*>(devXforms + 24ull * i);
+    s = tessellate(rr, frameHost, stream, i == 0);
+    if (s == TC_OK) s = tc_renderer_build_accel(rr, stream);
+  }
+  r->instances = r->ownInstances;
+  r->xforms = r->ownXforms;
+  r->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != TC_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return s != TC_OK ? s : TC_ERROR_LAUNCH;
+  }
+  r->graph = g;
+  if (hipGraphInstantiate(&r->graphExec, g, nullptr, nullptr, 0) != hipSuccess) return TC_ERROR_LAUNCH;
+  return TC_OK;
 }
 
 // Everything from instances_classify to blas_clusters_insert
