@@ -723,7 +723,7 @@ tc_status tc_frame_tessellate(const tc_bindings* b, tc_config config, uint32_t n
  * look-back state lives in a library-owned device array of 1 MiB + 1 KiB
  * per device (b->workspace is ignored and may be null), so it calls no
  * allocator.  Limits: ceil(maxVisibleClusters / 64), ceil(maxSplitTriangles
- * / 64) and ceil(maxPartTriangles / 256) must each be at most 16384, else
+ * / 64) and ceil(maxPartTriangles / 1024) must each be at most 16384, else
  * TC_ERROR_CAPACITY.  Calls that overlap on one device race on that array;
  * calls in stream order do not. */
 tc_status tc_frame_tessellate_noalloc(const tc_bindings* b, tc_config config, uint32_t numSplitPasses,

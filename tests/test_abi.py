@@ -137,7 +137,7 @@ def test_noalloc_tessellate_rejects_oversized_configs():
     for f in ("view", "tessTable", "readback", "instances", "build"):
         setattr(b, f, 0x1000)
     b.numRenderInstances = 1
-    for field, value in (("maxPartTriangles", (1 << 22) + 256), ("maxVisibleClusters", (1 << 20) + 64),
+    for field, value in (("maxPartTriangles", (1 << 24) + 1024), ("maxVisibleClusters", (1 << 20) + 64),
                          ("maxSplitTriangles", (1 << 20) + 64)):
         cfg = abi.default_config(**{field: value})
         assert L.tc_frame_tessellate_noalloc(C.byref(b), cfg, 0, None) == abi.TC_ERROR_CAPACITY, field

@@ -523,10 +523,13 @@ static tc_status launch_instantiate(const KArgs& a, tc_config config, int setupM
   launch_prologue(a, setupMode, 2, st);
   uint32_t tiles = (config.maxPartTriangles + kInstTile - 1) / kInstTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().instantiateBlocksPerCU);
+  hipLaunchKernelGGL(k_instantiate, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
+  // vertices: four waves per workgroup, each taking 64 parts at a time
+  const uint32_t vgrid = std::min<uint32_t>((config.maxPartTriangles + kInstTile - 1) / kInstTile, (uint32_t)num_cu() * 8u);
   if (config.flags & TC_FLAG_PN)
-    hipLaunchKernelGGL(k_instantiate<true>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
+    hipLaunchKernelGGL(k_instantiate_vertices<true>, dim3(vgrid), dim3(kInstWaves * kWave), 0, st, a);
   else
-    hipLaunchKernelGGL(k_instantiate<false>, dim3(grid), dim3(kInstWaves * kWave), 0, st, a);
+    hipLaunchKernelGGL(k_instantiate_vertices<false>, dim3(vgrid), dim3(kInstWaves * kWave), 0, st, a);
   return launched();
 }
 

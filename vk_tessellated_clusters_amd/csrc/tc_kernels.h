@@ -78,8 +78,9 @@ __global__ void k_mini_vertices(KArgs a);
 __global__ void k_triangle_split(KArgs a);
 __global__ void k_split_queue_init(KArgs a, uint32_t numPasses, uint32_t waitLimit);
 __global__ void k_triangle_split_persistent(KArgs a);
-template <bool PN>
 __global__ void k_instantiate(KArgs a);
+template <bool PN>
+__global__ void k_instantiate_vertices(KArgs a);
 __global__ void k_blas_setup_insertion(KArgs a);
 __global__ void k_instances_classify(KArgs a);
 __global__ void k_clusters_cull_scan(KArgs a, uint32_t* offsets);

@@ -10,7 +10,7 @@ enum Stage : uint32_t { kStageClassify = 0, kStageSplit = 1, kStageInstantiate =
 constexpr uint32_t kClassifyWaves = 4;   // waves per classify workgroup
 constexpr uint32_t kClassifyTile = 64;   // visible clusters per classify tile
 constexpr uint32_t kInstWaves = 4;       // waves per instantiate workgroup
-constexpr uint32_t kInstTile = kInstWaves * 64;  // part triangles per instantiate tile
+constexpr uint32_t kInstTile = kInstWaves * 4 * 64;  // part triangles per instantiate tile (4 per lane)
 
 __device__ __forceinline__ ViewParams view_params(const KArgs& a) {
   ViewParams vp;
