@@ -74,6 +74,30 @@ template <class T>
 __device__ __forceinline__ T gld(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
+// A record (a multiple of four bytes) through the global address space,
+// word by word; neighbouring words merge into wide loads.
+template <class T>
+__device__ __forceinline__ T gld_record(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  T out;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; i++) o[i] = gld(s + i);
+  return out;
+}
+// The value unchanged, opaque to the optimiser (no arithmetic is derived
+// from it), e.g. to keep neighbouring byte loads from being merged.
+__device__ __forceinline__ uint64_t opaque(uint64_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+// Store through the global address space, for the same reason: a flat
+// store holds the next LDS access until the store has completed.
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
 
 __device__ __forceinline__ v3 ld3(uint64_t base, uint32_t i) {
   const float* f = ptr<const float>(base) + 3ull * i;
@@ -86,9 +110,9 @@ __device__ __forceinline__ v2 ld2(uint64_t base, uint32_t i) {
 }
 __device__ __forceinline__ void st3(uint64_t base, uint64_t i, v3 p) {
   float* f = ptr<float>(base) + 3ull * i;
-  f[0] = p.x;
-  f[1] = p.y;
-  f[2] = p.z;
+  gst(f, p.x);
+  gst(f + 1, p.y);
+  gst(f + 2, p.z);
 }
 
 // --------------------------------------------------- tessellation.glsl
@@ -123,9 +147,10 @@ struct ViewParams {
   float rate;
 };
 
-// tessellation.glsl:78-99, returned as integers
-__device__ __forceinline__ void tess_factors(v3 a, v3 b, v3 c, const ViewParams& vp, uint32_t f[3]) {
-  float dA = distance(a, vp.eye), dB = distance(b, vp.eye), dC = distance(c, vp.eye);
+// tessellation.glsl:78-99, returned as integers; dA, dB, dC are the
+// corners' distances to the eye, distance(a, vp.eye) etc.
+__device__ __forceinline__ void tess_factors(v3 a, v3 b, v3 c, float dA, float dB, float dC, const ViewParams& vp,
+                                             uint32_t f[3]) {
   float sx = 1.0f / fmaxf(vp.nearPlane, fminf(dA, dB));
   float sy = 1.0f / fmaxf(vp.nearPlane, fminf(dB, dC));
   float sz = 1.0f / fmaxf(vp.nearPlane, fminf(dC, dA));
@@ -136,6 +161,9 @@ __device__ __forceinline__ void tess_factors(v3 a, v3 b, v3 c, const ViewParams&
   f[0] = (uint32_t)fminf(fmaxf(fx, 1.0f), 32768.0f);
   f[1] = (uint32_t)fminf(fmaxf(fy, 1.0f), 32768.0f);
   f[2] = (uint32_t)fminf(fmaxf(fz, 1.0f), 32768.0f);
+}
+__device__ __forceinline__ void tess_factors(v3 a, v3 b, v3 c, const ViewParams& vp, uint32_t f[3]) {
+  tess_factors(a, b, c, distance(a, vp.eye), distance(b, vp.eye), distance(c, vp.eye), vp, f);
 }
 
 // tessellation.glsl:101-104
@@ -233,7 +261,10 @@ __device__ __forceinline__ float texel(const tc_displacement_texture& t, uint32_
   return gld(ptr<const float>(t.texels) + i);
 }
 
+// i mod n in [0, n); coordinates already inside the texture (nearly all of
+// them) skip the integer division.
 __device__ __forceinline__ int32_t wrapi(int32_t i, int32_t n) {
+  if ((uint32_t)i < (uint32_t)n) return i;
   int32_t r = i % n;
   return r < 0 ? r + n : r;
 }
@@ -251,8 +282,18 @@ __device__ __forceinline__ float sample_height(const tc_displacement_texture& t,
   int32_t W = (int32_t)t.width, H = (int32_t)t.height;
   uint32_t xa = (uint32_t)wrapi(ix, W), xb = (uint32_t)wrapi(ix + 1, W);
   uint32_t ya = (uint32_t)wrapi(iy, H), yb = (uint32_t)wrapi(iy + 1, H);
-  float t00 = texel(t, xa, ya), t10 = texel(t, xb, ya);
-  float t01 = texel(t, xa, yb), t11 = texel(t, xb, yb);
+  // one format branch around all four fetches, so that they are in flight
+  // together (a branch per texel waits for each fetch in turn)
+  const uint64_t ra = (uint64_t)ya * t.width, rb = (uint64_t)yb * t.width;
+  float t00, t10, t01, t11;
+  if (t.format == TC_TEXTURE_R16_UNORM) {
+    const uint16_t* p = ptr<const uint16_t>(t.texels);
+    const uint16_t a00 = gld(p + ra + xa), a10 = gld(p + ra + xb), a01 = gld(p + rb + xa), a11 = gld(p + rb + xb);
+    t00 = unorm16(a00), t10 = unorm16(a10), t01 = unorm16(a01), t11 = unorm16(a11);
+  } else {
+    const float* p = ptr<const float>(t.texels);
+    t00 = gld(p + ra + xa), t10 = gld(p + ra + xb), t01 = gld(p + rb + xa), t11 = gld(p + rb + xb);
+  }
   float top = t00 * (1.0f - fx) + t10 * fx;
   float bot = t01 * (1.0f - fx) + t11 * fx;
   return top * (1.0f - fy) + bot * fy;

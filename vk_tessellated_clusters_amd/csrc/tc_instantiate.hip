@@ -47,7 +47,7 @@ struct PartBase {
 // part record as the stages read it: a malformed one allocates like the
 // (1,1,1) part of instance 0
 __device__ __forceinline__ tc_tess_triangle_info load_part(const KArgs& a, uint32_t p) {
-  tc_tess_triangle_info ti = ptr<const tc_tess_triangle_info>(a.build->partTriangles)[p];
+  tc_tess_triangle_info ti = gld_record(ptr<const tc_tess_triangle_info>(a.build->partTriangles) + p);
   if (!(ti.cluster.instanceID < a.numInstances && cfg_index(ti.subTriangle.triangleID_config >> 16) < TC_TESSTABLE_NUM_LOOKUP &&
         ti.cluster.clusterID < a.instances[ti.cluster.instanceID].numClusters))
     ti.subTriangle.triangleID_config = 0, ti.cluster.instanceID = 0, ti.cluster.clusterID = 0;
@@ -234,6 +234,9 @@ __global__ __launch_bounds__(256) void k_instantiate_vertices(KArgs a) {
   const tc_scene_building* b = a.build;
   const uint32_t accepted = b->tempInstantiateCounter - T0;
   const tc_template_instantiate_info* infos = ptr<const tc_template_instantiate_info>(b->tempInstantiations) + T0;
+  // loop invariants the vertex loop would otherwise reload after each store
+  const uint64_t genVertices = b->genVertices;
+  const DispGlobal dg = disp_global(a);
   PartBase* parts = s_parts[wave];
   uint32_t* starts = s_start[wave];
   const uint32_t numChunks = (accepted + kVertChunk - 1) / kVertChunk;
@@ -249,12 +252,12 @@ __global__ __launch_bounds__(256) void k_instantiate_vertices(KArgs a) {
       const tc_tess_table_entry e = cfg_entry(T, cf);
       nv = e.numVertices;
       const tc_render_instance& inst = a.instances[ti.cluster.instanceID];
-      const tc_cluster cl = ptr<const tc_cluster>(inst.clusters)[ti.cluster.clusterID];
+      const tc_cluster cl = gld_record(ptr<const tc_cluster>(inst.clusters) + ti.cluster.clusterID);
       const uint8_t* lt = ptr<const uint8_t>(inst.clusterLocalTriangles) + cl.firstLocalTriangle;
       const uint32_t tri = ti.subTriangle.triangleID_config & 0xFFFF;
 #pragma unroll
       for (int v = 0; v < 3; v++) {
-        const uint32_t vi = lt[tri * 3 + v] + cl.firstLocalVertex;
+        const uint32_t vi = gld(lt + tri * 3 + v) + cl.firstLocalVertex;
         pb.vtx[v] = ti.subTriangle.vtxEncoded[v];
         pb.pos[v] = ld3(inst.positions, vi);
         pb.nrm[v] = normalize(ld3(inst.normals, vi));
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void k_instantiate_vertices(KArgs a) {
       }
       pb.cfg = cf;
       pb.firstVertex = e.firstVertex;
-      pb.vertexOffset = (uint32_t)((infos[p].vertexBufferAddress - b->genVertices) / 12ull);
+      pb.vertexOffset = (uint32_t)((gld(&infos[p].vertexBufferAddress) - genVertices) / 12ull);
       pb.dispIndex = has_disp(a, inst.displacementIndex) ? inst.displacementIndex : -1;
       if (pb.dispIndex >= 0) pb.tex = a.textures[pb.dispIndex];
       pb.dispScale = inst.displacementScale;
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(256) void k_instantiate_vertices(KArgs a) {
       if (q.dispIndex >= 0) {
         const v3 n = interp(q.nrm[0], q.nrm[1], q.nrm[2], w);
         const v2 uv = interp2(q.uv[0], q.uv[1], q.uv[2], w);
-        const float hgt = displaced_height(a, q.tex, q.dispScale, q.dispOffset, uv);
+        const float hgt = displaced_height(dg, q.tex, q.dispScale, q.dispOffset, uv);
         if (a.tape)
           write_tape(a, (uint64_t)q.vertexOffset + vert, pos, normalize(n), uv, q.dispScale, q.dispOffset,
                      (uint32_t)q.dispIndex);
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(256) void k_instantiate_vertices(KArgs a) {
         write_tape(a, (uint64_t)q.vertexOffset + vert, pos, mk(0.f, 0.f, 0.f), v2{0.f, 0.f}, 0.f, 0.f, ~0u);
       }
       if (a.cfg.flags & TC_FLAG_ANIMATION) pos = ripple_deform(pos, q.instanceID, q.geoSize, a.view);  // :366-368
-      st3(b->genVertices, (uint64_t)q.vertexOffset + vert, pos);
+      st3(genVertices, (uint64_t)q.vertexOffset + vert, pos);
     }
     // the next chunk reuses this wave's LDS
     __builtin_amdgcn_wave_barrier();

@@ -9,6 +9,7 @@ enum Stage : uint32_t { kStageClassify = 0, kStageSplit = 1, kStageInstantiate =
 
 constexpr uint32_t kClassifyWaves = 4;   // waves per classify workgroup
 constexpr uint32_t kClassifyTile = 64;   // visible clusters per classify tile
+constexpr uint32_t kVertCache = 128;     // vertices per wave a classify cluster caches
 constexpr uint32_t kInstWaves = 4;       // waves per instantiate workgroup
 constexpr uint32_t kInstTile = kInstWaves * 4 * 64;  // part triangles per instantiate tile (4 per lane)
 
@@ -25,11 +26,25 @@ __device__ __forceinline__ bool has_disp(const KArgs& a, int32_t idx) {
   return (a.cfg.flags & TC_FLAG_DISPLACEMENT) && idx >= 0 && (uint32_t)idx < a.numTextures;
 }
 
+// The frame's displacement scale and offset, read once per kernel: vertex
+// loops store through pointers the compiler cannot tell apart from the
+// frame constants, so a read inside the loop is repeated every iteration.
+struct DispGlobal {
+  float scale, offset;
+};
+__device__ __forceinline__ DispGlobal disp_global(const KArgs& a) {
+  return DispGlobal{a.view->displacementScale, a.view->displacementOffset};
+}
+
 // cluster_classify.comp.glsl:477-479
-__device__ __forceinline__ float displaced_height(const KArgs& a, const tc_displacement_texture& tex, float scale,
+__device__ __forceinline__ float displaced_height(const DispGlobal& g, const tc_displacement_texture& tex, float scale,
                                                   float offset, v2 uv) {
   float h = sample_height(tex, uv.x, uv.y);
-  return (((h * scale) * a.view->displacementScale) + offset) + a.view->displacementOffset;
+  return (((h * scale) * g.scale) + offset) + g.offset;
+}
+__device__ __forceinline__ float displaced_height(const KArgs& a, const tc_displacement_texture& tex, float scale,
+                                                  float offset, v2 uv) {
+  return displaced_height(disp_global(a), tex, scale, offset, uv);
 }
 
 // Records how generated vertex `index` was displaced (tc_vertex_tape).
@@ -55,14 +70,20 @@ __device__ __forceinline__ uint64_t max_data_bytes(const tc_config& c) {
   return (uint64_t)c.maxGeneratedClusterMegs * 1024ull * 1024ull;
 }
 
-// last index j of the sorted s_start[0..n) with s_start[j] <= t (n a power of two)
+// Last index j of the non-decreasing s_start[0..N) with s_start[j] <= t (0
+// when there is none), as a count in two rounds of independent LDS reads:
+// the group of eight holding j from every eighth entry, then j within it.
 template <uint32_t N>
 __device__ __forceinline__ uint32_t find_owner_n(const uint32_t* s_start, uint32_t t) {
-  uint32_t lo = 0;
+  static_assert(N % 8 == 0 && N >= 16, "groups of eight");
+  uint32_t g = 0;
 #pragma unroll
-  for (uint32_t step = N / 2; step > 0; step >>= 1)
-    if (s_start[lo + step] <= t) lo += step;
-  return lo;
+  for (uint32_t j = 1; j < N / 8; j++) g += s_start[8 * j] <= t ? 1u : 0u;
+  const uint32_t* s = s_start + 8 * g;
+  uint32_t k = 0;
+#pragma unroll
+  for (uint32_t i = 1; i < 8; i++) k += s[i] <= t ? 1u : 0u;
+  return 8 * g + k;
 }
 __device__ __forceinline__ uint32_t find_owner(const uint32_t* s_start, uint32_t t) { return find_owner_n<64>(s_start, t); }
 
