@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t finish_block(char* dst, const float* s_v, co
       lo[c] = fminf(lo[c], glo[c]);
       hi[c] = fmaxf(hi[c], ghi[c]);
     }
-    groups[g] = box;
+    gst_record(groups + g, box);
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -76,7 +76,7 @@ __device__ __forceinline__ uint32_t finish_block(char* dst, const float* s_v, co
     for (uint32_t k = lane; k < npad; k += kWave) {
       const uint32_t a = (g1 - g0) / 4, b = a + (t1 - t0) / 4;
       const uint32_t byte = k < a ? g0 + 4 * k : (k < b ? t0 + 4 * (k - a) : v0 + 4 * (k - b));
-      w[byte / 4] = 0u;
+      gst(w + byte / 4, 0u);
     }
   }
   if (lane == 0) {
@@ -87,7 +87,7 @@ __device__ __forceinline__ uint32_t finish_block(char* dst, const float* s_v, co
     h.counts = nt | (nv << 8) | (ng << 16);
     h.vertexBase = vertexBase;
     h._pad[0] = h._pad[1] = h._pad[2] = 0;
-    *reinterpret_cast<tc_clas_header*>(dst) = h;
+    gst_record(reinterpret_cast<tc_clas_header*>(dst), h);
   }
   return (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
 }
@@ -105,6 +105,10 @@ constexpr uint32_t kSub = 16;
 constexpr uint32_t kSubTris = 128;
 constexpr uint32_t kSubVerts = 128;
 constexpr uint32_t kSubsPerWave = kWave / kSub;
+constexpr uint32_t kWaveFloatsPerLane = (kMaxClusterVerts * 3 + kWave - 1) / kWave;
+constexpr uint32_t kWaveTrisPerLane = (kMaxClusterTris + kWave - 1) / kWave;
+constexpr uint32_t kSubFloatsPerLane = (kSubVerts * 3 + kSub - 1) / kSub;
+constexpr uint32_t kSubTrisPerLane = (kSubTris + kSub - 1) / kSub;
 constexpr uint32_t kClasWaves = 4;
 
 __device__ __forceinline__ float sub_min(float v) {
@@ -147,36 +151,57 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
   tc_template_instantiate_info info1{}, info2{};
   tc_clas_template T1{};
   if (base + sub < n) {
-    info1 = infos[base + sub];
-    T1 = *ptr<const tc_clas_template>(info1.clusterTemplateAddress);
+    info1 = gld_record(infos + base + sub);
+    T1 = gld_record(ptr<const tc_clas_template>(info1.clusterTemplateAddress));
   }
-  if (base + stride + sub < n) info2 = infos[base + stride + sub];
+  if (base + stride + sub < n) info2 = gld_record(infos + base + stride + sub);
   for (; base < n; base += stride) {
     const uint32_t i = base + sub;
     const bool valid = i < n;
     const tc_template_instantiate_info info = info1;
     const tc_clas_template T = T1;
     info1 = info2;
-    if (i + stride < n) T1 = *ptr<const tc_clas_template>(info1.clusterTemplateAddress);
-    if (i + 2 * stride < n) info2 = infos[i + 2 * stride];
+    if (i + stride < n) T1 = gld_record(ptr<const tc_clas_template>(info1.clusterTemplateAddress));
+    if (i + 2 * stride < n) info2 = gld_record(infos + i + 2 * stride);
     const bool small = valid && T.numTriangles <= kSubTris && T.numVertices <= kSubVerts;
     const uint32_t nt = small ? T.numTriangles : 0u, nv = small ? T.numVertices : 0u;
     const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
-    char* dst = small ? ptr<char>(dsts[i]) : nullptr;
+    char* dst = small ? ptr<char>(gld(dsts + i)) : nullptr;
     const char* src = small ? ptr<const char>(info.vertexBufferAddress) : nullptr;
     if (small) {
       uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
       float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
-      for (uint32_t k = sl; k < nv * 3; k += kSub) {
-        const uint32_t v = k / 3, c = k - v * 3;
-        const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
-        s_sv[k] = f;
-        verts[k] = f;
+      // every load of the block is issued before the first is used, so the
+      // copy waits for one round trip rather than one per element; packed
+      // vertices (stride 12, what the instantiation records carry) are read
+      // from one base, other strides element by element
+      const uint32_t nf = nv * 3;
+      if (info.vertexBufferStride == 12u) {
+        const float* vs = reinterpret_cast<const float*>(src) + sl;
+        float vf[kSubFloatsPerLane];
+#pragma unroll
+        for (uint32_t j = 0; j < kSubFloatsPerLane; j++) vf[j] = sl + j * kSub < nf ? gld(vs + j * kSub) : 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < kSubFloatsPerLane; j++) {
+          const uint32_t k = sl + j * kSub;
+          if (k < nf) s_sv[k] = vf[j], gst(verts + k, vf[j]);
+        }
+      } else {
+        for (uint32_t k = sl; k < nf; k += kSub) {
+          const uint32_t v = k / 3, c = k - v * 3;
+          const float f = gld(reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride) + c);
+          s_sv[k] = f;
+          gst(verts + k, f);
+        }
       }
-      for (uint32_t t = sl; t < nt; t += kSub) {
-        const uint32_t w = ptr<const uint32_t>(T.triangles)[t];
-        s_st[t] = w;
-        tris[t] = w;
+      const uint32_t* ts = ptr<const uint32_t>(T.triangles) + sl;
+      uint32_t tw[kSubTrisPerLane];
+#pragma unroll
+      for (uint32_t j = 0; j < kSubTrisPerLane; j++) tw[j] = sl + j * kSub < nt ? gld(ts + j * kSub) : 0u;
+#pragma unroll
+      for (uint32_t j = 0; j < kSubTrisPerLane; j++) {
+        const uint32_t t = sl + j * kSub;
+        if (t < nt) s_st[t] = tw[j], gst(tris + t, tw[j]);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -198,7 +223,7 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
       tc_aabb box;
 #pragma unroll
       for (int c = 0; c < 3; c++) box.lo[c] = glo[c], box.hi[c] = ghi[c], lo[c] = fminf(lo[c], glo[c]), hi[c] = fmaxf(hi[c], ghi[c]);
-      reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header))[g] = box;
+      gst_record(reinterpret_cast<tc_aabb*>(dst + sizeof(tc_clas_header)) + g, box);
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) lo[c] = sub_min(lo[c]), hi[c] = sub_max(hi[c]);
@@ -208,7 +233,8 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
       const uint32_t g0 = (uint32_t)sizeof(tc_clas_header) + ng * 24u, g1 = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u);
       const uint32_t t0 = g1 + nt * 4u, t1 = g1 + r16(nt * 4u), v0 = t1 + nv * 12u, v1 = t1 + r16(nv * 12u);
       const uint32_t na = (g1 - g0) / 4, nb = na + (t1 - t0) / 4, nc = nb + (v1 - v0) / 4;
-      for (uint32_t q = sl; q < nc; q += kSub) w32[(q < na ? g0 + 4 * q : (q < nb ? t0 + 4 * (q - na) : v0 + 4 * (q - nb))) / 4] = 0u;
+      for (uint32_t q = sl; q < nc; q += kSub)
+        gst(w32 + (q < na ? g0 + 4 * q : (q < nb ? t0 + 4 * (q - na) : v0 + 4 * (q - nb))) / 4, 0u);
       if (sl == 0) {
         tc_clas_header h;
         h.lo[0] = lo[0], h.lo[1] = lo[1], h.lo[2] = lo[2];
@@ -217,8 +243,8 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
         h.counts = nt | (nv << 8) | (ng << 16);
         h.vertexBase = vertex_base(b, info.vertexBufferAddress);
         h._pad[0] = h._pad[1] = h._pad[2] = 0;
-        *reinterpret_cast<tc_clas_header*>(dst) = h;
-        sizes[i] = (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u);
+        gst_record(reinterpret_cast<tc_clas_header*>(dst), h);
+        gst(sizes + i, (uint32_t)sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u) + r16(nv * 12u));
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -228,30 +254,30 @@ __global__ __launch_bounds__(kClasWaves * kWave) void k_clas_instantiate(KArgs a
     while (big) {
       const uint32_t ij = base + (uint32_t)__builtin_ctzll(big) / kSub;
       big &= big - 1;
-      const tc_template_instantiate_info bi = infos[ij];
-      const tc_clas_template BT = *ptr<const tc_clas_template>(bi.clusterTemplateAddress);
+      const tc_template_instantiate_info bi = gld_record(infos + ij);
+      const tc_clas_template BT = gld_record(ptr<const tc_clas_template>(bi.clusterTemplateAddress));
       const uint32_t bnt = BT.numTriangles, bnv = BT.numVertices, bng = BT.numGroups;
-      char* bdst = ptr<char>(dsts[ij]);
+      char* bdst = ptr<char>(gld(dsts + ij));
       uint32_t* tris = reinterpret_cast<uint32_t*>(bdst + sizeof(tc_clas_header) + r16(bng * 24u));
       float* verts = reinterpret_cast<float*>(bdst + sizeof(tc_clas_header) + r16(bng * 24u) + r16(bnt * 4u));
       const char* bsrc = ptr<const char>(bi.vertexBufferAddress);
       for (uint32_t k = lane; k < bnv * 3; k += kWave) {
         const uint32_t v = k / 3, c = k - v * 3;
-        const float f = reinterpret_cast<const float*>(bsrc + (uint64_t)v * bi.vertexBufferStride)[c];
+        const float f = gld(reinterpret_cast<const float*>(bsrc + (uint64_t)v * bi.vertexBufferStride) + c);
         s_v[k] = f;
-        verts[k] = f;
+        gst(verts + k, f);
       }
       for (uint32_t t = lane; t < bnt; t += kWave) {
-        const uint32_t w = ptr<const uint32_t>(BT.triangles)[t];
+        const uint32_t w = gld(ptr<const uint32_t>(BT.triangles) + t);
         s_t[t] = w;
-        tris[t] = w;
+        gst(tris + t, w);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t size = finish_block(bdst, s_v, s_t, bnt, bnv, BT.clusterID + bi.clusterIdOffset,
                                          vertex_base(b, bi.vertexBufferAddress));
-      if (lane == 0) sizes[ij] = size;
+      if (lane == 0) gst(sizes + ij, size);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
@@ -272,30 +298,57 @@ __global__ __launch_bounds__(64) void k_clas_build_transient(KArgs a) {
   const uint64_t* dsts = ptr<const uint64_t>(b->transClusterAddresses);
   uint32_t* sizes = ptr<uint32_t>(b->transClusterSizes);
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const tc_clas_build_info info = infos[i];
+    const tc_clas_build_info info = gld_record(infos + i);
     const uint32_t nt = min(info.packed & 511u, kMaxClusterTris), nv = min((info.packed >> 9) & 511u, kMaxClusterVerts);
     const uint32_t ng = (nt + TC_CLAS_GROUP_TRIANGLES - 1) / TC_CLAS_GROUP_TRIANGLES;
-    char* dst = ptr<char>(dsts[i]);
+    char* dst = ptr<char>(gld(dsts + i));
     uint32_t* tris = reinterpret_cast<uint32_t*>(dst + sizeof(tc_clas_header) + r16(ng * 24u));
     float* verts = reinterpret_cast<float*>(dst + sizeof(tc_clas_header) + r16(ng * 24u) + r16(nt * 4u));
     const char* src = ptr<const char>(info.vertexBuffer);
     const uint8_t* ix = ptr<const uint8_t>(info.indexBuffer);
     const uint32_t is = info.indexBufferStride;
-    for (uint32_t k = lane; k < nv * 3; k += kWave) {
-      const uint32_t v = k / 3, c = k - v * 3;
-      const float f = reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride)[c];
-      s_v[k] = f;
-      verts[k] = f;
+    // the loads of a record are issued together (packed vertices from one
+    // base), so the copy waits for one round trip rather than one per element
+    const uint32_t nf = nv * 3;
+    if (info.vertexBufferStride == 12u) {
+      const float* vs = reinterpret_cast<const float*>(src) + lane;
+      float vf[kWaveFloatsPerLane];
+#pragma unroll
+      for (uint32_t j = 0; j < kWaveFloatsPerLane; j++) vf[j] = lane + j * kWave < nf ? gld(vs + j * kWave) : 0.0f;
+#pragma unroll
+      for (uint32_t j = 0; j < kWaveFloatsPerLane; j++) {
+        const uint32_t k = lane + j * kWave;
+        if (k < nf) s_v[k] = vf[j], gst(verts + k, vf[j]);
+      }
+    } else {
+      for (uint32_t k = lane; k < nf; k += kWave) {
+        const uint32_t v = k / 3, c = k - v * 3;
+        const float f = gld(reinterpret_cast<const float*>(src + (uint64_t)v * info.vertexBufferStride) + c);
+        s_v[k] = f;
+        gst(verts + k, f);
+      }
     }
-    for (uint32_t t = lane; t < nt; t += kWave) {
-      const uint8_t* q = ix + (uint64_t)t * 3u * is;
-      const uint32_t w = (uint32_t)q[0] | ((uint32_t)q[is] << 8) | ((uint32_t)q[2u * is] << 16) | (t << 24);
-      s_t[t] = w;
-      tris[t] = w;
+    uint32_t tb[kWaveTrisPerLane][3];
+#pragma unroll
+    for (uint32_t j = 0; j < kWaveTrisPerLane; j++) {
+      const uint32_t t = lane + j * kWave;
+      const uint8_t* q = ix + (uint64_t)(t < nt ? t : 0u) * 3u * is;
+      tb[j][0] = t < nt ? (uint32_t)gld(q) : 0u;
+      tb[j][1] = t < nt ? (uint32_t)gld(q + is) : 0u;
+      tb[j][2] = t < nt ? (uint32_t)gld(q + 2u * is) : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kWaveTrisPerLane; j++) {
+      const uint32_t t = lane + j * kWave;
+      if (t < nt) {
+        const uint32_t w = tb[j][0] | (tb[j][1] << 8) | (tb[j][2] << 16) | (t << 24);
+        s_t[t] = w;
+        gst(tris + t, w);
+      }
     }
     __syncthreads();
     const uint32_t size = finish_block(dst, s_v, s_t, nt, nv, info.clusterID, vertex_base(b, info.vertexBuffer));
-    if (lane == 0) sizes[i] = size;
+    if (lane == 0) gst(sizes + i, size);
     __syncthreads();
   }
 }

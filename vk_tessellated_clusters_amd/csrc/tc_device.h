@@ -99,6 +99,16 @@ __device__ __forceinline__ void gst(T* p, T v) {
   *(__attribute__((address_space(1))) T*)p = v;
 }
 
+// A record through the global address space, word by word.
+template <class T>
+__device__ __forceinline__ void gst_record(T* p, const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  uint32_t* d = reinterpret_cast<uint32_t*>(p);
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(&v);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; i++) gst(d + i, s[i]);
+}
+
 __device__ __forceinline__ v3 ld3(uint64_t base, uint32_t i) {
   const float* f = ptr<const float>(base) + 3ull * i;
   return mk(gld(f), gld(f + 1), gld(f + 2));
