@@ -13,6 +13,7 @@
 //   refit   bottom-up with one arrival counter per internal node, in LDS
 //           for nodes whose key range lies inside one workgroup's chunk of
 //           sorted positions, in global memory for the few that straddle;
+//           a node's four-wide record is written with it;
 //   TLAS    the same builder over the instances' world-space boxes, with a
 //           30-bit Morton code (morton_of) per instance.
 #include <algorithm>
@@ -171,21 +172,86 @@ __global__ void k_blas_hierarchy(BvhArgs a) {
     a.pendingCount[i] = 0;
 }
 
-// Box of sorted leaf k from its CLAS header.  Every leaf's box is read once,
-// by the refit of its parent, which also writes the leaf's info word
-// (counts | clusterID << 32) from the same header.
+// Box of sorted leaf k from its CLAS header; also writes the leaf's info
+// word (counts | clusterID << 32) from the same header.  The refit stages
+// every leaf once; the leaves of straddling nodes are read again.
 __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[3], float hi[3]) {
-  const tc_clas_header h = *ptr<const tc_clas_header>(a.leafAddr[k]);
+  const tc_clas_header h = gld_record(ptr<const tc_clas_header>(a.leafAddr[k]));
   for (int c = 0; c < 3; c++) lo[c] = h.lo[c], hi[c] = h.hi[c];
   a.leafInfo[k] = (uint64_t)h.counts | (uint64_t)h.clusterID << 32;
 }
 
+// One slot of a four-wide record: a box and its child word.
+struct Slot4 {
+  float lo[3], hi[3];
+  uint32_t c;
+};
+__device__ __forceinline__ Slot4 slot4(const float* lo, const float* hi, uint32_t c) {
+  Slot4 r;
+  for (int q = 0; q < 3; q++) r.lo[q] = lo[q], r.hi[q] = hi[q];
+  r.c = c;
+  return r;
+}
+
+// Four-wide record of a binary node: its children in order, each internal
+// child replaced by that child's two children, empty slots last.  Child 0
+// gives a0 (and a1 when internal), child 1 gives b0 (and b1).  The slots
+// are chosen by selects, so the record stays in registers.
+__device__ __forceinline__ Slot4 sel4(bool c, const Slot4& x, const Slot4& y) {
+  Slot4 r;
+  for (int q = 0; q < 3; q++) r.lo[q] = c ? x.lo[q] : y.lo[q], r.hi[q] = c ? x.hi[q] : y.hi[q];
+  r.c = c ? x.c : y.c;
+  return r;
+}
+__device__ __forceinline__ tc_bvh4_node make4(bool leaf0, const Slot4& a0, const Slot4& a1, bool leaf1, const Slot4& b0,
+                                              const Slot4& b1) {
+  Slot4 e;
+  for (int q = 0; q < 3; q++) e.lo[q] = INFINITY, e.hi[q] = -INFINITY;
+  e.c = TC_BVH4_EMPTY;
+  const Slot4 s[4] = {a0, sel4(leaf0, b0, a1), sel4(leaf0, sel4(leaf1, e, b1), b0), sel4(leaf0 || leaf1, e, b1)};
+  tc_bvh4_node o;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    for (int q = 0; q < 3; q++) o.lo[q][k] = s[k].lo[q], o.hi[q][k] = s[k].hi[q];
+    o.child[k] = s[k].c;
+  }
+  for (int q = 0; q < 4; q++) o._pad[q] = 0;
+  return o;
+}
+
+// Four-wide record of binary node n of a tree whose node array starts at
+// `nodes` (child indices relative to it; its internal children are final).
+__device__ __forceinline__ tc_bvh4_node collapse4_of(const tc_bvh_node& n, const tc_bvh_node* nodes) {
+  Slot4 g[2][2];
+  bool leaf[2];
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = c ? n.child1 : n.child0;
+    leaf[c] = (ch & TC_BVH_LEAF) != 0;
+    if (leaf[c]) {
+      g[c][0] = g[c][1] = slot4(c ? n.lo1 : n.lo0, c ? n.hi1 : n.hi0, ch);
+    } else {
+      const tc_bvh_node m = gld_record(nodes + ch);
+      g[c][0] = slot4(m.lo0, m.hi0, m.child0);
+      g[c][1] = slot4(m.lo1, m.hi1, m.child1);
+    }
+  }
+  return make4(leaf[0], g[0][0], g[0][1], leaf[1], g[1][0], g[1][1]);
+}
+__device__ __forceinline__ void collapse4(const tc_bvh_node* nodes, uint32_t i, tc_bvh4_node* out) {
+  gst_record(out + i, collapse4_of(gld_record(nodes + i), nodes));
+}
+
 // Bottom-up refit in two launches, with no device-scope fence.
 //
-// k_blas_refit: a workgroup owns a chunk of kRefitChunk sorted positions.
-// Every leaf climbs; the second arrival at a node writes it.  Nodes whose
-// key range lies inside the chunk keep their arrival counter and box in
-// LDS (workgroup-scope ordering).  A climb stops at the first node whose
+// k_blas_refit: a workgroup owns a chunk of kRefitChunk sorted positions,
+// one thread per position.  It first stages the chunk's topology (child
+// words, parents, range ends of the internal nodes, the leaves' parents)
+// and its leaves' boxes (from their CLAS headers, which also give the leaf
+// info words) in LDS, all loads in parallel.  Then every leaf climbs; the
+// second arrival at a node writes it, and its four-wide record.  Nodes
+// whose key range lies inside the chunk are read and counted in LDS
+// (workgroup-scope ordering), so a climb waits on no global load.  A climb
+// stops at the first node whose
 // range straddles chunks; the second arrival there lists the node as
 // pending for its instance (both children are then final).
 //
@@ -198,74 +264,140 @@ __device__ __forceinline__ void leaf_box(const BvhArgs& a, uint32_t k, float lo[
 // (first >> kRefitChunkBits) + instance.
 constexpr uint32_t kRefitChunkBits = 10;
 constexpr uint32_t kRefitChunk = 1u << kRefitChunkBits;
-constexpr uint32_t kRefitThreads = 256;
+constexpr uint32_t kRefitThreads = kRefitChunk;
+constexpr uint32_t kRefitTopThreads = 256;
 
-__device__ __forceinline__ void refit_node(const BvhArgs& a, uint32_t node, int64_t first, const tc_aabb* localBoxes,
-                                           uint32_t c0, tc_aabb& self) {
+// Writes internal node `node` from its children's boxes (both final) and
+// returns its union box in `self`.
+__device__ __forceinline__ tc_bvh_node write_node(const BvhArgs& a, uint32_t node, uint32_t c0w, uint32_t c1w,
+                                                  const float lo[2][3], const float hi[2][3], tc_aabb& self) {
   tc_bvh_node out;
-  float lo[2][3], hi[2][3];
-  for (int c = 0; c < 2; c++) {
-    const uint32_t ch = a.child[2 * node + c];
-    if (ch & TC_BVH_LEAF) {
-      leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
-    } else {
-      // a local node's children are local to the same chunk
-      const uint32_t g = (uint32_t)first + ch;
-      const tc_aabb bx = localBoxes ? localBoxes[g - c0] : a.nodeBox[g];
-      for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
-    }
-  }
   for (int q = 0; q < 3; q++) {
     out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
   }
-  out.child0 = a.child[2 * node + 0];
-  out.child1 = a.child[2 * node + 1];
+  out.child0 = c0w;
+  out.child1 = c1w;
   out._pad0 = out._pad1 = 0;
-  a.nodes[node] = out;
+  gst_record(a.nodes + node, out);
   for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
-  a.nodeBox[node] = self;  // read by straddling parents and the TLAS
+  gst_record(a.nodeBox + node, self);  // read by straddling parents and the TLAS
+  return out;
+}
+
+// refit of a straddling node: children from global memory
+__device__ __forceinline__ tc_bvh_node refit_node(const BvhArgs& a, uint32_t node, int64_t first, tc_aabb& self) {
+  float lo[2][3], hi[2][3];
+  const uint32_t cw[2] = {a.child[2 * node + 0], a.child[2 * node + 1]};
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = cw[c];
+    if (ch & TC_BVH_LEAF) {
+      leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
+    } else {
+      const tc_aabb bx = a.nodeBox[(uint32_t)first + ch];
+      for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+    }
+  }
+  return write_node(a, node, cw[0], cw[1], lo, hi, self);
 }
 
 __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
   __shared__ uint32_t s_cnt[kRefitChunk];
-  __shared__ tc_aabb s_box[kRefitChunk];
+  __shared__ tc_aabb s_box[kRefitChunk];   // internal nodes' union boxes
+  __shared__ tc_aabb s_leaf[kRefitChunk];  // leaves' boxes
+  __shared__ uint32_t s_child[kRefitChunk][2];
+  __shared__ uint32_t s_parent[kRefitChunk], s_leafParent[kRefitChunk], s_range[kRefitChunk];
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->blasClusterCounter, a.capacity);
+  const uint32_t j = threadIdx.x;
   for (uint32_t c0 = blockIdx.x * kRefitChunk; c0 < n; c0 += gridDim.x * kRefitChunk) {
     const uint32_t c1 = min(c0 + kRefitChunk, n);
-    for (uint32_t i = threadIdx.x; i < kRefitChunk; i += kRefitThreads) s_cnt[i] = 0;
+    const uint32_t k = c0 + j;
+    int64_t first = 0, last = 0;
+    s_cnt[j] = 0;
+    if (k < c1) {
+      blas_segment(a, (uint32_t)(a.sortedKeys[k] >> 32), n, first, last);
+      s_leafParent[j] = a.parent[a.capacity + k];
+      // internal node k exists when k < last; the other entries are unread
+      s_parent[j] = a.parent[k];
+      s_range[j] = a.rangeEnd[k];
+      s_child[j][0] = a.child[2 * k + 0];
+      s_child[j][1] = a.child[2 * k + 1];
+      float lo[3], hi[3];
+      leaf_box(a, k, lo, hi);
+      for (int q = 0; q < 3; q++) s_leaf[j].lo[q] = lo[q], s_leaf[j].hi[q] = hi[q];
+    }
     __syncthreads();
-    for (uint32_t k = c0 + threadIdx.x; k < c1; k += kRefitThreads) {
-      const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
-      int64_t first, last;
-      blas_segment(a, inst, n, first, last);
-      if (first == last) continue;  // single CLAS: the root is the leaf
-      uint32_t node = a.parent[a.capacity + k];
+    if (k < c1 && first != last) {  // a single CLAS is its instance's root
+      uint32_t node = s_leafParent[j];
       while (true) {
-        const uint32_t far = a.rangeEnd[node];
-        if (!(node >= c0 && node < c1 && far >= c0 && far < c1)) {
+        const bool local = node >= c0 && node < c1 && s_range[node - c0] >= c0 && s_range[node - c0] < c1;
+        if (!local) {
           // straddling: the second arrival lists it for the second launch
           if (atomicAdd(&a.counter[node], 1u) == 1u) {
+            const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
             const uint32_t slot = atomicAdd(&a.pendingCount[inst], 1u);
             a.pendingList[((uint32_t)first >> kRefitChunkBits) + inst + slot] = node;
           }
           break;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (atomicAdd(&s_cnt[node - c0], 1u) == 0) break;  // first arrival stops
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const uint32_t ln = node - c0;
+        // the arrival orders LDS only (s_box, s_cnt): no wait for the node
+        // stores, which no climber of this chunk reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        if (atomicAdd(&s_cnt[ln], 1u) == 0) break;  // first arrival stops
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        // a local node's children (leaves or internal nodes) lie in the chunk
+        float lo[2][3], hi[2][3];
+        const uint32_t cw[2] = {s_child[ln][0], s_child[ln][1]};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const uint32_t g = (uint32_t)first + (cw[c] & ~TC_BVH_LEAF) - c0;
+          const tc_aabb& bx = (cw[c] & TC_BVH_LEAF) ? s_leaf[g] : s_box[g];
+          for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+        }
         tc_aabb self;
-        refit_node(a, node, first, s_box, c0, self);
-        s_box[node - c0] = self;
+        write_node(a, node, cw[0], cw[1], lo, hi, self);
+        s_box[ln] = self;
         if ((int64_t)node == first) break;  // instance root
-        node = a.parent[node];
+        node = s_parent[ln];
       }
+    }
+    __syncthreads();
+    // four-wide records of the chunk's local internal nodes, one per
+    // thread, from LDS (the grandchildren of a local node lie in the chunk)
+    if (k < c1 && (int64_t)k < last && s_range[j] >= c0 && s_range[j] < c1) {
+      const uint32_t cw[2] = {s_child[j][0], s_child[j][1]};
+      Slot4 gs[2][2];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        if (cw[c] & TC_BVH_LEAF) {
+          const tc_aabb& bx = s_leaf[(uint32_t)first + (cw[c] & ~TC_BVH_LEAF) - c0];
+          gs[c][0] = gs[c][1] = slot4(bx.lo, bx.hi, cw[c]);
+        } else {
+          const uint32_t g = (uint32_t)first + cw[c] - c0;
+#pragma unroll
+          for (int d = 0; d < 2; d++) {
+            const uint32_t gw = s_child[g][d];
+            const uint32_t h = (uint32_t)first + (gw & ~TC_BVH_LEAF) - c0;
+            const tc_aabb& bx = (gw & TC_BVH_LEAF) ? s_leaf[h] : s_box[h];
+            gs[c][d] = slot4(bx.lo, bx.hi, gw);
+          }
+        }
+      }
+      gst_record(a.nodes4 + k, make4((cw[0] & TC_BVH_LEAF) != 0, gs[0][0], gs[0][1], (cw[1] & TC_BVH_LEAF) != 0,
+                                     gs[1][0], gs[1][1]));
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kRefitThreads) void k_blas_refit_top(BvhArgs a) {
+// Nodes a refit_top workgroup lists for its four-wide pass; further nodes
+// are collapsed where they are refit.
+constexpr uint32_t kRefitTopList = 4096;
+
+__global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) {
+  __shared__ uint32_t s_list[kRefitTopList];
+  __shared__ uint32_t s_listed;
   const uint32_t inst = blockIdx.x;
   if (inst >= a.numInstances) return;
   const tc_scene_building* b = a.build;
@@ -276,11 +408,18 @@ __global__ __launch_bounds__(kRefitThreads) void k_blas_refit_top(BvhArgs a) {
   if (first >= (int64_t)n) return;
   const uint32_t count = a.pendingCount[inst];
   const uint32_t base = ((uint32_t)first >> kRefitChunkBits) + inst;
-  for (uint32_t t = threadIdx.x; t < count; t += kRefitThreads) {
+  if (threadIdx.x == 0) s_listed = 0;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < count; t += kRefitTopThreads) {
     uint32_t node = a.pendingList[base + t];
     while (true) {
       tc_aabb self;
-      refit_node(a, node, first, nullptr, 0, self);
+      const tc_bvh_node rec = refit_node(a, node, first, self);
+      const uint32_t slot = atomicAdd(&s_listed, 1u);
+      if (slot < kRefitTopList)
+        s_list[slot] = node;
+      else
+        gst_record(a.nodes4 + node, collapse4_of(rec, a.nodes + first));
       if ((int64_t)node == first) break;
       node = a.parent[node];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -288,46 +427,12 @@ __global__ __launch_bounds__(kRefitThreads) void k_blas_refit_top(BvhArgs a) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
   }
-}
-
-// Four-wide record of internal node i of a tree whose node array starts at
-// `nodes` (child indices relative to it).
-__device__ __forceinline__ void collapse4(const tc_bvh_node* nodes, uint32_t i, tc_bvh4_node* out) {
-  const tc_bvh_node n = nodes[i];
-  tc_bvh4_node o;
-  int s = 0;
-  auto put = [&](const float* lo, const float* hi, uint32_t c) {
-    for (int q = 0; q < 3; q++) o.lo[q][s] = lo[q], o.hi[q][s] = hi[q];
-    o.child[s] = c;
-    s++;
-  };
-  for (int c = 0; c < 2; c++) {
-    const uint32_t ch = c ? n.child1 : n.child0;
-    if (ch & TC_BVH_LEAF) {
-      put(c ? n.lo1 : n.lo0, c ? n.hi1 : n.hi0, ch);
-    } else {
-      const tc_bvh_node g = nodes[ch];
-      put(g.lo0, g.hi0, g.child0);
-      put(g.lo1, g.hi1, g.child1);
-    }
-  }
-  for (; s < 4; s++) {
-    for (int q = 0; q < 3; q++) o.lo[q][s] = INFINITY, o.hi[q][s] = -INFINITY;
-    o.child[s] = TC_BVH4_EMPTY;
-  }
-  for (int q = 0; q < 4; q++) o._pad[q] = 0;
-  out[i] = o;
-}
-
-// One thread per internal BLAS node (after the refit).
-__global__ void k_blas_collapse4(BvhArgs a) {
-  const uint32_t n = min(a.build->blasClusterCounter, a.capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t inst = (uint32_t)(a.sortedKeys[k] >> 32);
-    int64_t first, last;
-    blas_segment(a, inst, n, first, last);
-    if ((int64_t)k < last) collapse4(a.nodes + first, (uint32_t)(k - first), a.nodes4 + first);
-  }
+  // four-wide records of the listed nodes, once every climb is done (their
+  // children's records are then final), one per thread
+  __syncthreads();
+  const uint32_t listed = min(s_listed, kRefitTopList);
+  for (uint32_t t = threadIdx.x; t < listed; t += kRefitTopThreads)
+    collapse4(a.nodes + first, s_list[t] - (uint32_t)first, a.nodes4 + first);
 }
 
 // TLAS record and world box of instance i.
@@ -684,8 +789,7 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   hipLaunchKernelGGL(k_blas_hierarchy, dim3(grid), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_blas_refit, dim3(std::min<uint32_t>((capacity + kRefitChunk - 1) / kRefitChunk, (uint32_t)num_cu_public() * 4)),
                      dim3(kRefitThreads), 0, st, a);
-  hipLaunchKernelGGL(k_blas_refit_top, dim3(a.numInstances), dim3(kRefitThreads), 0, st, a);
-  hipLaunchKernelGGL(k_blas_collapse4, dim3(grid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_blas_refit_top, dim3(a.numInstances), dim3(kRefitTopThreads), 0, st, a);
   // TLAS over instances, with key and value arrays of its own
   a.tlasKeys = reinterpret_cast<uint64_t*>(m + L.tlasKeys);
   a.tlasVals = reinterpret_cast<uint32_t*>(m + L.tlasVals);
