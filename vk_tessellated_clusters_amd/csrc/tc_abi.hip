@@ -479,13 +479,16 @@ static tc_status launch_classify(const KArgs& a, tc_config config, int setupMode
   uint32_t tiles = (config.maxVisibleClusters + kClassifyTile - 1) / kClassifyTile;
   uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)num_cu() * tuning_snapshot().classifyBlocksPerCU);
   if (config.flags & TC_FLAG_2X_TRANSIENT) {
-    hipLaunchKernelGGL(k_cluster_classify<true>, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+    hipLaunchKernelGGL((k_cluster_classify<true, true>), dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
     // the 2X batch vertices, flat over the batches
     const uint32_t mgrid = std::min<uint32_t>((config.maxGeneratedClusters + kClassifyWaves - 1) / kClassifyWaves,
                                               (uint32_t)num_cu() * 8u);
     hipLaunchKernelGGL(k_mini_vertices, dim3(mgrid), dim3(kClassifyWaves * kWave), 0, st, a);
-  } else
-    hipLaunchKernelGGL(k_cluster_classify<false>, dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+  } else if (config.flags & TC_FLAG_1X_TRANSIENT) {
+    hipLaunchKernelGGL((k_cluster_classify<false, true>), dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_cluster_classify<false, false>), dim3(grid), dim3(kClassifyWaves * kWave), 0, st, a);
+  }
   return launched();
 }
 

@@ -265,7 +265,7 @@ __device__ __forceinline__ void collapse4(const tc_bvh_node* nodes, uint32_t i, 
 constexpr uint32_t kRefitChunkBits = 10;
 constexpr uint32_t kRefitChunk = 1u << kRefitChunkBits;
 constexpr uint32_t kRefitThreads = kRefitChunk;
-constexpr uint32_t kRefitTopThreads = 256;
+constexpr uint32_t kRefitTopThreads = 1024;
 
 // Writes internal node `node` from its children's boxes (both final) and
 // returns its union box in `self`.
@@ -284,16 +284,18 @@ __device__ __forceinline__ tc_bvh_node write_node(const BvhArgs& a, uint32_t nod
   return out;
 }
 
-// refit of a straddling node: children from global memory
-__device__ __forceinline__ tc_bvh_node refit_node(const BvhArgs& a, uint32_t node, int64_t first, tc_aabb& self) {
+// refit of a straddling node from its child words cw: children's boxes
+// from global memory
+__device__ __forceinline__ tc_bvh_node refit_node(const BvhArgs& a, uint32_t node, const uint32_t cw[2], int64_t first,
+                                                  tc_aabb& self) {
   float lo[2][3], hi[2][3];
-  const uint32_t cw[2] = {a.child[2 * node + 0], a.child[2 * node + 1]};
+#pragma unroll
   for (int c = 0; c < 2; c++) {
     const uint32_t ch = cw[c];
     if (ch & TC_BVH_LEAF) {
       leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
     } else {
-      const tc_aabb bx = a.nodeBox[(uint32_t)first + ch];
+      const tc_aabb bx = gld_record(a.nodeBox + (uint32_t)first + ch);
       for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
     }
   }
@@ -394,9 +396,15 @@ __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
 // Nodes a refit_top workgroup lists for its four-wide pass; further nodes
 // are collapsed where they are refit.
 constexpr uint32_t kRefitTopList = 4096;
+// Arrivals of this launch are counted in LDS, one byte per internal node
+// (four per word), for instances of up to kRefitTopLdsNodes internal nodes;
+// the count the chunk launch left in the global counter is only read.
+// Larger instances count with global atomics.
+constexpr uint32_t kRefitTopLdsNodes = 131072;
 
 __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) {
   __shared__ uint32_t s_list[kRefitTopList];
+  __shared__ uint32_t s_arrive[kRefitTopLdsNodes / 4];
   __shared__ uint32_t s_listed;
   const uint32_t inst = blockIdx.x;
   if (inst >= a.numInstances) return;
@@ -408,22 +416,38 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
   if (first >= (int64_t)n) return;
   const uint32_t count = a.pendingCount[inst];
   const uint32_t base = ((uint32_t)first >> kRefitChunkBits) + inst;
+  const bool ldsCount = bi.clusterReferencesCount - 1 <= kRefitTopLdsNodes;
   if (threadIdx.x == 0) s_listed = 0;
+  if (ldsCount)
+    for (uint32_t w = threadIdx.x; w < (bi.clusterReferencesCount + 2) / 4; w += kRefitTopThreads) s_arrive[w] = 0;
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < count; t += kRefitTopThreads) {
     uint32_t node = a.pendingList[base + t];
+    // a node's child words and parent (written by the hierarchy launch) load
+    // together, the next node's while its arrival is counted
+    uint32_t cw[2] = {gld(a.child + 2 * node), gld(a.child + 2 * node + 1)};
+    uint32_t up = gld(a.parent + node);
     while (true) {
       tc_aabb self;
-      const tc_bvh_node rec = refit_node(a, node, first, self);
+      const tc_bvh_node rec = refit_node(a, node, cw, first, self);
       const uint32_t slot = atomicAdd(&s_listed, 1u);
       if (slot < kRefitTopList)
         s_list[slot] = node;
       else
         gst_record(a.nodes4 + node, collapse4_of(rec, a.nodes + first));
       if ((int64_t)node == first) break;
-      node = a.parent[node];
+      node = up;
+      cw[0] = gld(a.child + 2 * node), cw[1] = gld(a.child + 2 * node + 1);
+      up = gld(a.parent + node);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      if (atomicAdd(&a.counter[node], 1u) == 0) break;  // first arrival stops
+      uint32_t before;
+      if (ldsCount) {
+        const uint32_t rel = node - (uint32_t)first, sh = 8 * (rel & 3);
+        before = ((atomicAdd(&s_arrive[rel >> 2], 1u << sh) >> sh) & 0xFFu) + gld(a.counter + node);
+      } else {
+        before = atomicAdd(&a.counter[node], 1u);
+      }
+      if (before == 0) break;  // first arrival stops
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
   }

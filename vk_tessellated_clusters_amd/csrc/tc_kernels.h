@@ -72,7 +72,7 @@ tc_tuning tuning_snapshot();
 __global__ void k_prologue(KArgs a, uint32_t stage);
 __global__ void k_build_setup(KArgs a, int mode);
 __global__ void k_setup_prologue(KArgs a, int mode, uint32_t stage);
-template <bool TWO_X>
+template <bool TWO_X, bool TRANS>
 __global__ void k_cluster_classify(KArgs a);
 __global__ void k_mini_vertices(KArgs a);
 __global__ void k_triangle_split(KArgs a);
