@@ -219,20 +219,21 @@ __device__ __forceinline__ TableView load_table(const tc_tessellation_table* t) 
 }
 
 __device__ __forceinline__ tc_tess_table_entry cfg_entry(const TableView& T, uint32_t cfg) {
-  // one 8-byte load
-  uint2 raw = reinterpret_cast<const uint2*>(T.entries)[cfg_index(cfg)];
+  // one 8-byte load, through the global address space
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(T.entries) + 2 * cfg_index(cfg);
+  const uint32_t rx = gld(q), ry = gld(q + 1);
   tc_tess_table_entry e;
-  e.firstTriangle = (uint16_t)(raw.x & 0xFFFF);
-  e.firstVertex = (uint16_t)(raw.x >> 16);
-  e.numTriangles = (uint16_t)(raw.y & 0xFFFF);
-  e.numVertices = (uint16_t)(raw.y >> 16);
+  e.firstTriangle = (uint16_t)(rx & 0xFFFF);
+  e.firstVertex = (uint16_t)(rx >> 16);
+  e.numTriangles = (uint16_t)(ry & 0xFFFF);
+  e.numVertices = (uint16_t)(ry >> 16);
   return e;
 }
 
 // tessellation.glsl:165-176
 __device__ __forceinline__ void cfg_tri_vertices(const TableView& T, uint32_t cfg, uint32_t firstTriangle, uint32_t tri,
                                                  uint32_t out[3]) {
-  uint32_t packed = T.triangles[firstTriangle + tri];
+  uint32_t packed = gld(T.triangles + firstTriangle + tri);
   out[0] = packed & 0xFF;
   uint32_t i1 = (packed >> 8) & 0xFF, i2 = (packed >> 16) & 0xFF;
   bool fl = cfg_flipped(cfg);
@@ -242,7 +243,7 @@ __device__ __forceinline__ void cfg_tri_vertices(const TableView& T, uint32_t cf
 
 // tessellation.glsl:178-189
 __device__ __forceinline__ v3 cfg_vertex_bary(const TableView& T, uint32_t cfg, uint32_t firstVertex, uint32_t vert) {
-  v3 w = decode_bary(T.vertices[firstVertex + vert]);
+  v3 w = decode_bary(gld(T.vertices + firstVertex + vert));
   if (cfg_flipped(cfg)) {
     float t = w.x;
     w.x = w.y;
@@ -616,32 +617,50 @@ __device__ __forceinline__ uint32_t take_ticket_below(WsHeader* h, uint32_t numT
 // earlier launches read as "not yet published".
 // `abort` (optional): a word another wave sets to end every wait early; the
 // results are then invalid (the persistent split flags the launch).
+//
+// In two steps: lookback_publish writes the tile's aggregates of counters
+// [0, N) (its inclusive prefixes for tile 0); lookback_walk then walks back
+// over counters [C0, C0 + M) and publishes their inclusive prefixes.  A
+// tile may walk its counters in several groups: every predecessor published
+// all its aggregates before walking any group, so a later group never waits
+// on an unpublished word, and each group holds only its own M counters in
+// registers.
 template <int N>
-__device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
-                                         const uint64_t (&agg)[N], uint64_t (&excl)[N],
-                                         const uint32_t* abort = nullptr) {
+__device__ __forceinline__ void lookback_publish(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
+                                                 const uint64_t (&agg)[N]) {
+  const uint32_t lane = lane_id();
+  if (lane < (uint32_t)N) {
+    uint64_t a = 0;
+#pragma unroll
+    for (int c = 0; c < N; c++)
+      if ((uint32_t)c == lane) a = agg[c];
+    atomic_store_agent(&status[lane * (uint64_t)maxTiles + tile], pack_status(epoch, tile == 0 ? 2 : 1, a));
+  }
+}
+
+template <int C0, int M>
+__device__ __forceinline__ void lookback_walk(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
+                                              const uint64_t (&agg)[M], uint64_t (&excl)[M],
+                                              const uint32_t* abort = nullptr) {
   const uint32_t lane = lane_id();
   if (tile == 0) {
-    if (lane < (uint32_t)N) atomic_store_agent(&status[lane * (uint64_t)maxTiles], pack_status(epoch, 2, agg[lane]));
 #pragma unroll
-    for (int c = 0; c < N; c++) excl[c] = 0;
+    for (int c = 0; c < M; c++) excl[c] = 0;
     return;
   }
-  if (lane < (uint32_t)N) atomic_store_agent(&status[lane * (uint64_t)maxTiles + tile], pack_status(epoch, 1, agg[lane]));
-
-  uint64_t sum[N];
-  bool done[N];
+  uint64_t sum[M];
+  bool done[M];
 #pragma unroll
-  for (int c = 0; c < N; c++) sum[c] = 0, done[c] = false;
+  for (int c = 0; c < M; c++) sum[c] = 0, done[c] = false;
 
   int64_t pred = (int64_t)tile - 1 - (int64_t)lane;
   while (true) {
-    uint64_t w[N];
+    uint64_t w[M];
     bool ok = true;
 #pragma unroll
-    for (int c = 0; c < N; c++) {
+    for (int c = 0; c < M; c++) {
       if (pred >= 0) {
-        w[c] = atomic_load_agent(&status[c * (uint64_t)maxTiles + (uint64_t)pred]);
+        w[c] = atomic_load_agent(&status[(C0 + c) * (uint64_t)maxTiles + (uint64_t)pred]);
         ok = ok && ((uint32_t)(w[c] >> 44) == epoch) && ((w[c] >> 42) & 3u) != 0;
       } else {
         w[c] = pack_status(epoch, 2, 0);  // before tile 0: inclusive zero
@@ -654,7 +673,7 @@ __device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, ui
     }
     bool all = true;
 #pragma unroll
-    for (int c = 0; c < N; c++) {
+    for (int c = 0; c < M; c++) {
       if (done[c]) continue;
       uint64_t incl = ballot(((w[c] >> 42) & 3u) == 2);
       uint64_t val = w[c] & kValueMask;
@@ -671,14 +690,22 @@ __device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, ui
     pred -= kWave;
   }
 #pragma unroll
-  for (int c = 0; c < N; c++) excl[c] = uniform64(sum[c]);
-  if (lane < (uint32_t)N) {
+  for (int c = 0; c < M; c++) excl[c] = uniform64(sum[c]);
+  if (lane < (uint32_t)M) {
     uint64_t e = 0, a = 0;
 #pragma unroll
-    for (int c = 0; c < N; c++)
+    for (int c = 0; c < M; c++)
       if ((uint32_t)c == lane) e = excl[c], a = agg[c];
-    atomic_store_agent(&status[lane * (uint64_t)maxTiles + tile], pack_status(epoch, 2, e + a));
+    atomic_store_agent(&status[(C0 + lane) * (uint64_t)maxTiles + tile], pack_status(epoch, 2, e + a));
   }
+}
+
+template <int N>
+__device__ __forceinline__ void lookback(uint64_t* status, uint32_t maxTiles, uint32_t epoch, uint32_t tile,
+                                         const uint64_t (&agg)[N], uint64_t (&excl)[N],
+                                         const uint32_t* abort = nullptr) {
+  lookback_publish<N>(status, maxTiles, epoch, tile, agg);
+  lookback_walk<0, N>(status, maxTiles, epoch, tile, agg, excl, abort);
 }
 
 }  // namespace tc

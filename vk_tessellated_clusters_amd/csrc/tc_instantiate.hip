@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
         const tc_tess_table_entry e = cfg_entry(T, cf);
         nv[k] = e.numVertices;
         nt[k] = e.numTriangles;
-        ds[k] = T.templateSizes[cfg_index(cf)];
+        ds[k] = gld(T.templateSizes + cfg_index(cf));
       }
     }
     // :155-160 allocation in part order: scans over the wave's rounds
@@ -176,12 +176,12 @@ __global__ __launch_bounds__(256) void k_instantiate(KArgs a) {
         tc_template_instantiate_info info;
         info.clusterIdOffset = p | (TC_RT_CLUSTER_MODE_SINGLE_TESSELLATED << 30);
         info.geometryIndexOffset = 0;
-        info.clusterTemplateAddress = T.templateAddresses[cfg_index(cf)];
+        info.clusterTemplateAddress = gld(T.templateAddresses + cfg_index(cf));
         info.vertexBufferAddress = b->genVertices + vertexOffset * 12ull;
         info.vertexBufferStride = 12;
-        ptr<tc_template_instantiate_info>(b->tempInstantiations)[tempOffset] = info;
-        ptr<uint32_t>(b->tempInstanceIDs)[tempOffset] = ti[k].cluster.instanceID;
-        ptr<uint64_t>(b->tempClusterAddresses)[tempOffset] = b->genClusterData + dataOffset;
+        gst_record(ptr<tc_template_instantiate_info>(b->tempInstantiations) + tempOffset, info);
+        gst(ptr<uint32_t>(b->tempInstanceIDs) + tempOffset, ti[k].cluster.instanceID);
+        gst(ptr<uint64_t>(b->tempClusterAddresses) + tempOffset, b->genClusterData + dataOffset);
       }
       // :196-198 BLAS reference counts: a wave whose accepted parts all
       // belong to one instance reports one count; otherwise per round
