@@ -231,6 +231,14 @@ __device__ __forceinline__ tc_tess_table_entry cfg_entry(const TableView& T, uin
 }
 
 // tessellation.glsl:165-176
+// vertex indices of a table triangle word of config cfg
+__device__ __forceinline__ void tri_word_vertices(uint32_t packed, uint32_t cfg, uint32_t out[3]) {
+  out[0] = packed & 0xFF;
+  uint32_t i1 = (packed >> 8) & 0xFF, i2 = (packed >> 16) & 0xFF;
+  bool fl = cfg_flipped(cfg);
+  out[1] = fl ? i2 : i1;
+  out[2] = fl ? i1 : i2;
+}
 __device__ __forceinline__ void cfg_tri_vertices(const TableView& T, uint32_t cfg, uint32_t firstTriangle, uint32_t tri,
                                                  uint32_t out[3]) {
   uint32_t packed = gld(T.triangles + firstTriangle + tri);
