@@ -621,6 +621,9 @@ __global__ void k_tlas_root(BvhArgs a) {
 __global__ __launch_bounds__(kTlasRankSort) void k_tlas_build_small(BvhArgs a, const float* xforms) {
   __shared__ float s_b[6][kTlasRankSort / kWave];
   __shared__ uint64_t s_key[kTlasRankSort], s_sorted[kTlasRankSort];
+  __shared__ uint32_t s_val[kTlasRankSort], s_cnt[kTlasRankSort];
+  __shared__ uint32_t s_child[2 * kTlasRankSort], s_parent[2 * kTlasRankSort];
+  __shared__ tc_aabb s_leaf[kTlasRankSort], s_box[kTlasRankSort];  // by instance, by internal node
   const uint32_t n = a.numInstances, i = threadIdx.x, lane = lane_id(), wave = threadIdx.x / kWave;
   if (i < n) tlas_prep_one(a, xforms, i);
   if (i == 0) {
@@ -634,6 +637,7 @@ __global__ __launch_bounds__(kTlasRankSort) void k_tlas_build_small(BvhArgs a, c
   float c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   if (i < n) {
     bx = a.tlasLeafBox[i];
+    s_leaf[i] = bx;
     empty = !(bx.lo[0] <= bx.hi[0]);
     if (!empty)
       for (int q = 0; q < 3; q++) c[q] = c[3 + q] = (bx.lo[q] + bx.hi[q]) * 0.5f;
@@ -661,20 +665,69 @@ __global__ __launch_bounds__(kTlasRankSort) void k_tlas_build_small(BvhArgs a, c
     a.tlasKeys[rank] = k;
     a.tlasVals[rank] = i;
     s_sorted[rank] = k;
+    s_val[rank] = i;
   }
   __syncthreads();
-  // hierarchy over the sorted keys (k_tlas_hierarchy)
+  // hierarchy over the sorted keys (as k_tlas_hierarchy), refit (as
+  // tlas_refit_leaf) and four-wide view, with the topology, counters and
+  // boxes in LDS: a climb waits on no global load
   if (i < n) {
-    a.tlasCounter[i] = 0;
-    if (i + 1 < n) karras_node(s_sorted, 0, (int64_t)n - 1, i, a.tlasChild, a.tlasParent, n);
+    s_cnt[i] = 0;
+    if (i + 1 < n) karras_node(s_sorted, 0, (int64_t)n - 1, i, s_child, s_parent, n);
   }
   __syncthreads();
-  BvhArgs r = a;
-  r.tlasSortedKeys = a.tlasKeys;
-  r.tlasSortedVals = a.tlasVals;
-  if (i < n && n > 1) tlas_refit_leaf<true>(r, i);
+  // a TLAS leaf names its instance: the sorted position's value
+  const auto leaf_word = [&](uint32_t w) { return (w & TC_BVH_LEAF) ? (TC_BVH_LEAF | s_val[w & ~TC_BVH_LEAF]) : w; };
+  const auto box_of = [&](uint32_t w) -> const tc_aabb& {
+    return (w & TC_BVH_LEAF) ? s_leaf[s_val[w & ~TC_BVH_LEAF]] : s_box[w];
+  };
+  if (i < n && n > 1) {
+    uint32_t node = s_parent[n + i];
+    while (true) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      if (atomicAdd(&s_cnt[node], 1u) == 0) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const uint32_t cw[2] = {s_child[2 * node], s_child[2 * node + 1]};
+      float lo[2][3], hi[2][3];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const tc_aabb& bx = box_of(cw[c]);
+        for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
+      }
+      tc_bvh_node out;
+      for (int q = 0; q < 3; q++) out.lo0[q] = lo[0][q], out.hi0[q] = hi[0][q], out.lo1[q] = lo[1][q], out.hi1[q] = hi[1][q];
+      out.child0 = leaf_word(cw[0]);
+      out.child1 = leaf_word(cw[1]);
+      out._pad0 = out._pad1 = 0;
+      gst_record(a.tlasNodes + node, out);
+      tc_aabb self;
+      for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
+      s_box[node] = self;
+      if (node == 0) break;
+      node = s_parent[node];
+    }
+  }
   __syncthreads();
-  if (i + 1 < n) collapse4(a.tlasNodes, i, a.tlasNodes4);
+  if (i + 1 < n) {
+    const uint32_t cw[2] = {s_child[2 * i], s_child[2 * i + 1]};
+    Slot4 gs[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (cw[c] & TC_BVH_LEAF) {
+        const tc_aabb& bx = box_of(cw[c]);
+        gs[c][0] = gs[c][1] = slot4(bx.lo, bx.hi, leaf_word(cw[c]));
+      } else {
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+          const uint32_t gw = s_child[2 * cw[c] + d];
+          const tc_aabb& bx = box_of(gw);
+          gs[c][d] = slot4(bx.lo, bx.hi, leaf_word(gw));
+        }
+      }
+    }
+    gst_record(a.tlasNodes4 + i, make4((cw[0] & TC_BVH_LEAF) != 0, gs[0][0], gs[0][1], (cw[1] & TC_BVH_LEAF) != 0,
+                                       gs[1][0], gs[1][1]));
+  }
 }
 
 }  // namespace tc
