@@ -83,15 +83,40 @@ __device__ __forceinline__ uint32_t owner_instance(const tc_blas_build_info* inf
   return lo;
 }
 
-// BLAS keys over every inserted CLAS reference
-__global__ void k_blas_keys(BvhArgs a) {
+// Instances whose list starts a keys workgroup stages in LDS for the owner
+// search (more instances search the lists in global memory).
+constexpr uint32_t kKeysLdsInstances = 4096;
+
+// BLAS keys over every inserted CLAS reference.  A reference's header loads
+// while its owner is searched among the list starts.
+__global__ __launch_bounds__(256) void k_blas_keys(BvhArgs a) {
+  __shared__ uint32_t s_start[kKeysLdsInstances];
   const tc_scene_building* b = a.build;
   const uint32_t n = min(b->blasClusterCounter, a.capacity);
   const tc_blas_build_info* infos = ptr<const tc_blas_build_info>(b->blasBuildInfos);
   const uint64_t* refs = ptr<const uint64_t>(b->blasClusterAddresses);
+  const uint64_t base = b->blasClusterAddresses;
+  const bool lds = a.numInstances <= kKeysLdsInstances;
+  if (lds)
+    for (uint32_t i = threadIdx.x; i < a.numInstances; i += blockDim.x)
+      s_start[i] = (uint32_t)((gld(&infos[i].clusterReferences) - base) / 8);
+  __syncthreads();
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t inst = owner_instance(infos, a.numInstances, b->blasClusterAddresses, k);
-    const tc_clas_header h = *ptr<const tc_clas_header>(refs[k]);
+    const tc_clas_header h = gld_record(ptr<const tc_clas_header>(gld(refs + k)));
+    uint32_t inst;
+    if (lds) {
+      uint32_t lo = 0, hi = a.numInstances;  // last i with start(i) <= k
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_start[mid] <= k)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      inst = lo;
+    } else {
+      inst = owner_instance(infos, a.numInstances, base, k);
+    }
     const tc_render_instance& ri = a.instances[inst];
     const uint32_t m = morton24_of(h.lo, h.hi, ri.geoLo, ri.geoHi);
     a.keys[k] = ((uint64_t)inst << 32) | m;
