@@ -706,7 +706,12 @@ tc_status tc_triangle_split(const tc_bindings* b, tc_config config, tc_stream st
 /* triangle_tess_template_instantiate.comp.glsl:126-373 over build.partTriangles. */
 tc_status tc_triangle_tess_template_instantiate(const tc_bindings* b, tc_config config, tc_stream stream);
 
-/* blas_setup_insertion.comp.glsl:100-116 (exclusive scan over instances). */
+/* blas_setup_insertion.comp.glsl:100-116 (exclusive scan over instances).
+ * Lays every instance's reference list out after build.blasClusterCounter
+ * and advances it, as the shader does: run it once per frame, after the
+ * frame's reset.  tc_accel_build needs the lists to tile
+ * [0, blasClusterCounter) exactly, which a second run in the same frame
+ * breaks (the first lists then belong to no instance). */
 tc_status tc_blas_setup_insertion(const tc_bindings* b, tc_config config, tc_stream stream);
 
 /* blas_clusters_insert.comp.glsl:97-135; transient = push constant 1. */
@@ -769,7 +774,9 @@ uint64_t tc_accel_size(uint32_t capacity, uint32_t numInstances);
 /* Carves the arrays of a tc_accel out of caller-owned device memory. */
 tc_status tc_accel_bind(void* memory, uint64_t bytes, uint32_t capacity, uint32_t numInstances, tc_accel* out);
 /* Per-instance LBVH over the CLAS of build.blasBuildInfos (after
- * tc_blas_clusters_insert) and the TLAS over the instances.
+ * tc_blas_clusters_insert) and the TLAS over the instances.  The instances'
+ * reference lists must tile [0, build.blasClusterCounter), as one
+ * tc_blas_setup_insertion per frame leaves them.
  * instanceXforms: device array, per instance objectToWorld then
  * worldToObject, each 3x4 row-major (24 floats). */
 tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel* accel, const float* instanceXforms,

@@ -27,7 +27,7 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
   L.maxTiles = std::max(std::max(cfg.maxVisibleClusters, tilesSplit), std::max(tilesPart, 1u));
   L.sortCapacity = std::max(cfg.maxGeneratedClusters, 1u);
   // tiles of the larger of the CLAS sort and the TLAS instance sort
-  L.sortMaxTiles = (std::max(L.sortCapacity, kMaxInstances) + kSortTileSmall - 1) / kSortTileSmall;
+  L.sortMaxTiles = (std::max(L.sortCapacity, kMaxInstances) + kSortTileKeys - 1) / kSortTileKeys;
   L.instanceCapacity = kMaxInstances;
   uint64_t o = kWsHeaderBytes;
   L.status = o;
@@ -40,10 +40,12 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
     L.sortVals[i] = o;
     o = align256(o + (uint64_t)L.sortCapacity * 4);
   }
-  L.sortHist = o;
-  o = align256(o + (uint64_t)kSortRadix * L.sortMaxTiles * 4);
+  L.sortStatus = o;
+  o = align256(o + (uint64_t)kSortRadix * L.sortMaxTiles * 8);
+  L.sortState = o;
+  o = align256(o + sizeof(SortState));
   L.sortAux = o;
-  o = align256(o + (uint64_t)(kSortRadix + kMaxInstances) * 4);
+  o = align256(o + (uint64_t)kMaxInstances * 4);
   L.cullOffsets = o;
   o = align256(o + (uint64_t)kMaxInstances * 4);
   L.total = o;
@@ -125,8 +127,8 @@ static SortBuffers sort_buffers(const tc_bindings* b, const WsLayout& L) {
     sb.keys[i] = reinterpret_cast<uint64_t*>(w + L.sortKeys[i]);
     sb.vals[i] = reinterpret_cast<uint32_t*>(w + L.sortVals[i]);
   }
-  sb.hist = reinterpret_cast<uint32_t*>(w + L.sortHist);
-  sb.digitTotals = reinterpret_cast<uint32_t*>(w + L.sortAux);
+  sb.status = reinterpret_cast<uint64_t*>(w + L.sortStatus);
+  sb.state = reinterpret_cast<SortState*>(w + L.sortState);
   sb.capacity = L.sortCapacity;
   sb.maxTiles = L.sortMaxTiles;
   return sb;
@@ -560,7 +562,7 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
   const int tr = transient ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   SortBuffers sb = sort_buffers(b, L);
-  uint32_t* segStart = sb.digitTotals + kSortRadix;
+  uint32_t* segStart = reinterpret_cast<uint32_t*>(static_cast<char*>(b->workspace) + L.sortAux);
   uint32_t grid = std::min<uint32_t>((sb.capacity + 255) / 256, (uint32_t)num_cu() * 4);
   hipLaunchKernelGGL(k_insert_keys, dim3(grid), dim3(256), 0, st, a, sb.keys[0], sb.vals[0], sb.capacity, tr);
   uint32_t bits = 1;

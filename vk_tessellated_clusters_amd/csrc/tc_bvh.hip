@@ -851,8 +851,8 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
     sb.keys[i] = reinterpret_cast<uint64_t*>(w + W.sortKeys[i]);
     sb.vals[i] = reinterpret_cast<uint32_t*>(w + W.sortVals[i]);
   }
-  sb.hist = reinterpret_cast<uint32_t*>(w + W.sortHist);
-  sb.digitTotals = reinterpret_cast<uint32_t*>(w + W.sortAux);
+  sb.status = reinterpret_cast<uint64_t*>(w + W.sortStatus);
+  sb.state = reinterpret_cast<SortState*>(w + W.sortState);
   sb.capacity = W.sortCapacity;
   sb.maxTiles = W.sortMaxTiles;
   a.keys = sb.keys[0];
@@ -903,14 +903,14 @@ tc_status tc_accel_build(const tc_bindings* b, tc_config config, const tc_accel*
   hipLaunchKernelGGL(k_tlas_keys, dim3(1), dim3(256), 0, st, a);
   // the TLAS sort's scratch slot is the TLAS child / parent arrays, written
   // only afterwards (32 key bits = 4 passes, so the result lands back in
-  // slot 0); the workspace histogram holds kMaxInstances keys' tiles
+  // slot 0); it shares the workspace sort state and status words, which
+  // hold kMaxInstances keys' tiles
   SortBuffers tb = sb;
   tb.keys[0] = a.tlasKeys;
   tb.vals[0] = a.tlasVals;
   tb.keys[1] = reinterpret_cast<uint64_t*>(m + L.tlasChild);
   tb.vals[1] = reinterpret_cast<uint32_t*>(m + L.tlasParent);
   tb.capacity = a.numInstances;
-  tb.maxTiles = (a.numInstances + kSortTileSmall - 1) / kSortTileSmall;
   hipLaunchKernelGGL(k_tlas_root, dim3(1), dim3(64), 0, st, a);
   uint32_t tres = 0;
   radix_sort_pairs(tb, a.tlasRoot + 1, 32, 32, st, &tres);

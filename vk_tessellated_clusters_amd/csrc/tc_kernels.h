@@ -48,8 +48,9 @@ struct WsLayout {
   uint64_t status;       // look-back status words
   uint64_t sortKeys[2];  // radix sort ping-pong keys (u64)
   uint64_t sortVals[2];  // radix sort ping-pong values (u32)
-  uint64_t sortHist;     // per-digit per-tile counts
-  uint64_t sortAux;      // digit totals, segment starts, misc
+  uint64_t sortStatus;   // per-tile per-digit look-back words of the radix sort
+  uint64_t sortState;    // radix sort epoch, tickets and digit bases
+  uint64_t sortAux;      // segment starts of the BLAS insertion
   uint64_t cullOffsets;  // per-instance offsets of clusters_cull
   uint64_t total;
   uint32_t maxTiles;
@@ -58,10 +59,11 @@ struct WsLayout {
   uint32_t instanceCapacity;
 };
 
-constexpr uint32_t kSortTile = 1024;      // keys per radix-sort workgroup (large sorts)
-constexpr uint32_t kSortTileSmall = 256;  // ... for sorts of at most kSortSmallMax keys
-constexpr uint32_t kSortSmallMax = 1u << 19;
+constexpr uint32_t kSortThreads = 256;    // threads of a radix-sort pass workgroup
+constexpr uint32_t kSortKeysPerThread = 8;
+constexpr uint32_t kSortTileKeys = kSortThreads * kSortKeysPerThread;  // keys per pass tile
 constexpr uint32_t kSortRadix = 256;
+
 
 WsLayout ws_layout(const tc_config& cfg, uint32_t numInstances);
 

@@ -558,8 +558,9 @@ tc_status tc_renderer_graph_capture_meshwise(tc_renderer* rr, const tc_render_in
   return TC_OK;
 }
 
-// Everything from instances_classify to blas_clusters_insert
-// (renderer_raytrace_clusters_tess.cpp:433-686).
+// The frame's start and its tessellation: instances_classify through the
+// template instantiation (renderer_raytrace_clusters_tess.cpp:433-582);
+// tc_renderer_build_accel continues from the CLAS builds.
 tc_status tc_renderer_tessellate(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
   if (!rr) return TC_ERROR_INVALID_ARGUMENT;
   Renderer* r = R(rr);

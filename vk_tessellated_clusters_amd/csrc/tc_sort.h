@@ -5,13 +5,28 @@
 
 namespace tc {
 
+constexpr uint32_t kSortMaxPasses = 8;  // 8-bit digits of a u64 key
+
+// Sort state in the workspace (zeroed once by tc_workspace_init): the
+// upsweep leaves the digit bases of every pass and the pass tickets ready,
+// and clears what it accumulated, so a sort needs no clearing launch.
+struct SortState {
+  uint32_t epoch;                    // look-back epoch of the latest sort's last pass
+  uint32_t epoch0;                   // ... and of its first pass
+  uint32_t done;                     // upsweep workgroups finished (the last resets it)
+  uint32_t ticket[kSortMaxPasses];   // tile tickets of each pass
+  uint32_t _pad[5];
+  uint32_t totals[kSortMaxPasses][kSortRadix];  // digit counts being accumulated
+  uint32_t bases[kSortMaxPasses][kSortRadix];   // exclusive digit bases of each pass
+};
+
 struct SortBuffers {
   uint64_t* keys[2];
   uint32_t* vals[2];
-  uint32_t* hist;         // kSortRadix x maxTiles
-  uint32_t* digitTotals;  // kSortRadix
-  uint32_t capacity;      // max element count
-  uint32_t maxTiles;      // ceil(capacity / kSortTileSmall)
+  uint64_t* status;   // maxTiles x kSortRadix look-back words
+  SortState* state;
+  uint32_t capacity;  // max element count
+  uint32_t maxTiles;  // ceil(capacity / kSortTileKeys)
 };
 
 // Sorts keys[0]/vals[0] by key bits [bit0, bit0 + bits) (whole passes of 8

@@ -1,23 +1,38 @@
 // tc_sort.hip -- stable LSD radix sort of (u64 key, u32 value) pairs and the
 // deterministic BLAS insertion built on it.
 //
-// One pass per 8-bit digit: (1) per-tile digit histograms, (2) per-digit
-// exclusive scan over tiles, (3) stable scatter.  A tile is handled by one
-// wave in rounds of 64 keys; inside a round, lanes holding the same digit
-// find each other with 8 ballots, so ranks follow key order and the sort is
-// stable.  The element count is read on the device, and so is the tile
-// size: kSortTileSmall keys (4 rounds) for sorts of up to kSortSmallMax
-// keys, so that a frame's few hundred thousand keys spread over four times
-// as many waves, else kSortTile keys (16 rounds), which keeps the per-digit
-// scan over tiles short for large sorts.
+// A sort of P 8-bit digits is P + 1 launches:
+//   upsweep  every pass's digit counts in one read of the keys; the last
+//            workgroup to finish turns them into each pass's exclusive
+//            digit bases, resets the pass tickets and clears the counts,
+//            so the next sort finds the state as it found it
+//   pass     (one per digit) a workgroup takes the next tile ticket, ranks
+//            its kSortTileKeys keys by digit in LDS (lanes with the same
+//            digit find each other with 8 ballots, so ranks follow key
+//            order), publishes the tile's 256 digit counts and walks back
+//            over earlier tiles' counts (decoupled look-back, one digit per
+//            thread) for the keys of each digit before the tile, then
+//            scatters.  Stable, so LSD passes compose.
+// Status words carry the pass's epoch (a sort takes P consecutive epochs),
+// so words of earlier passes read as unpublished and are never cleared,
+// except when the epoch counter wraps.
 #include "tc_sort.h"
 
 namespace tc {
 
+constexpr uint32_t kUpsweepGroups = 64;
+constexpr uint32_t kUpsweepThreads = 1024;
+constexpr uint32_t kSortWaves = kSortThreads / kWave;
+constexpr uint32_t kSortLookback = 8;  // predecessors a look-back step reads per digit
+
+struct PassShifts {
+  uint32_t shift[kSortMaxPasses];
+  uint32_t numPasses;
+};
+
 __device__ __forceinline__ uint32_t sort_count(const uint32_t* count, uint32_t capacity) {
   return min(*count, capacity);
 }
-__device__ __forceinline__ uint32_t sort_tile(uint32_t n) { return n <= kSortSmallMax ? kSortTileSmall : kSortTile; }
 
 // lanes of this wave whose digit equals mine
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
@@ -30,129 +45,236 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   return valid ? peers : 0ull;
 }
 
-__global__ __launch_bounds__(64) void k_sort_hist(const uint64_t* keys, const uint32_t* count, uint32_t capacity,
-                                                  uint32_t* hist, uint32_t maxTiles, uint32_t shift) {
-  __shared__ uint32_t s_cnt[kSortRadix];
-  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
-  const uint32_t tile = blockIdx.x;
-  const uint32_t begin = tile * ts;
-  if (begin >= n) return;
-  const uint32_t lane = lane_id();
-  for (uint32_t d = lane; d < kSortRadix; d += kWave) s_cnt[d] = 0;
-  __syncthreads();
-  for (uint32_t r = 0; r < ts; r += kWave) {
-    const uint32_t i = begin + r + lane;
-    if (i < n) atomicAdd(&s_cnt[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
-  }
-  __syncthreads();
-  for (uint32_t d = lane; d < kSortRadix; d += kWave) hist[(uint64_t)d * maxTiles + tile] = s_cnt[d];
+// First epoch of the sort that follows state epoch `last` with P passes.
+__device__ __forceinline__ uint32_t sort_epoch0(uint32_t last, uint32_t P) {
+  const uint32_t e0 = last + 1;
+  return e0 + P > (1u << kEpochBits) ? 1u : e0;
 }
 
-// one workgroup per digit: exclusive scan of its per-tile counts
-__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* digitTotals, const uint32_t* count,
-                                                   uint32_t capacity, uint32_t maxTiles) {
-  __shared__ uint32_t s_wave[4];
-  __shared__ uint32_t s_carry;
-  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
-  const uint32_t numTiles = (n + ts - 1) / ts;
-  const uint32_t d = blockIdx.x;
-  uint32_t* h = hist + (uint64_t)d * maxTiles;
+__global__ __launch_bounds__(kUpsweepThreads) void k_sort_upsweep(const uint64_t* keys, const uint32_t* count,
+                                                                   uint32_t capacity, PassShifts ps, SortState* st,
+                                                                   uint64_t* status, uint32_t maxTiles) {
+  __shared__ uint32_t s_cnt[kSortMaxPasses][kSortRadix];
+  __shared__ uint32_t s_last;
+  const uint32_t n = sort_count(count, capacity), P = ps.numPasses;
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t t0 = 0; t0 < numTiles; t0 += blockDim.x) {
-    const uint32_t t = t0 + threadIdx.x;
-    const uint32_t v = t < numTiles ? h[t] : 0u;
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane == kWave - 1) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t waveBase = 0, total = 0;
-    for (uint32_t w = 0; w < 4; w++) {
-      if (w < wave) waveBase += s_wave[w];
-      total += s_wave[w];
-    }
-    if (t < numTiles) h[t] = s_carry + waveBase + incl - v;
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += total;
-    __syncthreads();
+  // read before any workgroup counts in; only the last one rewrites it
+  const uint32_t last = st->epoch, e0 = sort_epoch0(last, P);
+  for (uint32_t i = threadIdx.x; i < P * kSortRadix; i += blockDim.x) (&s_cnt[0][0])[i] = 0;
+  if (e0 != last + 1) {
+    // the epoch wraps: clear every status word, so none matches again
+    const uint64_t words = (uint64_t)maxTiles * kSortRadix;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+      status[i] = 0;
   }
-  if (threadIdx.x == 0) digitTotals[d] = s_carry;
+  __syncthreads();
+  for (uint32_t i0 = blockIdx.x * blockDim.x + wave * kWave; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + lane;
+    const bool valid = i < n;
+    const uint64_t k = valid ? gld(keys + i) : 0ull;
+    const uint64_t live = ballot(valid);
+#pragma unroll
+    for (uint32_t p = 0; p < kSortMaxPasses; p++) {
+      if (p >= P) break;
+      const uint32_t d = (uint32_t)(k >> ps.shift[p]) & 0xFFu;
+      // a wave whose keys share the digit (sorted high digits) adds once
+      if (ballot(valid && d == (uint32_t)__builtin_amdgcn_readfirstlane(d)) == live) {
+        if (lane == 0) atomicAdd(&s_cnt[p][d], (uint32_t)__popcll(live));
+      } else if (valid) {
+        atomicAdd(&s_cnt[p][d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  // The counts are device-scope atomics whose results are waited for, so
+  // each is performed before this workgroup counts in; an agent-scope fence
+  // would instead write back the whole L2 from every wave.
+  uint32_t seen = 0;
+  for (uint32_t i = threadIdx.x; i < P * kSortRadix; i += blockDim.x) {
+    const uint32_t v = (&s_cnt[0][0])[i];
+    if (v) seen |= __hip_atomic_fetch_add(&st->totals[0][0] + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (seen == 0xFFFFFFFFu) s_last = 2;  // never: makes the wait on the results explicit
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_last = __hip_atomic_fetch_add(&st->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (s_last != 1u) return;
+  // last workgroup: every other one has counted in; the totals are read
+  // with device-scope loads
+  for (uint32_t i = threadIdx.x; i < P * kSortRadix; i += blockDim.x) {
+    uint32_t* t = &st->totals[0][0] + i;
+    (&s_cnt[0][0])[i] = __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(t, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (wave < P) {  // wave p scans pass p's 256 totals, four per lane
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = s_cnt[wave][4 * lane + j], sum += v[j];
+    uint32_t run = wave_incl_scan(sum) - sum;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      st->bases[wave][4 * lane + j] = run;
+      run += v[j];
+    }
+  }
+  if (threadIdx.x == 0) {
+    st->done = 0;
+    for (uint32_t p = 0; p < kSortMaxPasses; p++) st->ticket[p] = 0;
+    st->epoch0 = e0;
+    st->epoch = e0 + P - 1;
+  }
 }
 
-__global__ __launch_bounds__(64) void k_sort_scatter(const uint64_t* keysIn, const uint32_t* valsIn, uint64_t* keysOut,
-                                                     uint32_t* valsOut, const uint32_t* hist, const uint32_t* digitTotals,
-                                                     const uint32_t* count, uint32_t capacity, uint32_t maxTiles,
-                                                     uint32_t shift) {
-  __shared__ uint32_t s_base[kSortRadix];
-  const uint32_t n = sort_count(count, capacity), ts = sort_tile(n);
-  const uint32_t tile = blockIdx.x;
-  const uint32_t begin = tile * ts;
-  if (begin >= n) return;
-  const uint32_t lane = lane_id();
-  // digit bases: exclusive scan of digit totals (4 digits per lane)
-  {
-    uint32_t v[4], s = 0;
+// Publishes the count `agg` of digit d in `tile` and walks back over the
+// earlier tiles' words of digit d, kSortLookback a step, until an
+// inclusive prefix; publishes the tile's own and returns the keys of digit
+// d in tiles 0..tile-1.  Predecessors are held by running workgroups
+// (tickets are taken in order) that publish their counts before they
+// wait, so the walk ends.
+__device__ __forceinline__ uint32_t digit_lookback(uint64_t* status, uint32_t epoch, uint32_t tile, uint32_t d,
+                                                   uint32_t agg) {
+  uint64_t* row = status + (uint64_t)tile * kSortRadix;
+  atomic_store_agent(&row[d], pack_status(epoch, tile == 0 ? 2 : 1, agg));
+  if (tile == 0) return 0;
+  uint64_t excl = 0;
+  int64_t pred = (int64_t)tile - 1;
+  while (true) {
+    uint64_t w[kSortLookback];
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = digitTotals[lane * 4 + k], s += v[k];
-    uint32_t run = wave_incl_scan(s) - s;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint32_t d = lane * 4 + k;
-      s_base[d] = run + hist[(uint64_t)d * maxTiles + tile];
-      run += v[k];
+    for (uint32_t j = 0; j < kSortLookback; j++) {
+      const int64_t q = pred - (int64_t)j;
+      w[j] = q >= 0 ? atomic_load_agent(&status[(uint64_t)q * kSortRadix + d]) : pack_status(epoch, 2, 0);
     }
+    bool stop = false, blocked = false;
+    uint32_t used = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kSortLookback; j++) {
+      if (stop || blocked) continue;
+      const uint32_t flag = (uint32_t)(w[j] >> 42) & 3u;
+      if ((uint32_t)(w[j] >> 44) != epoch || flag == 0) {
+        blocked = true;
+      } else {
+        excl += w[j] & kValueMask;
+        used++;
+        stop = flag == 2;
+      }
+    }
+    if (stop) break;
+    pred -= used;
+    if (used == 0) __builtin_amdgcn_s_sleep(1);
+  }
+  atomic_store_agent(&row[d], pack_status(epoch, 2, excl + agg));
+  return (uint32_t)excl;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t* keysIn, const uint32_t* valsIn,
+                                                            uint64_t* keysOut, uint32_t* valsOut, const uint32_t* count,
+                                                            uint32_t capacity, SortState* st, uint64_t* status,
+                                                            uint32_t pass, uint32_t shift) {
+  constexpr uint32_t K = kSortKeysPerThread;
+  __shared__ uint32_t s_wave[kSortWaves][kSortRadix];  // per-wave digit counts, then per-wave offsets
+  __shared__ uint32_t s_base[kSortRadix];
+  __shared__ uint32_t s_tile;
+  // the count, the pass's epoch and its digit bases load together
+  const uint32_t n = sort_count(count, capacity);
+  const uint32_t epoch = st->epoch0 + pass;
+  const uint32_t digitBase = threadIdx.x < kSortRadix ? st->bases[pass][threadIdx.x] : 0u;
+  const uint32_t numTiles = (n + kSortTileKeys - 1) / kSortTileKeys;
+  if (blockIdx.x >= numTiles) return;  // exactly numTiles workgroups take tickets
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&st->ticket[pass], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i = threadIdx.x; i < kSortWaves * kSortRadix; i += kSortThreads) (&s_wave[0][0])[i] = 0;
+  // wave w ranks the keys [first, first + 64 K) of the tile, 64 per round.
+  // Tickets almost always follow dispatch order, so the keys of tile
+  // blockIdx.x load while the ticket is taken, and load again otherwise.
+  uint64_t key[K];
+  uint32_t val[K], rank[K];
+  uint32_t tile = blockIdx.x;
+  uint32_t first = tile * kSortTileKeys + wave * (kWave * K);
+#pragma unroll
+  for (uint32_t r = 0; r < K; r++) {
+    const uint32_t i = first + r * kWave + lane;
+    key[r] = i < n ? gld(keysIn + i) : 0ull;
+    val[r] = i < n ? gld(valsIn + i) : 0u;
   }
   __syncthreads();
-  for (uint32_t r = 0; r < ts; r += kWave) {
-    const uint32_t i = begin + r + lane;
-    const bool valid = i < n;
-    uint64_t key = valid ? keysIn[i] : 0ull;
-    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
-    const uint64_t peers = match_digit(d, valid);
-    const uint32_t rank = mask_prefix(peers);
-    uint32_t dst = 0;
-    if (valid) dst = s_base[d] + rank;
-    __builtin_amdgcn_wave_barrier();
-    // the highest lane of each digit group advances the digit base
-    if (valid && (peers >> lane) == 1ull) s_base[d] += (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    if (valid) {
-      keysOut[dst] = key;
-      valsOut[dst] = valsIn[i];
+  if (s_tile >= numTiles) return;  // uniform; only a state left inconsistent gets here
+  if (s_tile != tile) {
+    tile = s_tile;
+    first = tile * kSortTileKeys + wave * (kWave * K);
+#pragma unroll
+    for (uint32_t r = 0; r < K; r++) {
+      const uint32_t i = first + r * kWave + lane;
+      key[r] = i < n ? gld(keysIn + i) : 0ull;
+      val[r] = i < n ? gld(valsIn + i) : 0u;
     }
   }
+#pragma unroll
+  for (uint32_t r = 0; r < K; r++) {
+    const bool valid = first + r * kWave + lane < n;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+    const uint64_t peers = match_digit(d, valid);
+    rank[r] = s_wave[wave][d] + mask_prefix(peers);
+    __builtin_amdgcn_wave_barrier();
+    // the highest lane of each digit group advances the wave's count
+    if (valid && (peers >> lane) == 1ull) s_wave[wave][d] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // one digit per thread of the first kSortRadix: the tile's count, the
+  // waves' offsets, the keys of the digit in earlier tiles
+  if (threadIdx.x < kSortRadix) {
+    const uint32_t d = threadIdx.x;
+    uint32_t agg = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kSortWaves; w++) {
+      const uint32_t c = s_wave[w][d];
+      s_wave[w][d] = agg;
+      agg += c;
+    }
+    s_base[d] = digitBase + digit_lookback(status, epoch, tile, d, agg);
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < K; r++) {
+    if (first + r * kWave + lane >= n) continue;
+    const uint32_t dd = (uint32_t)(key[r] >> shift) & 0xFFu;
+    const uint32_t dst = s_base[dd] + s_wave[wave][dd] + rank[r];
+    gst(keysOut + dst, key[r]);
+    gst(valsOut + dst, val[r]);
+  }
+}
+
+static void run_passes(const SortBuffers& sb, const uint32_t* count, const PassShifts& ps, hipStream_t stream,
+                       uint32_t* outIndex) {
+  hipLaunchKernelGGL(k_sort_upsweep, dim3(kUpsweepGroups), dim3(kUpsweepThreads), 0, stream, sb.keys[0], count,
+                     sb.capacity, ps, sb.state, sb.status, sb.maxTiles);
+  const uint32_t grid = std::max(1u, (sb.capacity + kSortTileKeys - 1) / kSortTileKeys);
+  uint32_t cur = 0;
+  for (uint32_t p = 0; p < ps.numPasses; p++) {
+    hipLaunchKernelGGL(k_sort_pass, dim3(grid), dim3(kSortThreads), 0, stream, sb.keys[cur], sb.vals[cur], sb.keys[cur ^ 1],
+                       sb.vals[cur ^ 1], count, sb.capacity, sb.state, sb.status, p, ps.shift[p]);
+    cur ^= 1;
+  }
+  if (outIndex) *outIndex = cur;
 }
 
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
                       uint32_t* outIndex) {
-  uint32_t cur = 0;
-  for (uint32_t shift = bit0; shift < bit0 + bits; shift += 8) {
-    hipLaunchKernelGGL(k_sort_hist, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], count, sb.capacity, sb.hist,
-                       sb.maxTiles, shift);
-    hipLaunchKernelGGL(k_sort_scan, dim3(kSortRadix), dim3(256), 0, stream, sb.hist, sb.digitTotals, count, sb.capacity,
-                       sb.maxTiles);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], sb.vals[cur], sb.keys[cur ^ 1],
-                       sb.vals[cur ^ 1], sb.hist, sb.digitTotals, count, sb.capacity, sb.maxTiles, shift);
-    cur ^= 1;
-  }
-  if (outIndex) *outIndex = cur;
+  PassShifts ps{};
+  for (uint32_t shift = bit0; shift < bit0 + bits && ps.numPasses < kSortMaxPasses; shift += 8)
+    ps.shift[ps.numPasses++] = shift;
+  run_passes(sb, count, ps, stream, outIndex);
 }
 
 void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t digitMask, hipStream_t stream,
                        uint32_t* outIndex) {
-  uint32_t cur = 0;
-  for (uint32_t d = 0; d < 8; d++) {
-    if (!((digitMask >> d) & 1u)) continue;
-    const uint32_t shift = 8 * d;
-    hipLaunchKernelGGL(k_sort_hist, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], count, sb.capacity, sb.hist,
-                       sb.maxTiles, shift);
-    hipLaunchKernelGGL(k_sort_scan, dim3(kSortRadix), dim3(256), 0, stream, sb.hist, sb.digitTotals, count, sb.capacity,
-                       sb.maxTiles);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(sb.maxTiles), dim3(64), 0, stream, sb.keys[cur], sb.vals[cur], sb.keys[cur ^ 1],
-                       sb.vals[cur ^ 1], sb.hist, sb.digitTotals, count, sb.capacity, sb.maxTiles, shift);
-    cur ^= 1;
-  }
-  if (outIndex) *outIndex = cur;
+  PassShifts ps{};
+  for (uint32_t d = 0; d < 8; d++)
+    if ((digitMask >> d) & 1u) ps.shift[ps.numPasses++] = 8 * d;
+  run_passes(sb, count, ps, stream, outIndex);
 }
 
 // ------------------------------------------------- blas_clusters_insert
