@@ -273,17 +273,18 @@ __device__ __forceinline__ void collapse4(const tc_bvh_node* nodes, uint32_t i, 
 // words, parents, range ends of the internal nodes, the leaves' parents)
 // and its leaves' boxes (from their CLAS headers, which also give the leaf
 // info words) in LDS, all loads in parallel.  Then every leaf climbs; the
-// second arrival at a node writes it, and its four-wide record.  Nodes
-// whose key range lies inside the chunk are read and counted in LDS
-// (workgroup-scope ordering), so a climb waits on no global load.  A climb
-// stops at the first node whose
+// second arrival at a node writes it.  Nodes whose key range lies inside
+// the chunk are read and counted in LDS (workgroup-scope ordering), so a
+// climb waits on no global load.  A climb stops at the first node whose
 // range straddles chunks; the second arrival there lists the node as
-// pending for its instance (both children are then final).
+// pending for its instance (both children are then final).  The chunk's
+// four-wide records are written from LDS once every climb is done.
 //
 // k_blas_refit_top: one workgroup per instance climbs from its pending
 // nodes through the straddling nodes with global counters (continuing the
 // counts of the first launch) and workgroup-scope ordering, since every
-// climber of an instance is in that workgroup.  A pending node's range
+// climber of an instance is in that workgroup; each node's four-wide
+// record is written where the node is refit.  A pending node's range
 // holds exactly one chunk boundary (its split), so an instance has fewer
 // pending nodes than chunks it spans; its list starts at
 // (first >> kRefitChunkBits) + instance.
@@ -307,24 +308,6 @@ __device__ __forceinline__ tc_bvh_node write_node(const BvhArgs& a, uint32_t nod
   for (int q = 0; q < 3; q++) self.lo[q] = fminf(lo[0][q], lo[1][q]), self.hi[q] = fmaxf(hi[0][q], hi[1][q]);
   gst_record(a.nodeBox + node, self);  // read by straddling parents and the TLAS
   return out;
-}
-
-// refit of a straddling node from its child words cw: children's boxes
-// from global memory
-__device__ __forceinline__ tc_bvh_node refit_node(const BvhArgs& a, uint32_t node, const uint32_t cw[2], int64_t first,
-                                                  tc_aabb& self) {
-  float lo[2][3], hi[2][3];
-#pragma unroll
-  for (int c = 0; c < 2; c++) {
-    const uint32_t ch = cw[c];
-    if (ch & TC_BVH_LEAF) {
-      leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
-    } else {
-      const tc_aabb bx = gld_record(a.nodeBox + (uint32_t)first + ch);
-      for (int q = 0; q < 3; q++) lo[c][q] = bx.lo[q], hi[c][q] = bx.hi[q];
-    }
-  }
-  return write_node(a, node, cw[0], cw[1], lo, hi, self);
 }
 
 __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
@@ -418,19 +401,42 @@ __global__ __launch_bounds__(kRefitThreads) void k_blas_refit(BvhArgs a) {
   }
 }
 
-// Nodes a refit_top workgroup lists for its four-wide pass; further nodes
-// are collapsed where they are refit.
-constexpr uint32_t kRefitTopList = 4096;
 // Arrivals of this launch are counted in LDS, one byte per internal node
 // (four per word), for instances of up to kRefitTopLdsNodes internal nodes;
 // the count the chunk launch left in the global counter is only read.
 // Larger instances count with global atomics.
 constexpr uint32_t kRefitTopLdsNodes = 131072;
 
+// Refit of straddling node `node` from its child words cw, with its
+// four-wide record: a leaf child's box comes from its CLAS, an internal
+// child's from its binary record, whose two slots are also the grandchildren
+// the four-wide record takes (the record's union is the child's node box,
+// to the bit).
+__device__ __forceinline__ void refit_top_node(const BvhArgs& a, uint32_t node, const uint32_t cw[2], int64_t first) {
+  float lo[2][3], hi[2][3];
+  Slot4 g[2][2];
+  bool leaf[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = cw[c];
+    leaf[c] = (ch & TC_BVH_LEAF) != 0;
+    if (leaf[c]) {
+      leaf_box(a, (uint32_t)first + (ch & ~TC_BVH_LEAF), lo[c], hi[c]);
+      g[c][0] = g[c][1] = slot4(lo[c], hi[c], ch);
+    } else {
+      const tc_bvh_node m = gld_record(a.nodes + (uint32_t)first + ch);
+      for (int q = 0; q < 3; q++) lo[c][q] = fminf(m.lo0[q], m.lo1[q]), hi[c][q] = fmaxf(m.hi0[q], m.hi1[q]);
+      g[c][0] = slot4(m.lo0, m.hi0, m.child0);
+      g[c][1] = slot4(m.lo1, m.hi1, m.child1);
+    }
+  }
+  tc_aabb self;
+  write_node(a, node, cw[0], cw[1], lo, hi, self);
+  gst_record(a.nodes4 + node, make4(leaf[0], g[0][0], g[0][1], leaf[1], g[1][0], g[1][1]));
+}
+
 __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) {
-  __shared__ uint32_t s_list[kRefitTopList];
   __shared__ uint32_t s_arrive[kRefitTopLdsNodes / 4];
-  __shared__ uint32_t s_listed;
   const uint32_t inst = blockIdx.x;
   if (inst >= a.numInstances) return;
   const tc_scene_building* b = a.build;
@@ -442,7 +448,6 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
   const uint32_t count = a.pendingCount[inst];
   const uint32_t base = ((uint32_t)first >> kRefitChunkBits) + inst;
   const bool ldsCount = bi.clusterReferencesCount - 1 <= kRefitTopLdsNodes;
-  if (threadIdx.x == 0) s_listed = 0;
   if (ldsCount)
     for (uint32_t w = threadIdx.x; w < (bi.clusterReferencesCount + 2) / 4; w += kRefitTopThreads) s_arrive[w] = 0;
   __syncthreads();
@@ -453,13 +458,7 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
     uint32_t cw[2] = {gld(a.child + 2 * node), gld(a.child + 2 * node + 1)};
     uint32_t up = gld(a.parent + node);
     while (true) {
-      tc_aabb self;
-      const tc_bvh_node rec = refit_node(a, node, cw, first, self);
-      const uint32_t slot = atomicAdd(&s_listed, 1u);
-      if (slot < kRefitTopList)
-        s_list[slot] = node;
-      else
-        gst_record(a.nodes4 + node, collapse4_of(rec, a.nodes + first));
+      refit_top_node(a, node, cw, first);
       if ((int64_t)node == first) break;
       node = up;
       cw[0] = gld(a.child + 2 * node), cw[1] = gld(a.child + 2 * node + 1);
@@ -476,12 +475,6 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
   }
-  // four-wide records of the listed nodes, once every climb is done (their
-  // children's records are then final), one per thread
-  __syncthreads();
-  const uint32_t listed = min(s_listed, kRefitTopList);
-  for (uint32_t t = threadIdx.x; t < listed; t += kRefitTopThreads)
-    collapse4(a.nodes + first, s_list[t] - (uint32_t)first, a.nodes4 + first);
 }
 
 // TLAS record and world box of instance i.
