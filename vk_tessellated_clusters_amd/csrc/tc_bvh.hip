@@ -463,11 +463,14 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
       node = up;
       cw[0] = gld(a.child + 2 * node), cw[1] = gld(a.child + 2 * node + 1);
       up = gld(a.parent + node);
+      // the chunk launch's arrivals at the node are final: loaded with the
+      // node's words, ahead of the fence, not after it
+      const uint32_t prior = ldsCount ? gld(a.counter + node) : 0u;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       uint32_t before;
       if (ldsCount) {
         const uint32_t rel = node - (uint32_t)first, sh = 8 * (rel & 3);
-        before = ((atomicAdd(&s_arrive[rel >> 2], 1u << sh) >> sh) & 0xFFu) + gld(a.counter + node);
+        before = ((atomicAdd(&s_arrive[rel >> 2], 1u << sh) >> sh) & 0xFFu) + prior;
       } else {
         before = atomicAdd(&a.counter[node], 1u);
       }
