@@ -453,19 +453,24 @@ __global__ __launch_bounds__(kRefitTopThreads) void k_blas_refit_top(BvhArgs a) 
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < count; t += kRefitTopThreads) {
     uint32_t node = a.pendingList[base + t];
-    // a node's child words and parent (written by the hierarchy launch) load
-    // together, the next node's while its arrival is counted
+    // A node's child words and parent (written by the hierarchy launch) and
+    // the chunk launch's arrivals there are final, so the parent's load
+    // together with the node's child records: one round trip a level.
     uint32_t cw[2] = {gld(a.child + 2 * node), gld(a.child + 2 * node + 1)};
     uint32_t up = gld(a.parent + node);
     while (true) {
+      const bool root = (int64_t)node == first;
+      uint32_t ncw[2] = {0u, 0u}, nup = 0u, prior = 0u;
+      if (!root) {
+        ncw[0] = gld(a.child + 2 * up), ncw[1] = gld(a.child + 2 * up + 1);
+        nup = gld(a.parent + up);
+        prior = ldsCount ? gld(a.counter + up) : 0u;
+      }
       refit_top_node(a, node, cw, first);
-      if ((int64_t)node == first) break;
+      if (root) break;
       node = up;
-      cw[0] = gld(a.child + 2 * node), cw[1] = gld(a.child + 2 * node + 1);
-      up = gld(a.parent + node);
-      // the chunk launch's arrivals at the node are final: loaded with the
-      // node's words, ahead of the fence, not after it
-      const uint32_t prior = ldsCount ? gld(a.counter + node) : 0u;
+      cw[0] = ncw[0], cw[1] = ncw[1];
+      up = nup;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       uint32_t before;
       if (ldsCount) {
