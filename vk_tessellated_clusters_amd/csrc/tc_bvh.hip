@@ -5,9 +5,9 @@
 //   keys    24-bit Morton code (8 bits per axis) of every CLAS centre inside
 //           its instance's geometry bounds, the instance index above it:
 //           (instance << 32) | morton24, one 64-bit key;
-//   sort    the LDS radix sort of tc_sort.hip over the three Morton digits
-//           and the one or two instance digits (stable, CLAS order breaks
-//           ties);
+//   sort    the radix sort of tc_sort.hip (one upsweep, then one
+//           single-pass launch per digit) over the three Morton digits and
+//           the one or two instance digits (stable, CLAS order breaks ties);
 //   build   Karras' binary radix tree over each instance's key segment
 //           (internal nodes of instance i live at [s_i, s_i + n_i - 1));
 //   refit   bottom-up with one arrival counter per internal node, in LDS
