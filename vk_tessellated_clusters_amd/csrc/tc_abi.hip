@@ -564,15 +564,23 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
   SortBuffers sb = sort_buffers(b, L);
   uint32_t* segStart = reinterpret_cast<uint32_t*>(static_cast<char*>(b->workspace) + L.sortAux);
   uint32_t grid = std::min<uint32_t>((sb.capacity + 255) / 256, (uint32_t)num_cu() * 4);
-  hipLaunchKernelGGL(k_insert_keys, dim3(grid), dim3(256), 0, st, a, sb.keys[0], sb.vals[0], sb.capacity, tr);
   uint32_t bits = 1;
   while ((1u << bits) < a.numInstances) bits++;
   uint32_t res = 0;
-  radix_sort_pairs(sb, tr ? &b->build->transBuildCounter : &b->build->tempInstantiateCounter, 0, bits, st, &res);
-  hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart, sb.capacity,
-                     tr);
+  // the sort reads the list's instance IDs itself (the field holding the
+  // list's address lies in the device-side building record)
+  const uint64_t* ids = tr ? &b->build->transInstanceIDs : &b->build->tempInstanceIDs;
+  radix_sort_pairs(sb, tr ? &b->build->transBuildCounter : &b->build->tempInstantiateCounter, 0, bits, st, &res, ids);
+  const uint32_t* starts = segStart;
+  if (!tr && bits <= 8) {
+    // one digit pass: an instance's first sorted position is its digit base
+    starts = sb.state->bases[0];
+  } else {
+    hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart,
+                       sb.capacity, tr);
+  }
   hipLaunchKernelGGL(k_insert_scatter, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res],
-                     (const uint32_t*)sb.vals[res], (const uint32_t*)segStart, sb.capacity, tr);
+                     (const uint32_t*)sb.vals[res], starts, sb.capacity, tr);
   return launched();
 }
 

@@ -30,16 +30,17 @@ struct SortBuffers {
 };
 
 // Sorts keys[0]/vals[0] by key bits [bit0, bit0 + bits) (whole passes of 8
-// bits); *outIndex receives the ping-pong slot holding the result.
+// bits); *outIndex receives the ping-pong slot holding the result.  With
+// listField (the device address of a field holding a u32 list's address)
+// the sort starts from that list instead, each key's value its position.
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
-                      uint32_t* outIndex);
+                      uint32_t* outIndex, const uint64_t* listField = nullptr);
 // The same over the 8-bit digits named by digitMask (bit d: key bits
 // [8d, 8d + 8)), lowest digit first; digits where every key agrees may be
 // left out.
 void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t digitMask, hipStream_t stream,
                        uint32_t* outIndex);
 
-__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity, int transient);
 __global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segStart, uint32_t capacity, int transient);
 __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
                                  uint32_t capacity, int transient);

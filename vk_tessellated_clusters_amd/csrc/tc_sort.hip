@@ -34,6 +34,13 @@ __device__ __forceinline__ uint32_t sort_count(const uint32_t* count, uint32_t c
   return min(*count, capacity);
 }
 
+// The u32 key list a sort may start from instead of a key slot (LIST
+// instances): `field` is the device address of a field holding the list's
+// address.  Its keys take their positions as values.
+__device__ __forceinline__ const uint32_t* sort_list(const uint64_t* field) {
+  return ptr<const uint32_t>(gld(field));
+}
+
 // lanes of this wave whose digit equals mine
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   uint64_t peers = ballot(valid);
@@ -51,12 +58,14 @@ __device__ __forceinline__ uint32_t sort_epoch0(uint32_t last, uint32_t P) {
   return e0 + P > (1u << kEpochBits) ? 1u : e0;
 }
 
-__global__ __launch_bounds__(kUpsweepThreads) void k_sort_upsweep(const uint64_t* keys, const uint32_t* count,
-                                                                   uint32_t capacity, PassShifts ps, SortState* st,
-                                                                   uint64_t* status, uint32_t maxTiles) {
+template <bool LIST>
+__global__ __launch_bounds__(kUpsweepThreads) void k_sort_upsweep(const uint64_t* keys, const uint64_t* listField,
+                                                                   const uint32_t* count, uint32_t capacity, PassShifts ps,
+                                                                   SortState* st, uint64_t* status, uint32_t maxTiles) {
   __shared__ uint32_t s_cnt[kSortMaxPasses][kSortRadix];
   __shared__ uint32_t s_last;
   const uint32_t n = sort_count(count, capacity), P = ps.numPasses;
+  const uint32_t* list = LIST ? sort_list(listField) : nullptr;
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
   // read before any workgroup counts in; only the last one rewrites it
   const uint32_t last = st->epoch, e0 = sort_epoch0(last, P);
@@ -71,7 +80,7 @@ __global__ __launch_bounds__(kUpsweepThreads) void k_sort_upsweep(const uint64_t
   for (uint32_t i0 = blockIdx.x * blockDim.x + wave * kWave; i0 < n; i0 += gridDim.x * blockDim.x) {
     const uint32_t i = i0 + lane;
     const bool valid = i < n;
-    const uint64_t k = valid ? gld(keys + i) : 0ull;
+    const uint64_t k = valid ? (LIST ? (uint64_t)gld(list + i) : gld(keys + i)) : 0ull;
     const uint64_t live = ballot(valid);
 #pragma unroll
     for (uint32_t p = 0; p < kSortMaxPasses; p++) {
@@ -169,16 +178,18 @@ __device__ __forceinline__ uint32_t digit_lookback(uint64_t* status, uint32_t ep
   return (uint32_t)excl;
 }
 
+template <bool LIST>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t* keysIn, const uint32_t* valsIn,
-                                                            uint64_t* keysOut, uint32_t* valsOut, const uint32_t* count,
-                                                            uint32_t capacity, SortState* st, uint64_t* status,
-                                                            uint32_t pass, uint32_t shift) {
+                                                            const uint64_t* listField, uint64_t* keysOut, uint32_t* valsOut,
+                                                            const uint32_t* count, uint32_t capacity, SortState* st,
+                                                            uint64_t* status, uint32_t pass, uint32_t shift) {
   constexpr uint32_t K = kSortKeysPerThread;
   __shared__ uint32_t s_wave[kSortWaves][kSortRadix];  // per-wave digit counts, then per-wave offsets
   __shared__ uint32_t s_base[kSortRadix];
   __shared__ uint32_t s_tile;
   // the count, the pass's epoch and its digit bases load together
   const uint32_t n = sort_count(count, capacity);
+  const uint32_t* list = LIST ? sort_list(listField) : nullptr;  // a first pass over a u32 list
   const uint32_t epoch = st->epoch0 + pass;
   const uint32_t digitBase = threadIdx.x < kSortRadix ? st->bases[pass][threadIdx.x] : 0u;
   const uint32_t numTiles = (n + kSortTileKeys - 1) / kSortTileKeys;
@@ -196,8 +207,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t* keys
 #pragma unroll
   for (uint32_t r = 0; r < K; r++) {
     const uint32_t i = first + r * kWave + lane;
-    key[r] = i < n ? gld(keysIn + i) : 0ull;
-    val[r] = i < n ? gld(valsIn + i) : 0u;
+    key[r] = i < n ? (LIST ? (uint64_t)gld(list + i) : gld(keysIn + i)) : 0ull;
+    val[r] = i < n ? (LIST ? i : gld(valsIn + i)) : 0u;
   }
   __syncthreads();
   if (s_tile >= numTiles) return;  // uniform; only a state left inconsistent gets here
@@ -207,8 +218,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t* keys
 #pragma unroll
     for (uint32_t r = 0; r < K; r++) {
       const uint32_t i = first + r * kWave + lane;
-      key[r] = i < n ? gld(keysIn + i) : 0ull;
-      val[r] = i < n ? gld(valsIn + i) : 0u;
+      key[r] = i < n ? (LIST ? (uint64_t)gld(list + i) : gld(keysIn + i)) : 0ull;
+      val[r] = i < n ? (LIST ? i : gld(valsIn + i)) : 0u;
     }
   }
 #pragma unroll
@@ -247,26 +258,37 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t* keys
   }
 }
 
+// The first pass reads slot 0, or the list `listField` names and writes
+// slot 0; each later pass reads the slot the one before wrote.
 static void run_passes(const SortBuffers& sb, const uint32_t* count, const PassShifts& ps, hipStream_t stream,
-                       uint32_t* outIndex) {
-  hipLaunchKernelGGL(k_sort_upsweep, dim3(kUpsweepGroups), dim3(kUpsweepThreads), 0, stream, sb.keys[0], count,
-                     sb.capacity, ps, sb.state, sb.status, sb.maxTiles);
+                       uint32_t* outIndex, const uint64_t* listField) {
+  if (listField)
+    hipLaunchKernelGGL(k_sort_upsweep<true>, dim3(kUpsweepGroups), dim3(kUpsweepThreads), 0, stream, sb.keys[0], listField,
+                       count, sb.capacity, ps, sb.state, sb.status, sb.maxTiles);
+  else
+    hipLaunchKernelGGL(k_sort_upsweep<false>, dim3(kUpsweepGroups), dim3(kUpsweepThreads), 0, stream, sb.keys[0], listField,
+                       count, sb.capacity, ps, sb.state, sb.status, sb.maxTiles);
   const uint32_t grid = std::max(1u, (sb.capacity + kSortTileKeys - 1) / kSortTileKeys);
-  uint32_t cur = 0;
+  int cur = listField ? -1 : 0;  // -1: the list
   for (uint32_t p = 0; p < ps.numPasses; p++) {
-    hipLaunchKernelGGL(k_sort_pass, dim3(grid), dim3(kSortThreads), 0, stream, sb.keys[cur], sb.vals[cur], sb.keys[cur ^ 1],
-                       sb.vals[cur ^ 1], count, sb.capacity, sb.state, sb.status, p, ps.shift[p]);
-    cur ^= 1;
+    const int out = cur < 0 ? 0 : cur ^ 1;
+    if (cur < 0)
+      hipLaunchKernelGGL(k_sort_pass<true>, dim3(grid), dim3(kSortThreads), 0, stream, nullptr, nullptr, listField,
+                         sb.keys[out], sb.vals[out], count, sb.capacity, sb.state, sb.status, p, ps.shift[p]);
+    else
+      hipLaunchKernelGGL(k_sort_pass<false>, dim3(grid), dim3(kSortThreads), 0, stream, sb.keys[cur], sb.vals[cur], nullptr,
+                         sb.keys[out], sb.vals[out], count, sb.capacity, sb.state, sb.status, p, ps.shift[p]);
+    cur = out;
   }
-  if (outIndex) *outIndex = cur;
+  if (outIndex) *outIndex = cur < 0 ? 0u : (uint32_t)cur;
 }
 
 void radix_sort_pairs(const SortBuffers& sb, const uint32_t* count, uint32_t bit0, uint32_t bits, hipStream_t stream,
-                      uint32_t* outIndex) {
+                      uint32_t* outIndex, const uint64_t* listField) {
   PassShifts ps{};
   for (uint32_t shift = bit0; shift < bit0 + bits && ps.numPasses < kSortMaxPasses; shift += 8)
     ps.shift[ps.numPasses++] = shift;
-  run_passes(sb, count, ps, stream, outIndex);
+  run_passes(sb, count, ps, stream, outIndex, listField);
 }
 
 void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t digitMask, hipStream_t stream,
@@ -274,7 +296,7 @@ void radix_sort_digits(const SortBuffers& sb, const uint32_t* count, uint32_t di
   PassShifts ps{};
   for (uint32_t d = 0; d < 8; d++)
     if ((digitMask >> d) & 1u) ps.shift[ps.numPasses++] = 8 * d;
-  run_passes(sb, count, ps, stream, outIndex);
+  run_passes(sb, count, ps, stream, outIndex, nullptr);
 }
 
 // ------------------------------------------------- blas_clusters_insert
@@ -298,14 +320,6 @@ __device__ __forceinline__ InsertList insert_list(const tc_scene_building* b, in
   return l;
 }
 
-__global__ void k_insert_keys(KArgs a, uint64_t* keys, uint32_t* vals, uint32_t capacity, int transient) {
-  const InsertList l = insert_list(a.build, transient, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
-    keys[i] = l.ids[i];
-    vals[i] = i;
-  }
-}
-
 // segStart[instance] = first sorted position of the instance minus the
 // references it already holds (templates, for the transient pass), so that
 // k - segStart is the slot in its list (modulo 2^32).
@@ -323,6 +337,9 @@ __global__ void k_insert_seg_start(KArgs a, const uint64_t* keys, uint32_t* segS
 // the CLAS of its instance (CLAS order), in that instance's reference list.
 __global__ void k_insert_scatter(KArgs a, const uint64_t* keys, const uint32_t* vals, const uint32_t* segStart,
                                  uint32_t capacity, int transient) {
+  // segStart: as k_insert_seg_start leaves it; for a template pass sorted
+  // in one digit pass it may be that pass's digit bases instead (the first
+  // sorted position of each instance, as every instance is its own digit)
   __shared__ unsigned long long s_sizeSum;
   if (threadIdx.x == 0) s_sizeSum = 0;
   __syncthreads();
