@@ -30,6 +30,9 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
   L.sortMaxTiles = (std::max(L.sortCapacity, kMaxInstances) + kSortTileKeys - 1) / kSortTileKeys;
   L.instanceCapacity = kMaxInstances;
   uint64_t o = kWsHeaderBytes;
+  // the radix sort's state follows the stage header at a fixed offset
+  L.sortState = o;
+  o = align256(o + sizeof(SortState));
   L.status = o;
   o = align256(o + (uint64_t)L.maxTiles * kMaxCounters * 8);
   for (int i = 0; i < 2; i++) {
@@ -42,8 +45,6 @@ WsLayout ws_layout(const tc_config& cfg, uint32_t) {
   }
   L.sortStatus = o;
   o = align256(o + (uint64_t)kSortRadix * L.sortMaxTiles * 8);
-  L.sortState = o;
-  o = align256(o + sizeof(SortState));
   L.sortAux = o;
   o = align256(o + (uint64_t)kMaxInstances * 4);
   L.cullOffsets = o;
