@@ -30,8 +30,8 @@ struct Renderer {
   tc_vertex_tape* tape = nullptr;  // with TC_FLAG_VERTEX_TAPE
   tc_scene_building hostBuild{};
   // Frame constants travel through a ring of pinned slots: the host fills
-  // slot hostFrame % kViewSlots, k_fetch_view (in the stream or the graph)
-  // reads the slot named by a device counter that advances once per
+  // slot hostFrame % kViewSlots, k_frame_start (in the stream or the
+  // graph) reads the slot named by a device counter that advances once per
   // frame, and an event per slot keeps the host from refilling a slot
   // that a queued frame has not read yet.
   tc_frame_constants* pinnedRing = nullptr;
@@ -58,25 +58,6 @@ struct Renderer {
 
 static uint64_t a256(uint64_t v) { return (v + 255) & ~255ull; }
 constexpr uint32_t kViewSlots = 8;
-
-// Copies the pinned slot the device frame counter names into the frame
-// constants and advances the counter (one workgroup).  The constants they
-// replace become viewLast (the first frame's viewLast is its own view).
-__global__ __launch_bounds__(256) void k_fetch_view(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view,
-                                                    tc_frame_constants* viewLast) {
-  const uint32_t frame = counter[0];
-  const uint32_t slot = frame % kViewSlots;
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(ring + slot);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(view);
-  uint32_t* last = reinterpret_cast<uint32_t*>(viewLast);
-  for (uint32_t i = threadIdx.x; i < sizeof(tc_frame_constants) / 4; i += blockDim.x) {
-    const uint32_t v = src[i];
-    last[i] = frame ? dst[i] : v;
-    dst[i] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) counter[0] = frame + 1;
-}
 
 // inverse of a 4x4 row-major matrix in double precision (cofactors)
 static bool invert4(const double m[16], double out[16]) {
@@ -118,6 +99,44 @@ __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building*
   for (uint64_t i = gid; i < sizeof(tc_readback) / 4; i += stride) reinterpret_cast<uint32_t*>(rb)[i] = 0;
   for (uint64_t i = gid; i < splitWords; i += stride) split[i] = ~0u;
 }
+
+// The frame's start in one launch: workgroup 0 copies the pinned slot the
+// device frame counter names into the frame constants (the constants they
+// replace become viewLast; the first frame's viewLast is its own view),
+// advances the counter and resets SceneBuilding, taking the eye position
+// from that slot; every workgroup clears Readback and fills the split list
+// with ~0.  No workgroup waits on another.
+__global__ void k_frame_start(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view,
+                              tc_frame_constants* viewLast, tc_scene_building* build, const tc_scene_building* tmpl,
+                              tc_readback* rb, uint32_t* split, uint64_t splitWords) {
+  __shared__ uint32_t s_frame;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) s_frame = counter[0];
+    __syncthreads();
+    const uint32_t frame = s_frame;
+    const tc_frame_constants* src = ring + frame % kViewSlots;
+    const uint32_t* sw = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(view);
+    uint32_t* last = reinterpret_cast<uint32_t*>(viewLast);
+    for (uint32_t i = threadIdx.x; i < sizeof(tc_frame_constants) / 4; i += blockDim.x) {
+      const uint32_t v = sw[i];
+      last[i] = frame ? dst[i] : v;
+      dst[i] = v;
+    }
+    if (threadIdx.x < sizeof(tc_scene_building) / 4) {
+      uint32_t v = reinterpret_cast<const uint32_t*>(tmpl)[threadIdx.x];
+      if (threadIdx.x < 3) v = __float_as_uint(src->viewPos[threadIdx.x]);
+      reinterpret_cast<uint32_t*>(build)[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) counter[0] = frame + 1;
+  }
+  const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gid; i < sizeof(tc_readback) / 4; i += stride) reinterpret_cast<uint32_t*>(rb)[i] = 0;
+  for (uint64_t i = gid; i < splitWords; i += stride) split[i] = ~0u;
+}
+static_assert(sizeof(tc_scene_building) / 4 <= 256, "the building record is reset by workgroup 0 of k_frame_start");
 
 // Writes the frame constants into a ring slot, first waiting until the
 // frame that used the slot last has read it.
@@ -491,13 +510,15 @@ static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, h
     const uint32_t slot = (uint32_t)(r->hostFrame % kViewSlots);
     if (!r->capturing && !ensure_hiz(r, frameHost)) return TC_ERROR_OUT_OF_MEMORY;
     if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
-    hipLaunchKernelGGL(k_fetch_view, dim3(1), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter,
-                       r->view, r->viewLast);
+    hipLaunchKernelGGL(k_frame_start, dim3(256), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter,
+                       r->view, r->viewLast, r->build, (const tc_scene_building*)r->buildTemplate, r->readback,
+                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4);
     if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
+  } else {
+    hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
+                       (const tc_frame_constants*)r->view, r->readback,
+                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4);
   }
-  hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
-                     (const tc_frame_constants*)r->view, r->readback, reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles),
-                     r->splitBytes / 4);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 
