@@ -439,19 +439,40 @@ __global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
   const ViewParams vp = view_params(a);
   const TableView T = load_table(a.tessTable);
   const uint32_t waitLimit = ld_agent(&q->waitLimit);
-  uint32_t p = 0;
+  // p: the pass of this wave's last tile; seen: the passes published when
+  // this wave last acquired the pass table (lane 0's copies)
+  uint32_t p = 0, seen = 0;
   while (true) {
-    const uint32_t g = take_ticket(h);
+    // A finished queue has no tile left: leave without an atomic on the one
+    // ticket word (every wave of the launch, when nothing splits).
+    uint32_t g = 0, fin = 0;
+    if (lane == 0) {
+      fin = ld_agent(&q->finished);
+      if (!fin) g = __hip_atomic_fetch_add(&h->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (uniform(fin)) break;
+    g = uniform(g);
     uint32_t found = 0xFFFFFFFFu;
     if (lane == 0) {
       for (uint32_t spins = 0;; spins++) {
-        const uint32_t ready = __hip_atomic_load(&q->passReady, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        while (p < ready && g >= ld_agent(&q->pass[p].tileBegin) + ld_agent(&q->pass[p].numTiles)) p++;
-        if (p < ready) {
+        while (p < seen && g >= ld_agent(&q->pass[p].tileBegin) + ld_agent(&q->pass[p].numTiles)) p++;
+        if (p < seen) {
           found = p;
           break;
         }
-        if (__hip_atomic_load(&q->finished, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
+        // Poll with plain agent-scope loads: an acquire at that scope
+        // invalidates the XCD's L2, and thousands of waiting waves would
+        // keep clearing it under the waves at work.  The acquire runs once
+        // per publication seen.
+        const uint32_t ready = ld_agent(&q->passReady);
+        if (ready > seen) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          seen = ready;
+          continue;
+        }
+        // passReady reached numPasses before `finished` was set, so a wave
+        // that finds the queue finished holds a ticket past every pass
+        if (ld_agent(&q->finished)) break;
         // a wait far beyond any frame's split time: flag the launch and end
         // every wait (look-backs included) so the grid drains
         if (spins >= waitLimit || ld_agent(&q->error)) {
@@ -478,8 +499,12 @@ __global__ __launch_bounds__(64) void k_triangle_split_persistent(KArgs a) {
     // completion: this tile's lists and counters are visible before it counts
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) {
-      const uint32_t old = __hip_atomic_fetch_add(&q->pass[p].done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (old + 1 == numTiles) publish_split_passes(a, q, h, p);
+      const uint32_t old = __hip_atomic_fetch_add(&q->pass[p].done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == numTiles) {
+        // every other tile of the pass released before it counted
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        publish_split_passes(a, q, h, p);
+      }
     }
   }
 }
