@@ -867,7 +867,13 @@ tc_status tc_renderer_get_building(tc_renderer* renderer, tc_scene_building* out
 /* Uploads frame constants, resets SceneBuilding / Readback, fills the split
  * list with ~0 (renderer_raytrace_clusters_tess.cpp:412-419). */
 tc_status tc_renderer_begin_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
-/* begin_frame, instances_classify, clusters_cull and tc_frame_tessellate. */
+/* begin_frame, instances_classify, clusters_cull and tc_frame_tessellate.
+ * In ray-tracing mode clusters_cull lists every cluster of every instance,
+ * so over the renderer's own instances that list is built once, at
+ * creation, in a buffer of its own: the frame's SceneBuilding then names it
+ * (visibleClusters, visibleClusterCounter) and the culling launches are
+ * skipped.  Over instances bound with tc_renderer_rebind_instances the frame
+ * culls into the buffer tc_renderer_get_building names. */
 tc_status tc_renderer_tessellate(tc_renderer* renderer, const tc_frame_constants* frameHost, tc_stream stream);
 
 /* Accel arrays of the renderer and its per-instance transforms (device). */

@@ -29,6 +29,13 @@ struct Renderer {
   uint64_t workspaceBytes = 0;
   tc_vertex_tape* tape = nullptr;  // with TC_FLAG_VERTEX_TAPE
   tc_scene_building hostBuild{};
+  // The visible-cluster list of the renderer's own instances.  In
+  // ray-tracing mode culling appends every cluster of every instance
+  // (clusters_cull.comp.glsl:112-161), a function of the instance records
+  // alone, so it is built once at creation; frames over these instances
+  // start from it and skip the culling launches.
+  uint64_t clusterList = 0;      // device tc_cluster_info[maxVisibleClusters]
+  uint32_t listedClusters = 0;   // its visibleClusterCounter (unclamped)
   // Frame constants travel through a ring of pinned slots: the host fills
   // slot hostFrame % kViewSlots, k_frame_start (in the stream or the
   // graph) reads the slot named by a device counter that advances once per
@@ -84,18 +91,33 @@ static bool invert4(const double m[16], double out[16]) {
   return true;
 }
 
+// Word w of the frame's initial SceneBuilding: the template's, with the
+// eye position of the frame and, when `list` is set, the renderer's
+// prebuilt cluster list (its address and count) in place of an empty one.
+__device__ __forceinline__ uint32_t building_word(const tc_scene_building* tmpl, uint32_t w, const float* eye,
+                                                  uint64_t list, uint32_t listCount) {
+  constexpr uint32_t kList = offsetof(tc_scene_building, visibleClusters) / 4;
+  constexpr uint32_t kCount = offsetof(tc_scene_building, visibleClusterCounter) / 4;
+  static_assert(offsetof(tc_scene_building, viewPos) == 0, "the eye position leads the building record");
+  uint32_t v = reinterpret_cast<const uint32_t*>(tmpl)[w];
+  if (w < 3) v = __float_as_uint(eye[w]);
+  if (list) {
+    if (w == kList) v = (uint32_t)list;
+    if (w == kList + 1) v = (uint32_t)(list >> 32);
+    if (w == kCount) v = listCount;
+  }
+  return v;
+}
+
 // Resets the per-frame state: SceneBuilding from its template (with the
 // eye position of the frame), Readback to zero and the split list to ~0
 // (renderer_raytrace_clusters_tess.cpp:412-419).
 __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building* tmpl, const tc_frame_constants* view,
-                              tc_readback* rb, uint32_t* split, uint64_t splitWords) {
+                              tc_readback* rb, uint32_t* split, uint64_t splitWords, uint64_t list, uint32_t listCount) {
   const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  if (gid < sizeof(tc_scene_building) / 4) {
-    uint32_t v = reinterpret_cast<const uint32_t*>(tmpl)[gid];
-    if (gid < 3) v = __float_as_uint(view->viewPos[gid]);
-    reinterpret_cast<uint32_t*>(build)[gid] = v;
-  }
+  if (gid < sizeof(tc_scene_building) / 4)
+    reinterpret_cast<uint32_t*>(build)[gid] = building_word(tmpl, (uint32_t)gid, view->viewPos, list, listCount);
   for (uint64_t i = gid; i < sizeof(tc_readback) / 4; i += stride) reinterpret_cast<uint32_t*>(rb)[i] = 0;
   for (uint64_t i = gid; i < splitWords; i += stride) split[i] = ~0u;
 }
@@ -108,7 +130,7 @@ __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building*
 // with ~0.  No workgroup waits on another.
 __global__ void k_frame_start(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view,
                               tc_frame_constants* viewLast, tc_scene_building* build, const tc_scene_building* tmpl,
-                              tc_readback* rb, uint32_t* split, uint64_t splitWords) {
+                              tc_readback* rb, uint32_t* split, uint64_t splitWords, uint64_t list, uint32_t listCount) {
   __shared__ uint32_t s_frame;
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) s_frame = counter[0];
@@ -123,11 +145,8 @@ __global__ void k_frame_start(const tc_frame_constants* ring, uint32_t* counter,
       last[i] = frame ? dst[i] : v;
       dst[i] = v;
     }
-    if (threadIdx.x < sizeof(tc_scene_building) / 4) {
-      uint32_t v = reinterpret_cast<const uint32_t*>(tmpl)[threadIdx.x];
-      if (threadIdx.x < 3) v = __float_as_uint(src->viewPos[threadIdx.x]);
-      reinterpret_cast<uint32_t*>(build)[threadIdx.x] = v;
-    }
+    if (threadIdx.x < sizeof(tc_scene_building) / 4)
+      reinterpret_cast<uint32_t*>(build)[threadIdx.x] = building_word(tmpl, threadIdx.x, src->viewPos, list, listCount);
     __syncthreads();
     if (threadIdx.x == 0) counter[0] = frame + 1;
   }
@@ -245,7 +264,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oViewLast, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter;
+  uint64_t oView, oViewLast, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter, oList;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
       {&oViewLast, sizeof(tc_frame_constants)},
@@ -277,6 +296,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances)},
       {&oTape, (c.flags & TC_FLAG_VERTEX_TAPE) ? sizeof(tc_vertex_tape) * (uint64_t)c.maxGeneratedVertices : 0},
       {&oCounter, 256},
+      {&oList, sizeof(tc_cluster_info) * (uint64_t)c.maxVisibleClusters},
   };
   uint64_t o = 0;
   for (auto& s : slots) {
@@ -339,6 +359,20 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
     tc_renderer_get_bindings(reinterpret_cast<tc_renderer*>(r), &b);
     s = tc_workspace_init(&b, config, nullptr);
     if (s == TC_OK && hipDeviceSynchronize() != hipSuccess) s = TC_ERROR_LAUNCH;
+  }
+  if (s == TC_OK) {
+    // the own instances' cluster list, by the culling kernels into the
+    // list's own buffer (the building record is reset at every frame start)
+    tc_scene_building L = B;
+    L.visibleClusters = base + oList;
+    L.visibleClusterCounter = 0;
+    tc_bindings b;
+    tc_renderer_get_bindings(reinterpret_cast<tc_renderer*>(r), &b);
+    if (hipMemcpy(r->build, &L, sizeof(L), hipMemcpyHostToDevice) != hipSuccess) s = TC_ERROR_LAUNCH;
+    if (s == TC_OK) s = tc_clusters_cull(&b, *config, nullptr);
+    if (s == TC_OK && hipMemcpy(&r->listedClusters, &r->build->visibleClusterCounter, 4, hipMemcpyDeviceToHost) != hipSuccess)
+      s = TC_ERROR_LAUNCH;
+    r->clusterList = base + oList;
   }
   if (s != TC_OK) {
     tc_renderer_destroy(reinterpret_cast<tc_renderer*>(r));
@@ -503,7 +537,9 @@ tc_status tc_renderer_get_building(tc_renderer* rr, tc_scene_building* out) {
 // Frame start: frame constants upload (from pinned host memory, unless
 // fetchView is false: the frame keeps the constants already fetched) and
 // per-frame state reset.
-static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, hipStream_t st, bool fetchView) {
+static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, hipStream_t st, bool fetchView,
+                             bool listed) {
+  const uint64_t list = listed ? r->clusterList : 0;
   if (fetchView) {
     // during graph capture only the fetch is recorded; each replay fills
     // its slot in tc_renderer_graph_launch
@@ -512,24 +548,31 @@ static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, h
     if (!r->capturing && !fill_slot(r, slot, frameHost)) return TC_ERROR_LAUNCH;
     hipLaunchKernelGGL(k_frame_start, dim3(256), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter,
                        r->view, r->viewLast, r->build, (const tc_scene_building*)r->buildTemplate, r->readback,
-                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4);
+                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4, list,
+                       r->listedClusters);
     if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
   } else {
     hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,
                        (const tc_frame_constants*)r->view, r->readback,
-                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4);
+                       reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4, list,
+                       r->listedClusters);
   }
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
 }
 
+// The frame's start and its tessellation: instances_classify through the
+// template instantiation (renderer_raytrace_clusters_tess.cpp:433-582).
+// Over the renderer's own instances the frame starts from their prebuilt
+// cluster list; over instance records bound in their place it culls.
 static tc_status tessellate(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream, bool fetchView) {
   Renderer* r = R(rr);
   tc_status s;
-  if ((s = begin_frame(r, frameHost, (hipStream_t)stream, fetchView)) != TC_OK) return s;
+  const bool listed = r->clusterList && r->instances == r->ownInstances;
+  if ((s = begin_frame(r, frameHost, (hipStream_t)stream, fetchView, listed)) != TC_OK) return s;
   tc_bindings b;
   tc_renderer_get_bindings(rr, &b);
   if ((s = tc_instances_classify(&b, r->cfg, stream)) != TC_OK) return s;
-  if ((s = tc_clusters_cull(&b, r->cfg, stream)) != TC_OK) return s;
+  if (!listed && (s = tc_clusters_cull(&b, r->cfg, stream)) != TC_OK) return s;
   return tc_frame_tessellate(&b, r->cfg, 0, stream);
 }
 
@@ -537,7 +580,7 @@ extern "C" {
 
 tc_status tc_renderer_begin_frame(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
   if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
-  return begin_frame(R(rr), frameHost, (hipStream_t)stream, true);
+  return begin_frame(R(rr), frameHost, (hipStream_t)stream, true, false);
 }
 
 // Captures the per-mesh form of a multi-mesh frame into the renderer's
@@ -579,20 +622,11 @@ tc_status tc_renderer_graph_capture_meshwise(tc_renderer* rr, const tc_render_in
   return TC_OK;
 }
 
-// The frame's start and its tessellation: instances_classify through the
-// template instantiation (renderer_raytrace_clusters_tess.cpp:433-582);
-// tc_renderer_build_accel continues from the CLAS builds.
+// The frame's start and its tessellation; tc_renderer_build_accel
+// continues from the CLAS builds.
 tc_status tc_renderer_tessellate(tc_renderer* rr, const tc_frame_constants* frameHost, tc_stream stream) {
-  if (!rr) return TC_ERROR_INVALID_ARGUMENT;
-  Renderer* r = R(rr);
-  tc_status s;
-  if ((s = tc_renderer_begin_frame(rr, frameHost, stream)) != TC_OK) return s;
-  tc_bindings b;
-  tc_renderer_get_bindings(rr, &b);
-  if ((s = tc_instances_classify(&b, r->cfg, stream)) != TC_OK) return s;
-  if ((s = tc_clusters_cull(&b, r->cfg, stream)) != TC_OK) return s;
-  if ((s = tc_frame_tessellate(&b, r->cfg, 0, stream)) != TC_OK) return s;
-  return TC_OK;
+  if (!rr || !frameHost) return TC_ERROR_INVALID_ARGUMENT;
+  return tessellate(rr, frameHost, stream, true);
 }
 
 }  // extern "C"
