@@ -580,7 +580,11 @@ tc_status tc_blas_clusters_insert(const tc_bindings* b, tc_config config, int tr
     hipLaunchKernelGGL(k_insert_seg_start, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res], segStart,
                        sb.capacity, tr);
   }
-  hipLaunchKernelGGL(k_insert_scatter, dim3(grid), dim3(256), 0, st, a, (const uint64_t*)sb.keys[res],
+  // 1024-thread workgroups, at most one per CU: each ends with one
+  // device-scope atomic on the readback's size total, and 1024 workgroups'
+  // atomics on that one word took 9 of the launch's 17 us at grid64
+  const uint32_t sgrid = std::min<uint32_t>((sb.capacity + 1023) / 1024, (uint32_t)num_cu());
+  hipLaunchKernelGGL(k_insert_scatter, dim3(sgrid), dim3(1024), 0, st, a, (const uint64_t*)sb.keys[res],
                      (const uint32_t*)sb.vals[res], starts, sb.capacity, tr);
   return launched();
 }
