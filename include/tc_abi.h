@@ -881,6 +881,11 @@ tc_status tc_renderer_get_accel(tc_renderer* renderer, tc_accel* out, const floa
 /* tc_clas_instantiate, tc_blas_setup_insertion, tc_blas_clusters_insert,
  * tc_accel_build over the renderer's buffers. */
 tc_status tc_renderer_build_accel(tc_renderer* renderer, tc_stream stream);
+/* tc_trace_rays over the renderer's accel arrays.  Camera traces take their
+ * 8 x 8 tile rows in the order the renderer's last frame start ranked from
+ * the previous frame's traces of the same height (rows with the longest
+ * waves first); hits do not depend on it.  A renderer's traces share that
+ * state and must not overlap. */
 tc_status tc_renderer_trace(tc_renderer* renderer, const tc_trace_params* params, tc_stream stream);
 /* tessellate + build_accel + trace (params may be NULL). */
 tc_status tc_renderer_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, const tc_trace_params* params,

@@ -198,6 +198,48 @@ def test_trace_parity_transient_2x():
     print("hit agreement", agree)
 
 
+def test_tile_row_order_keeps_hits():
+    """A renderer's camera traces time their tile rows, the next frame start
+    ranks the rows (longest wave first), and that frame's traces of the same
+    height take the rows in that order.  Hits and decoded records do not
+    depend on the order: ordered traces, traces after a frame that traced
+    another height (tile order again), and the 128- and 256-thread workgroup
+    forms all equal the first trace byte for byte."""
+    import ctypes as C
+    host = small_scene()
+    cfg = harness.small_config()
+    flags = abi.TRACE_CULL_BACK | abi.TRACE_SHADOW
+    W, H = 160, 90
+    L = abi.lib()
+    g = harness.GpuFrame(host, cfg)
+
+    def frame_and_trace(w, h):
+        g.run(host.frame(), stages="accel")
+        return g.trace(w, h, flags)
+
+    try:
+        first = frame_and_trace(W, H)
+        later = [frame_and_trace(W, H), frame_and_trace(W, H)]
+        small = frame_and_trace(96, 54)
+        small2 = frame_and_trace(96, 54)
+        later += [frame_and_trace(W, H), frame_and_trace(W, H)]
+        for threads in (128, 256):
+            t = abi.Tuning()
+            abi.check(L.tc_get_tuning(C.byref(t)))
+            t.traceThreads = threads
+            abi.check(L.tc_set_tuning(C.byref(t)))
+            later += [frame_and_trace(W, H), frame_and_trace(W, H)]
+    finally:
+        abi.check(L.tc_set_tuning(None))
+        g.close()
+    assert (first[0]["t"] > 0).mean() > 0.3, "camera must see geometry"
+    for h, d in later:
+        assert np.array_equal(h.view(np.uint8), first[0].view(np.uint8))
+        assert np.array_equal(d.view(np.uint8), first[1].view(np.uint8))
+    assert np.array_equal(small2[0].view(np.uint8), small[0].view(np.uint8))
+    assert np.array_equal(small2[1].view(np.uint8), small[1].view(np.uint8))
+
+
 def test_packet_traversal_matches_per_ray():
     """Packet traversal gives the per-ray traversal's hits byte for byte,
     for camera rays (ragged tiles) and explicit rays in any directions."""

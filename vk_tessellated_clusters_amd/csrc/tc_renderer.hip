@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "tc_kernels.h"
+#include "tc_trace.h"
 
 namespace tc {
 
@@ -36,6 +37,7 @@ struct Renderer {
   // start from it and skip the culling launches.
   uint64_t clusterList = 0;      // device tc_cluster_info[maxVisibleClusters]
   uint32_t listedClusters = 0;   // its visibleClusterCounter (unclamped)
+  tc::TraceOrder* traceOrder = nullptr;  // tile-row order of the camera traces (device)
   // Frame constants travel through a ring of pinned slots: the host fills
   // slot hostFrame % kViewSlots, k_frame_start (in the stream or the
   // graph) reads the slot named by a device counter that advances once per
@@ -126,12 +128,17 @@ __global__ void k_frame_reset(tc_scene_building* build, const tc_scene_building*
 // device frame counter names into the frame constants (the constants they
 // replace become viewLast; the first frame's viewLast is its own view),
 // advances the counter and resets SceneBuilding, taking the eye position
-// from that slot; every workgroup clears Readback and fills the split list
-// with ~0.  No workgroup waits on another.
+// from that slot; workgroup 1 ranks the tile rows of the traces timed since
+// the last frame start (the order this frame's camera traces take); every
+// workgroup clears Readback and fills the split list with ~0.  No workgroup
+// waits on another.
 __global__ void k_frame_start(const tc_frame_constants* ring, uint32_t* counter, tc_frame_constants* view,
                               tc_frame_constants* viewLast, tc_scene_building* build, const tc_scene_building* tmpl,
-                              tc_readback* rb, uint32_t* split, uint64_t splitWords, uint64_t list, uint32_t listCount) {
+                              tc_readback* rb, uint32_t* split, uint64_t splitWords, uint64_t list, uint32_t listCount,
+                              tc::TraceOrder* order) {
   __shared__ uint32_t s_frame;
+  __shared__ uint32_t s_rows[tc::kMaxOrderRows];
+  if (blockIdx.x == 1) tc::rank_tile_rows(order, s_rows);
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) s_frame = counter[0];
     __syncthreads();
@@ -264,7 +271,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   tc_scene_building& B = r->hostBuild;
   memset(&B, 0, sizeof(B));
   B.numRenderInstances = numInstances;
-  uint64_t oView, oViewLast, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter, oList;
+  uint64_t oView, oViewLast, oRb, oInst, oTex, oBuild, oTmpl, oWs, oXf, oAccel, oTape, oCounter, oList, oOrder;
   std::vector<Slot> slots = {
       {&oView, sizeof(tc_frame_constants)},
       {&oViewLast, sizeof(tc_frame_constants)},
@@ -297,6 +304,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       {&oTape, (c.flags & TC_FLAG_VERTEX_TAPE) ? sizeof(tc_vertex_tape) * (uint64_t)c.maxGeneratedVertices : 0},
       {&oCounter, 256},
       {&oList, sizeof(tc_cluster_info) * (uint64_t)c.maxVisibleClusters},
+      {&oOrder, sizeof(tc::TraceOrder)},
   };
   uint64_t o = 0;
   for (auto& s : slots) {
@@ -322,6 +330,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
   r->xforms = reinterpret_cast<float*>(mem + oXf);
   if (c.flags & TC_FLAG_VERTEX_TAPE) r->tape = reinterpret_cast<tc_vertex_tape*>(mem + oTape);
   r->frameCounter = reinterpret_cast<uint32_t*>(mem + oCounter);
+  r->traceOrder = reinterpret_cast<tc::TraceOrder*>(mem + oOrder);
   r->ownInstances = r->instances;
   r->ownXforms = r->xforms;
   tc_accel_bind(mem + oAccel, tc_accel_size(c.maxGeneratedClusters, numInstances), c.maxGeneratedClusters, numInstances,
@@ -346,6 +355,7 @@ tc_status tc_renderer_create(const tc_render_instance* instancesHost, uint32_t n
       hipMemcpy(r->buildTemplate, &B, sizeof(B), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(r->xforms, xf.data(), xf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(r->arena, 0, oInst) != hipSuccess || hipMemset(r->frameCounter, 0, 256) != hipSuccess ||
+      hipMemset(r->traceOrder, 0, sizeof(tc::TraceOrder)) != hipSuccess ||
       hipMemset(reinterpret_cast<void*>(B.blasBuildSizes), 0, 4ull * numInstances) != hipSuccess)
     s = TC_ERROR_LAUNCH;
   if (s == TC_OK) s = tc_table_upload(&r->table);
@@ -447,7 +457,9 @@ tc_status tc_renderer_trace(tc_renderer* rr, const tc_trace_params* params, tc_s
   Renderer* r = R(rr);
   tc_bindings b;
   tc_renderer_get_bindings(rr, &b);
-  tc_status s = tc_trace_rays(&b, r->cfg, &r->accel, params, stream);
+  // camera traces take their tile rows longest first (the order ranked at
+  // the frame start from the previous frame's traces of the same height)
+  tc_status s = tc::trace_rays(&b, r->cfg, &r->accel, params, r->traceOrder, stream);
   if (s != TC_OK) return s;
   // resources.cpp:503-512: the HiZ of this frame's depth, for the next
   // frame's instance culling (camera rays of the HiZ's viewport only)
@@ -549,7 +561,7 @@ static tc_status begin_frame(Renderer* r, const tc_frame_constants* frameHost, h
     hipLaunchKernelGGL(k_frame_start, dim3(256), dim3(256), 0, st, (const tc_frame_constants*)r->pinnedRing, r->frameCounter,
                        r->view, r->viewLast, r->build, (const tc_scene_building*)r->buildTemplate, r->readback,
                        reinterpret_cast<uint32_t*>(r->hostBuild.splitTriangles), r->splitBytes / 4, list,
-                       r->listedClusters);
+                       r->listedClusters, r->traceOrder);
     if (!r->capturing && !queued_slot(r, slot, st)) return TC_ERROR_LAUNCH;
   } else {
     hipLaunchKernelGGL(k_frame_reset, dim3(256), dim3(256), 0, st, r->build, (const tc_scene_building*)r->buildTemplate,

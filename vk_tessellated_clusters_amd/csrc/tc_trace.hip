@@ -538,24 +538,35 @@ __device__ __forceinline__ void normalize4(float v[4]) {
 // (w / wavesPerGroup) % 8.  The image is cut into eight vertical strips of
 // tiles and the waves of XCD x walk strip x in row order, so each XCD's L2
 // holds the geometry seen through one strip.  Strips differ in width by at
-// most one tile; a wave past the end of its strip has no tile (false).
+// most one tile; a wave past the end of its strip has no tile (false).  With
+// `rowOrder` the strip's rows are walked in that order instead (a
+// permutation of the tile rows, TraceOrder).
 constexpr uint32_t kXcds = 8;
+
+// A wave's tile time into its row's maximum (no return value: the wave does
+// not wait for it).
+__device__ __forceinline__ void time_tile_row(TraceOrder* ord, uint32_t row, uint64_t t0) {
+  const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t0;
+  __hip_atomic_fetch_max(&ord->rowTime[row], (uint32_t)min(dt, (uint64_t)0xFFFFFFFFu), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __host__ __device__ __forceinline__ uint32_t strip_waves(uint32_t tilesX, uint32_t tilesY, uint32_t wavesPerGroup) {
   const uint32_t perXcd = (tilesX + kXcds - 1) / kXcds * tilesY;
   return kXcds * ((perXcd + wavesPerGroup - 1) / wavesPerGroup) * wavesPerGroup;
 }
 
-__device__ __forceinline__ bool strip_tile(uint32_t w, uint32_t wavesPerGroup, uint32_t tilesX, uint32_t tilesY, uint32_t& tx,
-                                           uint32_t& ty) {
+__device__ __forceinline__ bool strip_tile(uint32_t w, uint32_t wavesPerGroup, uint32_t tilesX, uint32_t tilesY,
+                                           const uint32_t* rowOrder, uint32_t& tx, uint32_t& ty) {
   const uint32_t b = w / wavesPerGroup, x = b % kXcds;
   const uint32_t k = (b / kXcds) * wavesPerGroup + w % wavesPerGroup;
   const uint32_t c0 = x * tilesX / kXcds, sw = (x + 1) * tilesX / kXcds - c0;
   if (sw == 0 || k / sw >= tilesY) return false;
   tx = c0 + k % sw;
-  ty = k / sw;
+  ty = rowOrder ? rowOrder[uniform(k / sw)] : k / sw;
   return true;
 }
+
 
 // render_raytrace.rgen.glsl:52-86
 __device__ __forceinline__ void primary_ray(const tc_frame_constants* view, uint32_t x, uint32_t y, uint32_t W, uint32_t H,
@@ -673,7 +684,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
   const bool cull = (a.flags & TC_TRACE_CULL_BACK) != 0;
   const bool shadows = (a.flags & TC_TRACE_SHADOW) != 0 && a.decoded;
   const uint32_t tilesX = (a.width + 7) / 8, tilesY = (a.height + 7) / 8;
+  // tile-row order: the last ranking when it ranked this many rows.  Each
+  // wave's tile time goes to its row, timed from the top of one iteration
+  // to the top of the next (where the wave has reconverged), by an atomic
+  // nobody waits for.
+  TraceOrder* ord = !a.rays && a.order && tilesY <= kMaxOrderRows ? a.order : nullptr;
+  const uint32_t* rowOrder = ord && ord->rows == tilesY ? ord->order : nullptr;
+  if (ord && blockIdx.x == 0 && threadIdx.x == 0) ord->pendingRows = tilesY;
+  constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+  uint32_t timedRow = kNoRow;
+  uint64_t t0 = 0;
   for (uint32_t item = blockIdx.x * T + threadIdx.x; item < a.numItems; item += gridDim.x * T) {
+    if (ord) {
+      if (timedRow != kNoRow && lane_id() == 0)
+        time_tile_row(ord, timedRow, t0);
+      timedRow = kNoRow;
+      t0 = __builtin_amdgcn_s_memrealtime();
+    }
     float o[3], d[3], tmin, tmax;
     uint32_t r = item;
     if (a.rays) {
@@ -683,7 +710,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
       tmax = ray.tMax;
     } else {
       uint32_t tx, ty;
-      if (!strip_tile(item >> 6, T / kWave, tilesX, tilesY, tx, ty)) continue;
+      if (!strip_tile(item >> 6, T / kWave, tilesX, tilesY, rowOrder, tx, ty)) continue;
+      timedRow = ty;
       const uint32_t in = item & 63, x = tx * 8 + (in & 7), y = ty * 8 + (in >> 3);
       if (x >= a.width || y >= a.height) continue;
       r = y * a.width + x;
@@ -716,6 +744,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
     }
     a.decoded[r] = dec;
   }
+  if (ord && timedRow != kNoRow && lane_id() == 0) time_tile_row(ord, timedRow, t0);
 }
 
 // Packet form of k_trace: one wave per 64 work items (an 8 x 8 pixel tile
@@ -742,7 +771,7 @@ __global__ __launch_bounds__(T) void k_trace_packet(TraceArgs a) {
         tmax = ray.tMax;
       } else {
         uint32_t tx = 0, ty = 0;
-        const bool tiled = strip_tile(item >> 6, T / kWave, tilesX, tilesY, tx, ty);
+        const bool tiled = strip_tile(item >> 6, T / kWave, tilesX, tilesY, nullptr, tx, ty);
         const uint32_t in = item & 63, x = tx * 8 + (in & 7), y = ty * 8 + (in >> 3);
         if (tiled && x < a.width && y < a.height) {
           r = y * a.width + x;
@@ -788,8 +817,8 @@ __global__ __launch_bounds__(T) void k_trace_packet(TraceArgs a) {
 
 using namespace tc;
 
-extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* p,
-                                   tc_stream stream) {
+tc_status tc::trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* p,
+                         TraceOrder* order, tc_stream stream) {
   KArgs ka;
   tc_status s = make_args_public(b, config, ka);
   if (s != TC_OK) return s;
@@ -814,6 +843,7 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   a.hits = p->hits;
   a.decoded = p->decoded;
   a.hitVertices = p->hitVertices;
+  a.order = order;
   a.flags = p->flags;
   if (!p->rays && (!p->width || !p->height)) return TC_ERROR_INVALID_ARGUMENT;
   if (a.numRays == 0) return TC_OK;
@@ -837,4 +867,9 @@ extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const
   else
     hipLaunchKernelGGL(k_trace<128>, dim3(grid), dim3(128), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? TC_OK : TC_ERROR_LAUNCH;
+}
+
+extern "C" tc_status tc_trace_rays(const tc_bindings* b, tc_config config, const tc_accel* accel, const tc_trace_params* p,
+                                   tc_stream stream) {
+  return trace_rays(b, config, accel, p, nullptr, stream);
 }
