@@ -11,8 +11,12 @@ namespace tc {
 // tile and the longest time of each tile row is kept; the renderer's next
 // frame start ranks the rows by it, longest first, and the traces of that
 // frame with the same height take the rows in that order, so the costly
-// rows start first and the cheap ones fill the end.  Results do not depend
-// on the order.
+// rows start first and the cheap ones fill the end.  That tail is a small
+// part of a launch whose waves run many rounds per resident wave slot, and
+// there the timing atomics cost more than the order saves: launches of more
+// than kOrderRounds waves per slot are neither timed nor ordered.  Results
+// do not depend on the order.
+constexpr uint32_t kOrderRounds = 16;
 constexpr uint32_t kMaxOrderRows = 1536;
 struct TraceOrder {
   uint32_t rows;         // tile rows `order` ranks (0: none yet)
@@ -22,13 +26,19 @@ struct TraceOrder {
   uint32_t order[kMaxOrderRows];    // dispatch position -> tile row
 };
 
-// Ranks the timed rows longest first (ties by row) into `order` and clears
-// the times; no-op when nothing was timed.  One workgroup; `lds` holds
+// Ranks the timed rows into `order` and clears the times; no-op when
+// nothing was timed.  Rows rank by half-octave classes of their longest
+// wave, longest first, and in tile order within a class: rows of like cost
+// keep the row-to-row cache locality of tile order, and only rows clearly
+// costlier than the rest move ahead.  One workgroup; `lds` holds
 // kMaxOrderRows words.
+__device__ __forceinline__ uint32_t row_time_class(uint32_t ticks) {
+  return __float_as_uint((float)ticks + 1.0f) >> 22;  // exponent and first mantissa bit
+}
 __device__ __forceinline__ void rank_tile_rows(TraceOrder* ord, uint32_t* lds) {
   const uint32_t rows = ord->pendingRows;
   if (rows == 0 || rows > kMaxOrderRows) return;
-  for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) lds[r] = ord->rowTime[r];
+  for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) lds[r] = row_time_class(ord->rowTime[r]);
   __syncthreads();
   for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) {
     const uint32_t c = lds[r];

@@ -851,6 +851,8 @@ tc_status tc::trace_rays(const tc_bindings* b, tc_config config, const tc_accel*
   const uint32_t T = tn.traceThreads;
   a.numItems = p->rays ? p->numRays : strip_waves((p->width + 7) / 8, (p->height + 7) / 8, T / kWave) * kWave;
   const uint32_t grid = std::min<uint32_t>((a.numItems + T - 1) / T, (uint32_t)num_cu_public() * tn.traceBlocksPerCU);
+  // resident waves: four per SIMD (128 VGPRs), four SIMDs per CU
+  if ((uint64_t)grid * (T / kWave) > (uint64_t)kOrderRounds * num_cu_public() * 16u) a.order = nullptr;
   if (a.flags & TC_TRACE_PACKETS) {
     if (T == 64)
       hipLaunchKernelGGL(k_trace_packet<64>, dim3(grid), dim3(64), 0, (hipStream_t)stream, a);
