@@ -26,19 +26,13 @@ struct TraceOrder {
   uint32_t order[kMaxOrderRows];    // dispatch position -> tile row
 };
 
-// Ranks the timed rows into `order` and clears the times; no-op when
-// nothing was timed.  Rows rank by half-octave classes of their longest
-// wave, longest first, and in tile order within a class: rows of like cost
-// keep the row-to-row cache locality of tile order, and only rows clearly
-// costlier than the rest move ahead.  One workgroup; `lds` holds
+// Ranks the timed rows longest first (ties by row) into `order` and clears
+// the times; no-op when nothing was timed.  One workgroup; `lds` holds
 // kMaxOrderRows words.
-__device__ __forceinline__ uint32_t row_time_class(uint32_t ticks) {
-  return __float_as_uint((float)ticks + 1.0f) >> 22;  // exponent and first mantissa bit
-}
 __device__ __forceinline__ void rank_tile_rows(TraceOrder* ord, uint32_t* lds) {
   const uint32_t rows = ord->pendingRows;
   if (rows == 0 || rows > kMaxOrderRows) return;
-  for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) lds[r] = row_time_class(ord->rowTime[r]);
+  for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) lds[r] = ord->rowTime[r];
   __syncthreads();
   for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) {
     const uint32_t c = lds[r];
