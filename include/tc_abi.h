@@ -884,8 +884,9 @@ tc_status tc_renderer_build_accel(tc_renderer* renderer, tc_stream stream);
 /* tc_trace_rays over the renderer's accel arrays.  Camera traces take their
  * 8 x 8 tile rows in the order the renderer's last frame start ranked from
  * the previous frame's traces of the same height (rows with the longest
- * waves first); hits do not depend on it.  A renderer's traces share that
- * state and must not overlap. */
+ * waves first); hits do not depend on it.  Launches of more than 16 waves
+ * per resident wave slot (e.g. 3840 x 2160 on 256 CUs) keep tile order.  A
+ * renderer's traces share that state and must not overlap. */
 tc_status tc_renderer_trace(tc_renderer* renderer, const tc_trace_params* params, tc_stream stream);
 /* tessellate + build_accel + trace (params may be NULL). */
 tc_status tc_renderer_frame(tc_renderer* renderer, const tc_frame_constants* frameHost, const tc_trace_params* params,
